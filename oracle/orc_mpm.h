@@ -1,0 +1,71 @@
+/*
+ * CPU oracle for the MLS-MPM substep -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Restates MPMSolver.p2g_bls / grid_op / g2p_bls / compute_cauchy /
+ * compute_ek / update_limits_* of the reference
+ * (mpm_exps/engine/mpm_solver.py:578-758, 960-1052) and
+ * polar_decompose2d (TS/python/taichi/lang/linalg_impl.py:9-26).
+ *
+ * State layout: component-major SoA, state[c * n + p], components in the
+ * reference's bit_pack_assign order (mpm_solver.py:700-744):
+ *   x[dim], v[dim], C[dim*dim] (row major), then J (fluid) or F[dim*dim].
+ * Grid: dense n_grid^dim nodes, grid_m[node], grid_v[d * nodes + node],
+ * node = (i * n + j) * n + k.
+ */
+#ifndef AQ_ORACLE_MPM_H
+#define AQ_ORACLE_MPM_H
+#include <stdint.h>
+#include "orc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_FLUID = 0, ORC_ELASTIC = 1 };
+enum { ORC_SEPARATING = 0, ORC_STICKY = 1 };
+
+typedef struct orc_mpm {
+  int32_t dim;       /* 2 or 3 */
+  int32_t material;  /* ORC_FLUID (J, sigma = la (J-1) I) or ORC_ELASTIC
+                        (fixed-corotated, 2D only, sigma = 2 mu (F-R) F^T + la J (J-1) I) */
+  int32_t n_grid;
+  int32_t boundary;  /* ORC_SEPARATING (mpm_solver.py:659-664) or ORC_STICKY */
+  int32_t bound;     /* wall thickness in cells: lb = bound, ub = n_grid - bound */
+  int32_t pad;
+  double dx, inv_dx, dt, p_vol, p_mass, gravity;
+  double la;         /* fluid bulk modulus (MPMSolver.la) */
+  double mu_0, la_0; /* elastic Lame parameters */
+} orc_mpm;
+
+int orc_num_comps(const orc_mpm *s);
+
+/* one phase each; grid arrays are caller-owned, p2g accumulates (caller zeroes) */
+void orc_p2g_f32(const orc_mpm *s, const float *state, int64_t n, float *grid_m, float *grid_v);
+void orc_grid_op_f32(const orc_mpm *s, const float *grid_m, float *grid_v);
+void orc_g2p_f32(const orc_mpm *s, const float *state, const float *grid_v, float *out, int64_t n);
+void orc_p2g_f64(const orc_mpm *s, const double *state, int64_t n, double *grid_m, double *grid_v);
+void orc_grid_op_f64(const orc_mpm *s, const double *grid_m, double *grid_v);
+void orc_g2p_f64(const orc_mpm *s, const double *state, const double *grid_v, double *out, int64_t n);
+
+/* whole substeps on plain (unquantized) storage */
+void orc_run_plain_f32(const orc_mpm *s, float *state, int64_t n, int nsteps,
+                       float *grid_m, float *grid_v, int nthreads);
+void orc_run_plain_f64(const orc_mpm *s, double *state, int64_t n, int nsteps,
+                       double *grid_m, double *grid_v, int nthreads);
+/* whole substeps on quantized storage (f32 compute): decode, substep, encode.
+   Dithering noise of substep t is orc_noise_u(seed, step0 + t, p, c). */
+void orc_run_quant_f32(const orc_mpm *s, const orc_comp *comps, int nwords,
+                       uint64_t *words, int64_t n, int nsteps, uint64_t seed,
+                       uint32_t step0, float *grid_m, float *grid_v,
+                       float *scratch /* 2 * nc * n floats */, int nthreads);
+
+/* compute_ek (mpm_solver.py:1043-1052), accumulated in double */
+double orc_energy_f32(const orc_mpm *s, const float *state, int64_t n);
+double orc_energy_f64(const orc_mpm *s, const double *state, int64_t n);
+/* update_limits_* (mpm_solver.py:960-979): per-component max and min */
+void orc_limits_f32(const orc_mpm *s, const float *state, int64_t n, float *mx, float *mn);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,245 @@
+/*
+ * Body of the oracle MPM phases, included by mpm_ref.c once per compute
+ * type with REAL (float/double) and SFX (f32/f64) defined.  TEST
+ * INFRASTRUCTURE ONLY.
+ *
+ * Expressions keep the reference's operand order so that each rounding
+ * step is the reference's (unfused):
+ *   base/fx/w            mpm_solver.py:599-606, 676-678
+ *   stress/affine        mpm_solver.py:610-613, compute_cauchy :1012-1031
+ *   P2G scatter          mpm_solver.py:615-622
+ *   grid update          mpm_solver.py:633-665 (friction off)
+ *   G2P gather + update  mpm_solver.py:682-694, 702, 750-758
+ * Deviation: the stencil base is clamped to [0, n_grid-3] so a particle that
+ * left the domain cannot index outside the dense grid (the reference's
+ * sparse grid spans 4096 cells and would accept it).
+ */
+#define CAT_(a, b) a##_##b
+#define CAT(a, b) CAT_(a, b)
+#define FN(name) CAT(name, SFX)
+
+static int FN(base_of)(REAL xs, REAL inv_dx, int n) {
+  REAL t = xs * inv_dx;               /* x * inv_dx */
+  t = t - (REAL)0.5;                  /* - 0.5 */
+  int b = (int)FLOOR(t);              /* floor(...).cast(int) */
+  if (t != t) b = 0;
+  if (b < 0) b = 0;
+  if (b > n - 3) b = n - 3;
+  return b;
+}
+
+static void FN(weights)(REAL fx, REAL w[3]) {
+  REAL a = (REAL)1.5 - fx;
+  w[0] = (REAL)0.5 * (a * a);
+  REAL b = fx - (REAL)1;
+  w[1] = (REAL)0.75 - b * b;
+  REAL c = fx - (REAL)0.5;
+  w[2] = (REAL)0.5 * (c * c);
+}
+
+/* compute_cauchy (mpm_solver.py:1012-1031) times the stress coefficient,
+   plus p_mass * C (:610-613).  aff is row major dim x dim. */
+static void FN(affine)(const orc_mpm *s, const REAL *st, int64_t n, int64_t p,
+                       REAL *aff) {
+  const int d = s->dim;
+  const int cC = 2 * d, cJF = 2 * d + d * d;
+  REAL cauchy[9];
+  for (int i = 0; i < d * d; i++) cauchy[i] = (REAL)0;
+  if (s->material == ORC_FLUID) {
+    REAL J = st[(int64_t)cJF * n + p];
+    REAL sig = (REAL)s->la * (J - (REAL)1.0);
+    for (int i = 0; i < d; i++) cauchy[i * d + i] = sig;
+  } else {
+    /* 2D fixed-corotated */
+    REAL F00 = st[(int64_t)(cJF + 0) * n + p], F01 = st[(int64_t)(cJF + 1) * n + p];
+    REAL F10 = st[(int64_t)(cJF + 2) * n + p], F11 = st[(int64_t)(cJF + 3) * n + p];
+    REAL J = F00 * F11 - F01 * F10;                 /* Matrix.determinant 2x2 */
+    /* polar_decompose2d, linalg_impl.py:9-26 */
+    REAL x = F00 + F11, y = F10 - F01;
+    REAL sc = (REAL)1.0 / SQRT(x * x + y * y);
+    REAL c = x * sc, sn = y * sc;
+    REAL R00 = c, R01 = -sn, R10 = sn, R11 = c;
+    REAL twomu = (REAL)(2 * s->mu_0);
+    REAL A00 = twomu * (F00 - R00), A01 = twomu * (F01 - R01);
+    REAL A10 = twomu * (F10 - R10), A11 = twomu * (F11 - R11);
+    /* (2 mu (F - R)) @ F^T */
+    REAL P00 = A00 * F00 + A01 * F01, P01 = A00 * F10 + A01 * F11;
+    REAL P10 = A10 * F00 + A11 * F01, P11 = A10 * F10 + A11 * F11;
+    /* + ((I * la) * J) * (J - 1) */
+    REAL dg = ((REAL)s->la_0 * J) * (J - (REAL)1);
+    REAL og = (((REAL)0 * (REAL)s->la_0) * J) * (J - (REAL)1);
+    cauchy[0] = P00 + dg;
+    cauchy[1] = P01 + og;
+    cauchy[2] = P10 + og;
+    cauchy[3] = P11 + dg;
+  }
+  REAL k = (REAL)(-(s->dt * s->p_vol * 4 * s->inv_dx * s->inv_dx));
+  REAL pm = (REAL)s->p_mass;
+  for (int i = 0; i < d * d; i++)
+    aff[i] = k * cauchy[i] + pm * st[(int64_t)(cC + i) * n + p];
+}
+
+void FN(orc_p2g)(const orc_mpm *s, const REAL *st, int64_t n, REAL *gm, REAL *gv) {
+  const int d = s->dim, ng = s->n_grid;
+  const int64_t nodes = d == 3 ? (int64_t)ng * ng * ng : (int64_t)ng * ng;
+  const REAL inv_dx = (REAL)s->inv_dx, dx = (REAL)s->dx, pm = (REAL)s->p_mass;
+  for (int64_t p = 0; p < n; p++) {
+    int base[3] = {0, 0, 0};
+    REAL fx[3] = {0, 0, 0}, w[3][3];
+    for (int a = 0; a < d; a++) {
+      REAL xs = st[(int64_t)a * n + p];
+      base[a] = FN(base_of)(xs, inv_dx, ng);
+      fx[a] = xs * inv_dx - (REAL)base[a];
+      FN(weights)(fx[a], w[a]);
+    }
+    REAL aff[9];
+    FN(affine)(s, st, n, p, aff);
+    REAL mv[3] = {0, 0, 0};
+    for (int a = 0; a < d; a++) mv[a] = pm * st[(int64_t)(d + a) * n + p];
+    const int nk = d == 3 ? 3 : 1;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        for (int k = 0; k < nk; k++) {
+          int off[3] = {i, j, k};
+          REAL dpos[3];
+          for (int a = 0; a < d; a++) dpos[a] = ((REAL)off[a] - fx[a]) * dx;
+          REAL weight = (REAL)1.0;
+          for (int a = 0; a < d; a++) weight *= w[a][off[a]];
+          int64_t node = d == 3 ? ((int64_t)(base[0] + i) * ng + (base[1] + j)) * ng + (base[2] + k)
+                                : (int64_t)(base[0] + i) * ng + (base[1] + j);
+          for (int a = 0; a < d; a++) {
+            REAL ad = aff[a * d + 0] * dpos[0];
+            for (int b = 1; b < d; b++) ad = ad + aff[a * d + b] * dpos[b];
+            gv[(int64_t)a * nodes + node] += weight * (mv[a] + ad);
+          }
+          gm[node] += weight * pm;
+        }
+  }
+}
+
+void FN(orc_grid_op)(const orc_mpm *s, const REAL *gm, REAL *gv) {
+  const int d = s->dim, ng = s->n_grid;
+  const int64_t nodes = d == 3 ? (int64_t)ng * ng * ng : (int64_t)ng * ng;
+  const REAL g = (REAL)(s->dt * s->gravity);
+  const int lb = s->bound, ub = ng - s->bound;
+  for (int64_t node = 0; node < nodes; node++) {
+    int I[3];
+    if (d == 3) {
+      I[0] = (int)(node / ((int64_t)ng * ng));
+      I[1] = (int)((node / ng) % ng);
+      I[2] = (int)(node % ng);
+    } else {
+      I[0] = (int)(node / ng);
+      I[1] = (int)(node % ng);
+      I[2] = 0;
+    }
+    REAL inv_m = (REAL)1 / (gm[node] + (REAL)1e-10);
+    REAL v[3];
+    for (int a = 0; a < d; a++) v[a] = inv_m * gv[(int64_t)a * nodes + node];
+    v[1] = v[1] - g;
+    if (s->boundary == ORC_STICKY) {
+      int wall = 0;
+      for (int a = 0; a < d; a++)
+        if (I[a] < lb || I[a] > ub) wall = 1;
+      if (wall)
+        for (int a = 0; a < d; a++) v[a] = (REAL)0;
+    } else {
+      for (int a = 0; a < d; a++) {
+        if (I[a] < lb && v[a] < 0) v[a] = (REAL)0;
+        if (I[a] > ub && v[a] > 0) v[a] = (REAL)0;
+      }
+    }
+    for (int a = 0; a < d; a++) gv[(int64_t)a * nodes + node] = v[a];
+  }
+}
+
+void FN(orc_g2p)(const orc_mpm *s, const REAL *st, const REAL *gv, REAL *out, int64_t n) {
+  const int d = s->dim, ng = s->n_grid;
+  const int64_t nodes = d == 3 ? (int64_t)ng * ng * ng : (int64_t)ng * ng;
+  const REAL inv_dx = (REAL)s->inv_dx, dt = (REAL)s->dt;
+  const REAL c4 = (REAL)(4 * s->inv_dx);
+  const int cC = 2 * d, cJF = 2 * d + d * d;
+  for (int64_t p = 0; p < n; p++) {
+    int base[3] = {0, 0, 0};
+    REAL fx[3] = {0, 0, 0}, w[3][3];
+    for (int a = 0; a < d; a++) {
+      REAL xs = st[(int64_t)a * n + p];
+      base[a] = FN(base_of)(xs, inv_dx, ng);
+      fx[a] = xs * inv_dx - (REAL)base[a];
+      FN(weights)(fx[a], w[a]);
+    }
+    REAL nv[3] = {0, 0, 0}, nC[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int nk = d == 3 ? 3 : 1;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        for (int k = 0; k < nk; k++) {
+          int off[3] = {i, j, k};
+          REAL dpos[3];
+          for (int a = 0; a < d; a++) dpos[a] = (REAL)off[a] - fx[a];
+          int64_t node = d == 3 ? ((int64_t)(base[0] + i) * ng + (base[1] + j)) * ng + (base[2] + k)
+                                : (int64_t)(base[0] + i) * ng + (base[1] + j);
+          REAL weight = (REAL)1.0;
+          for (int a = 0; a < d; a++) weight *= w[a][off[a]];
+          REAL g[3];
+          for (int a = 0; a < d; a++) g[a] = gv[(int64_t)a * nodes + node];
+          for (int a = 0; a < d; a++) nv[a] = nv[a] + weight * g[a];
+          REAL cw = c4 * weight;
+          for (int a = 0; a < d; a++)
+            for (int b = 0; b < d; b++) nC[a * d + b] = nC[a * d + b] + cw * (g[a] * dpos[b]);
+        }
+    for (int a = 0; a < d; a++) {
+      out[(int64_t)a * n + p] = st[(int64_t)a * n + p] + dt * nv[a];
+      out[(int64_t)(d + a) * n + p] = nv[a];
+    }
+    for (int i = 0; i < d * d; i++) out[(int64_t)(cC + i) * n + p] = nC[i];
+    if (s->material == ORC_FLUID) {
+      REAL tr = nC[0];
+      for (int a = 1; a < d; a++) tr = tr + nC[a * d + a];
+      out[(int64_t)cJF * n + p] = ((REAL)1.0 + dt * tr) * st[(int64_t)cJF * n + p];
+    } else {
+      /* (I + dt C) @ F */
+      REAL M[9], F[9];
+      for (int a = 0; a < d; a++)
+        for (int b = 0; b < d; b++) {
+          M[a * d + b] = (a == b ? (REAL)1 : (REAL)0) + dt * nC[a * d + b];
+          F[a * d + b] = st[(int64_t)(cJF + a * d + b) * n + p];
+        }
+      for (int a = 0; a < d; a++)
+        for (int b = 0; b < d; b++) {
+          REAL acc = M[a * d + 0] * F[0 * d + b];
+          for (int k = 1; k < d; k++) acc = acc + M[a * d + k] * F[k * d + b];
+          out[(int64_t)(cJF + a * d + b) * n + p] = acc;
+        }
+    }
+  }
+}
+
+double FN(orc_energy)(const orc_mpm *s, const REAL *st, int64_t n) {
+  const int d = s->dim;
+  double e = 0.0;
+  for (int64_t p = 0; p < n; p++) {
+    REAL v2 = (REAL)0;
+    for (int a = 0; a < d; a++) {
+      REAL v = st[(int64_t)(d + a) * n + p];
+      v2 = v2 + v * v;
+    }
+    if (d == 3) {
+      e += (double)((REAL)(0.5 * s->p_mass) * v2);
+      e += (double)((REAL)(s->gravity * s->p_mass) * st[(int64_t)1 * n + p]);
+    } else {
+      e += (double)v2;
+    }
+  }
+  return e;
+}
+
+static void FN(zero_grid)(const orc_mpm *s, REAL *gm, REAL *gv) {
+  const int d = s->dim, ng = s->n_grid;
+  const int64_t nodes = d == 3 ? (int64_t)ng * ng * ng : (int64_t)ng * ng;
+  memset(gm, 0, sizeof(REAL) * (size_t)nodes);
+  memset(gv, 0, sizeof(REAL) * (size_t)nodes * (size_t)d);
+}
+
+#undef FN
+#undef CAT
+#undef CAT_

@@ -1,0 +1,184 @@
+"""Pins the oracle MPM substep.
+
+- the reference's own plain-MPM regression (taichi_submission/tests/python/test_mpm88.py:97-104);
+- closed forms of the quadratic B-spline transfer used by
+  mpm_exps/engine/mpm_solver.py:590-758: partition of unity, P2G mass and
+  momentum conservation, G2P reproduction of an affine velocity field;
+- grid boundary conditions (mpm_solver.py:659-664) and the sticky variant;
+- rest state and free fall closed forms.
+"""
+import numpy as np
+import pytest
+from pytest import approx
+
+
+def mpm88_state(orc):
+    """Initial state of test_mpm88.py:85-88: 64x64 lattice, v=(0,-3), J=1."""
+    N = 64
+    n = N * N
+    i = np.arange(n)
+    st = np.zeros((orc.mpm_params(2, orc.FLUID, 128, 2e-4).dim * 2 + 4 + 1, n), dtype=np.float32)
+    st[0] = (i % N) / N * 0.4 + 0.2
+    st[1] = i / N / N * 0.4 + 0.05
+    st[3] = -3.0
+    st[8] = 1.0
+    return st
+
+
+def test_mpm88_regression(orc):
+    """test_mpm88.py:9-104: mean(pos^k) after 500 substeps, rel 1e-2."""
+    s = orc.mpm_params(2, orc.FLUID, 128, 2e-4, la=400.0, gravity=9.8, bound=3)
+    st = mpm88_state(orc)
+    orc.run_plain(s, st, 500)
+    pos = st[:2].T.astype(np.float64).copy()
+    pos[:, 1] *= 2
+    regression = [0.31722742, 0.15826741, 0.10224003, 0.07810827]
+    for k in range(4):
+        assert (pos ** (k + 1)).mean() == approx(regression[k], rel=1e-2)
+
+
+def random_state(orc, s, n, seed=0, f64=False):
+    rng = np.random.default_rng(seed)
+    d = s.dim
+    nc = orc.num_comps(s)
+    st = np.zeros((nc, n), dtype=np.float64 if f64 else np.float32)
+    st[:d] = rng.uniform(0.3, 0.6, size=(d, n))
+    st[d:2 * d] = rng.uniform(-1, 1, size=(d, n))
+    st[2 * d:2 * d + d * d] = rng.uniform(-5, 5, size=(d * d, n))
+    if s.material == orc.FLUID:
+        st[-1] = rng.uniform(0.9, 1.1, size=n)
+    else:
+        F = np.eye(d).reshape(-1, 1) + rng.uniform(-0.05, 0.05, size=(d * d, n))
+        st[-d * d:] = F
+    return st
+
+
+@pytest.mark.parametrize("dim,material", [(2, 0), (3, 0), (2, 1)])
+def test_p2g_conservation(orc, dim, material):
+    """Sum of node masses = n * p_mass; sum of node momenta = sum m v
+    (sum_i w_ip (x_i - x_p) = 0 for quadratic B-splines, so the affine term cancels)."""
+    s = orc.mpm_params(dim, material, 32, 1e-4)
+    n = 500
+    st = random_state(orc, s, n, seed=dim, f64=True)
+    gm, gv = orc.grids(s, f64=True)
+    orc.p2g(s, st, gm, gv)
+    assert gm.sum() == approx(n * s.p_mass, rel=1e-12)
+    mv = s.p_mass * st[dim:2 * dim].sum(axis=1)
+    assert gv.sum(axis=1) == approx(mv, rel=1e-9, abs=1e-12 * s.p_mass)
+    # partition of unity per particle: one particle, mass sums to p_mass
+    gm1, gv1 = orc.grids(s, f64=True)
+    orc.p2g(s, np.ascontiguousarray(st[:, :1]), gm1, gv1)
+    assert gm1.sum() == approx(s.p_mass, rel=1e-14)
+    assert np.count_nonzero(gm1) == 3 ** dim
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_g2p_affine_reproduction(orc, dim):
+    """A linear grid velocity v(x) = a + B x is gathered exactly: v_p = a + B x_p,
+    C_p = B (APIC with quadratic B-splines, mpm_solver.py:682-694)."""
+    s = orc.mpm_params(dim, orc.FLUID, 32, 1e-4)
+    n = 200
+    st = random_state(orc, s, n, seed=3, f64=True)
+    rng = np.random.default_rng(7)
+    a = rng.uniform(-1, 1, size=dim)
+    B = rng.uniform(-1, 1, size=(dim, dim))
+    ng = s.n_grid
+    idx = np.indices((ng,) * dim).reshape(dim, -1).astype(np.float64) * s.dx
+    gv = (a[:, None] + B @ idx).copy()
+    out = orc.g2p(s, st, gv)
+    xp = st[:dim]
+    assert out[dim:2 * dim] == approx(a[:, None] + B @ xp, rel=1e-10, abs=1e-12)
+    C = out[2 * dim:2 * dim + dim * dim].reshape(dim, dim, n)
+    assert C == approx(np.broadcast_to(B[:, :, None], C.shape), rel=1e-9, abs=1e-10)
+    assert out[:dim] == approx(xp + s.dt * out[dim:2 * dim], rel=1e-15)
+
+
+def test_grid_op_boundaries(orc):
+    """Separating walls zero only the inward normal component (mpm_solver.py:659-664);
+    sticky walls zero all components; gravity is subtracted from v_y first."""
+    for boundary in (orc.SEPARATING, orc.STICKY):
+        s = orc.mpm_params(2, orc.FLUID, 16, 1e-3, boundary=boundary, bound=3, gravity=9.8)
+        gm, gv = orc.grids(s, f64=True)
+        gm[:] = 2.0
+        vel = np.array([[-1.0, 1.0], [1.0, -1.0], [0.5, 0.5]])
+        nodes = [(1, 8), (14, 8), (8, 1), (8, 14), (8, 8)]
+        for (i, j) in nodes:
+            for v in vel:
+                gv[:, i * 16 + j] = 2.0 * v
+                orc.grid_op(s, gm.copy(), gv_ := gv.copy())
+                got = gv_[:, i * 16 + j]
+                exp = v.copy()
+                exp[1] -= 1e-3 * 9.8
+                scale = 2.0 / (2.0 + 1e-10)
+                exp = np.array([v[0] * scale, v[1] * scale - 1e-3 * 9.8])
+                I = (i, j)
+                for k in range(2):
+                    wall = I[k] < 3 or I[k] > 13
+                    if boundary == orc.STICKY:
+                        if I[0] < 3 or I[0] > 13 or I[1] < 3 or I[1] > 13:
+                            exp[k] = 0.0
+                    elif wall:
+                        if (I[k] < 3 and exp[k] < 0) or (I[k] > 13 and exp[k] > 0):
+                            exp[k] = 0.0
+                assert got == approx(exp, rel=1e-14, abs=1e-300)
+
+
+def test_rest_state(orc):
+    """No gravity, v = 0, C = 0, F = I: the elastic body stays exactly at rest."""
+    s = orc.mpm_params(2, orc.ELASTIC, 64, 1e-4, gravity=0.0, E=1e4, nu=0.2)
+    n = 300
+    st = random_state(orc, s, n, seed=5)
+    st[2:8] = 0.0
+    st[8:] = np.eye(2).reshape(-1, 1)
+    x0 = st[:2].copy()
+    orc.run_plain(s, st, 20)
+    assert np.array_equal(st[:2], x0)
+    assert np.all(st[2:8] == 0)
+
+
+def test_free_fall(orc):
+    """Far from walls and with J = 1 a fluid blob falls freely: after k substeps
+    v_y = -k dt g (grid velocities are exact to rounding), C = 0.  Unit particle
+    mass keeps the 1e-10 guard of inv_m = 1/(m + 1e-10) below rounding."""
+    s = orc.mpm_params(3, orc.FLUID, 32, 1e-4, gravity=9.8, p_vol=1.0)
+    n = 200
+    st = random_state(orc, s, n, seed=9, f64=True)
+    st[3:] = 0.0
+    st[-1] = 1.0
+    orc.run_plain(s, st, 10)
+    assert st[4] == approx(np.full(n, -10 * 1e-4 * 9.8), rel=1e-7)
+    # C = 4 inv_dx sum w v (x_i - x_p): rounding of the nearly uniform v only
+    assert np.abs(st[6:15]).max() < 1e-6
+
+
+def test_energy(orc):
+    """compute_ek (mpm_solver.py:1043-1052): 3D 1/2 m |v|^2 + g m y; 2D sum |v|^2."""
+    s3 = orc.mpm_params(3, orc.FLUID, 32, 1e-4)
+    st = random_state(orc, s3, 50, seed=1, f64=True)
+    e = 0.5 * s3.p_mass * (st[3:6] ** 2).sum() + 9.8 * s3.p_mass * st[1].sum()
+    assert orc.energy(s3, st) == approx(e, rel=1e-12)
+    s2 = orc.mpm_params(2, orc.ELASTIC, 32, 1e-4)
+    st2 = random_state(orc, s2, 50, seed=2, f64=True)
+    assert orc.energy(s2, st2) == approx((st2[2:4] ** 2).sum(), rel=1e-12)
+
+
+def test_quant_run_matches_plain_then_quantize(orc):
+    """orc_run_quant is decode -> substep -> encode: one quantized substep equals a
+    plain substep on the decoded state followed by pack."""
+    s = orc.mpm_params(2, orc.ELASTIC, 64, 1e-4, E=1e4, boundary=orc.STICKY)
+    n = 256
+    st = random_state(orc, s, n, seed=11)
+    comps = []
+    off = 0
+    for c in range(12):
+        comps.append(dict(kind=orc.FIXED, nbits=21, is_signed=1, bit_offset=off,
+                          scale=orc.fixed_scale(21, 8.0, True)))
+        off += 21
+    nw = (off + 63) // 64
+    w = orc.pack(comps, nw, st)
+    dec = orc.unpack(comps, nw, w)
+    w2 = w.copy()
+    orc.run_quant(s, comps, nw, w2, 1, seed=0)
+    ref = dec.copy()
+    orc.run_plain(s, ref, 1)
+    assert np.array_equal(w2, orc.pack(comps, nw, ref))
