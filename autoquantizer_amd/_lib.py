@@ -1,0 +1,112 @@
+"""ctypes binding of libaqmpm.so (include/aq_mpm.h).
+
+The library is the only compute path: if it is missing or cannot load, every
+entry point raises.  There is no CPU fallback.
+"""
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libaqmpm.so")
+
+AQ_MAX_COMPS = 32
+AQ_MAX_WORDS = 16
+
+AQ_FIXED, AQ_INT, AQ_FLOAT, AQ_F32 = 0, 1, 2, 3
+AQ_BIT_STRUCT, AQ_BIT_PACK = 0, 1
+AQ_FLUID, AQ_ELASTIC = 0, 1
+AQ_SEPARATING, AQ_STICKY = 0, 1
+AQ_STORE_QUANT, AQ_STORE_F32, AQ_STORE_F64 = 0, 1, 2
+
+
+class Component(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("num_bits", ctypes.c_int32),
+                ("is_signed", ctypes.c_int32), ("bit_offset", ctypes.c_int32),
+                ("exp_bits", ctypes.c_int32), ("exp_offset", ctypes.c_int32),
+                ("shared", ctypes.c_int32), ("use_dithering", ctypes.c_int32),
+                ("scale", ctypes.c_double), ("offset", ctypes.c_double)]
+
+
+class Place(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("num_bits", ctypes.c_int32),
+                ("is_signed", ctypes.c_int32), ("exp_bits", ctypes.c_int32),
+                ("use_dithering", ctypes.c_int32), ("container", ctypes.c_int32),
+                ("shared_group", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("scale", ctypes.c_double), ("offset", ctypes.c_double)]
+
+
+class Container(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("num_bits", ctypes.c_int32)]
+
+
+class SimDesc(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("material", ctypes.c_int32),
+                ("n_grid", ctypes.c_int32), ("boundary", ctypes.c_int32),
+                ("bound", ctypes.c_int32), ("storage", ctypes.c_int32),
+                ("ncomp", ctypes.c_int32), ("nwords", ctypes.c_int32),
+                ("capacity", ctypes.c_int64),
+                ("dx", ctypes.c_double), ("inv_dx", ctypes.c_double),
+                ("dt", ctypes.c_double), ("p_vol", ctypes.c_double),
+                ("p_mass", ctypes.c_double), ("gravity", ctypes.c_double),
+                ("la", ctypes.c_double), ("mu_0", ctypes.c_double),
+                ("la_0", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("comps", Component * AQ_MAX_COMPS)]
+
+
+class AQError(RuntimeError):
+    pass
+
+
+EXPORTS = [
+    "aq_last_error", "aq_version", "aq_layout_plan", "aq_pack", "aq_unpack",
+    "aq_store_members", "aq_sim_workspace_bytes", "aq_sim_create", "aq_sim_destroy",
+    "aq_substep", "aq_p2g", "aq_grid_op", "aq_g2p", "aq_advance", "aq_sort",
+    "aq_update_limits", "aq_energy", "aq_sim_grid", "aq_advance_timed",
+]
+
+_lib = None
+
+
+def lib():
+    """Load libaqmpm.so; raises AQError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AQError(f"{LIB_PATH} is not built; run `python -m autoquantizer_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    P, i32, i64, u32, u64 = (ctypes.c_void_p, ctypes.c_int, ctypes.c_int64,
+                             ctypes.c_uint32, ctypes.c_uint64)
+    L.aq_last_error.restype = ctypes.c_char_p
+    L.aq_last_error.argtypes = []
+    L.aq_version.restype = ctypes.c_char_p
+    L.aq_version.argtypes = []
+    L.aq_layout_plan.argtypes = [P, i32, P, i32, P, P]
+    L.aq_pack.argtypes = [P, i32, i32, P, P, u64, u32, P, i64, P]
+    L.aq_unpack.argtypes = [P, i32, i32, P, P, i64, P]
+    L.aq_store_members.argtypes = [P, i32, i32, P, i64, u32, P, P, i64, u64, u32, P]
+    L.aq_sim_workspace_bytes.restype = i64
+    L.aq_sim_workspace_bytes.argtypes = [P]
+    L.aq_sim_create.argtypes = [P, P, i64, ctypes.POINTER(P)]
+    L.aq_sim_destroy.argtypes = [P]
+    L.aq_substep.argtypes = [P, P, i64, u32, P]
+    L.aq_p2g.argtypes = [P, P, i64, P]
+    L.aq_grid_op.argtypes = [P, P]
+    L.aq_g2p.argtypes = [P, P, i64, u32, P]
+    L.aq_advance.argtypes = [P, P, i64, i32, u32, i32, P, P]
+    L.aq_sort.argtypes = [P, P, i64, P, P]
+    L.aq_update_limits.argtypes = [P, P, i64, P, P]
+    L.aq_energy.argtypes = [P, P, i64, P, P]
+    L.aq_sim_grid.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(i64)]
+    L.aq_advance_timed.argtypes = [P, P, i64, i32, u32, i32, P, P, P, P, P]
+    for name in EXPORTS:
+        if name not in ("aq_last_error", "aq_version", "aq_sim_workspace_bytes"):
+            getattr(L, name).restype = i32
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise AQError(f"libaqmpm error {rc}: {lib().aq_last_error().decode()}")
+    return rc

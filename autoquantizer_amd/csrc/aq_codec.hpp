@@ -1,0 +1,274 @@
+// Device-side codec for quantized particle records on gfx950.
+//
+// A record is held in registers as 32-bit words r[0..2*NW).  Component
+// parameters are uniform (kernel arguments), so every branch on a
+// component's kind/width is a scalar branch and every r[] index is an SGPR
+// index.  Bit semantics follow the reference codegen (TS/ =
+// taichi_submission/):
+//   fixed encode   TS/taichi/codegen/codegen_llvm_quant.cpp:44-80 and
+//                  TS/taichi/runtime/llvm/runtime.cpp:1732-1762
+//   fixed decode   codegen_llvm_quant.cpp:624-641
+//   custom float   codegen_llvm_quant.cpp:232-301 (store), 660-757 (load)
+//   shared exp.    codegen_llvm_quant.cpp:347-499 (store), 660-757 (load)
+//   bit placement  TS/taichi/struct/struct_llvm.cpp:71-116, split fields
+//                  codegen_llvm_quant.cpp:798-878
+// This file is compiled with -ffp-contract=off: each float operation of the
+// encode/decode is rounded on its own, as in the reference's unfused form.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "aq_mpm.h"
+
+namespace aq {
+
+enum : uint8_t { K_FIXED = AQ_FIXED, K_INT = AQ_INT, K_FLOAT = AQ_FLOAT, K_F32 = AQ_F32 };
+
+// Kernel-side component: 24 bytes, uniform.
+struct CompK {
+  uint8_t kind, nbits, is_signed, dither;
+  uint8_t exp_bits, shared, gfirst, glast;  // shared groups are contiguous runs
+  uint16_t off, eoff;                        // bit offsets of digits / exponent
+  uint32_t conv;                             // exponent conversion offset 127-2^(e-1)+1
+  float enc_s;                               // (float)(1.0 / scale)
+  float dec_s;                               // (float)scale
+  float offs;                                // (float)offset
+};
+
+struct LayoutK {
+  int32_t nc;
+  int32_t nw;
+  CompK c[AQ_MAX_COMPS];
+};
+
+__device__ __forceinline__ uint32_t lowmask(uint32_t nb) {
+  return nb >= 32 ? 0xFFFFFFFFu : ((1u << nb) - 1u);
+}
+
+__device__ __forceinline__ int32_t sext(uint32_t q, uint32_t nb) {
+  const uint32_t sh = 32u - nb;
+  return (int32_t)(q << sh) >> sh;
+}
+
+// Field [off, off+nb) of the record, nb <= 32; a field may cross one 32-bit
+// word boundary (and so one u64 word boundary).
+template <int NW32>
+__device__ __forceinline__ uint32_t get_field(const uint32_t (&r)[NW32], uint32_t off,
+                                              uint32_t nb) {
+  const uint32_t i = off >> 5, b = off & 31u;
+  uint32_t v;
+  if (b + nb > 32u) {
+    v = __builtin_amdgcn_alignbit(r[i + 1], r[i], b);
+  } else {
+    v = r[i] >> b;
+  }
+  return v & lowmask(nb);
+}
+
+// OR a field into a record whose field bits are zero.
+template <int NW32>
+__device__ __forceinline__ void put_field(uint32_t (&r)[NW32], uint32_t off, uint32_t nb,
+                                          uint32_t v) {
+  const uint32_t i = off >> 5, b = off & 31u;
+  v &= lowmask(nb);
+  r[i] |= v << b;
+  if (b + nb > 32u) r[i + 1] |= v >> (32u - b);
+}
+
+// Replace a field (masked store).
+template <int NW32>
+__device__ __forceinline__ void set_field(uint32_t (&r)[NW32], uint32_t off, uint32_t nb,
+                                          uint32_t v) {
+  const uint32_t i = off >> 5, b = off & 31u;
+  const uint32_t m = lowmask(nb);
+  v &= m;
+  r[i] = (r[i] & ~(m << b)) | (v << b);
+  if (b + nb > 32u) {
+    const uint32_t s = 32u - b;
+    r[i + 1] = (r[i + 1] & ~(m >> s)) | (v >> s);
+  }
+}
+
+// fptosi / fptoui with saturation and NaN -> 0 (the reference's values are
+// LLVM poison outside the int range).
+__device__ __forceinline__ uint32_t cvt_sat(float y, bool is_signed) {
+  if (is_signed) {
+    if (!(y == y)) return 0u;
+    if (y >= 2147483648.0f) return 0x7FFFFFFFu;
+    if (y < -2147483648.0f) return 0x80000000u;
+    return (uint32_t)(int32_t)y;
+  }
+  if (!(y > 0.0f)) return 0u;
+  if (y >= 4294967296.0f) return 0xFFFFFFFFu;
+  return (uint32_t)y;
+}
+
+// Dithering noise in [0, 1): lowbias32 (C. Wellons, hash-prospector) of the
+// record slot xor a per-(seed, step, component) key.
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+__device__ __forceinline__ float noise_u(uint32_t slot, uint32_t key) {
+  return (float)(hash32(slot ^ key) >> 8) * (1.0f / 16777216.0f);
+}
+
+__device__ __forceinline__ uint32_t enc_fixed(float v, const CompK &c, float u) {
+  float y = v - c.offs;
+  y = y * c.enc_s;
+  if (c.dither) {
+    y = y + u;
+    y = y - 0.5f;
+  }
+  y = y + __builtin_copysignf(0.5f, y);
+  return cvt_sat(y, c.is_signed) & lowmask(c.nbits);
+}
+
+__device__ __forceinline__ float dec_fixed(uint32_t q, const CompK &c) {
+  const float cf = c.is_signed ? (float)sext(q, c.nbits) : (float)q;
+  const float t = cf * c.dec_s;
+  return t + c.offs;
+}
+
+__device__ __forceinline__ uint32_t f32_exp(float f) {
+  return (__float_as_uint(f) >> 23) & 0xFFu;
+}
+
+// Custom float with its own exponent: returns digits, writes the exponent.
+__device__ __forceinline__ uint32_t enc_float(float f, const CompK &c, uint32_t &e_out) {
+  const uint32_t db = c.nbits - c.is_signed;
+  uint32_t b = __float_as_uint(f);
+  if (db < 23u) b += 1u << (22u - db);
+  const uint32_t e = (b >> 23) & 0xFFu;
+  uint32_t d = (uint32_t)((int32_t)b >> (23u - db)) & lowmask(db);
+  if (c.is_signed) d |= (b & 0x80000000u) >> (31u - db);
+  int32_t es = (int32_t)e - (e ? (int32_t)c.conv : 0);
+  es = es < 0 ? 0 : es;
+  e_out = (uint32_t)es & lowmask(c.exp_bits);
+  return es ? (d & lowmask(c.nbits)) : 0u;
+}
+
+__device__ __forceinline__ float dec_float(uint32_t e, uint32_t q, const CompK &c) {
+  const uint32_t db = c.nbits - c.is_signed;
+  const uint32_t off = (c.exp_bits < 8 && e == 0u) ? 0u : c.conv;
+  const uint32_t d = (q & lowmask(c.nbits)) << (23u - db);
+  uint32_t fb = ((e + off) << 23) | (d & 0x7FFFFFu);
+  if (c.is_signed) fb |= (d & 0x800000u) << 8;
+  return __uint_as_float(fb);
+}
+
+// Digits of one member of a shared-exponent group with max f32 exponent m.
+__device__ __forceinline__ uint32_t enc_shared(float f, uint32_t m, const CompK &c) {
+  const uint32_t b = __float_as_uint(f);
+  const uint32_t e = (b >> 23) & 0xFFu;
+  uint32_t sh = m - e;
+  sh = sh > 31u ? 31u : sh;
+  uint32_t d = ((b & 0x7FFFFFu) | 0x800000u) >> sh;
+  const uint32_t db = c.nbits - c.is_signed;
+  const uint32_t rs = 24u - db;  // 23 + signed - db, +1 when unsigned (:418-422)
+  d = d + (1u << (rs - 1u));
+  d = d > 0xFFFFFFu ? 0xFFFFFFu : d;
+  d >>= rs;
+  if (c.is_signed) d |= (b & 0x80000000u) >> (31u - db);
+  return d & lowmask(c.nbits);
+}
+
+__device__ __forceinline__ uint32_t shared_exp_field(uint32_t m, const CompK &c) {
+  int32_t es = (int32_t)m - (m ? (int32_t)c.conv : 0);
+  es = es < 0 ? 0 : es;
+  return (uint32_t)es & lowmask(c.exp_bits);
+}
+
+__device__ __forceinline__ float dec_shared(uint32_t e, uint32_t q, const CompK &c) {
+  const uint32_t db = c.nbits - c.is_signed;
+  q &= lowmask(c.nbits);
+  // zero test on the digits as loaded (sign-extended for signed members)
+  const bool zero_digits = c.is_signed ? (sext(q, c.nbits) == 0) : (q == 0u);
+  uint32_t off = (c.exp_bits < 8 && e == 0u) ? 0u : c.conv;
+  uint32_t d = q, sign = 0u;
+  if (c.is_signed) {
+    sign = d & (1u << db);
+    d ^= sign;
+    sign <<= (31u - db);
+    d <<= 1;
+  }
+  const int32_t lz = d ? __builtin_clz(d) : 32;
+  const int32_t extra = (int32_t)(31u - db) - lz;
+  off += (uint32_t)extra;
+  if (!c.is_signed) off += 1u;
+  d <<= (uint32_t)((int32_t)(23u - db) - extra);
+  uint32_t fb = ((e + off) << 23) | (d & 0x7FFFFFu);
+  if (e == 0u || zero_digits) fb = 0u;
+  return __uint_as_float(fb | sign);
+}
+
+// Decode component c of a record.
+template <int NW32>
+__device__ __forceinline__ float decode_comp(const uint32_t (&r)[NW32], const CompK &c) {
+  if (c.kind == K_FIXED) return dec_fixed(get_field(r, c.off, c.nbits), c);
+  if (c.kind == K_F32) return __uint_as_float(get_field(r, c.off, 32u));
+  if (c.kind == K_INT) {
+    const uint32_t q = get_field(r, c.off, c.nbits);
+    return c.is_signed ? (float)sext(q, c.nbits) : (float)q;
+  }
+  const uint32_t e = get_field(r, c.eoff, c.exp_bits);
+  const uint32_t q = get_field(r, c.off, c.nbits);
+  return c.shared ? dec_shared(e, q, c) : dec_float(e, q, c);
+}
+
+// Encode NC values (compile-time count, components lay.c[0..NC)) into a
+// zeroed record.  keys[c] is the dithering key of component c.
+template <int NC, int NW32>
+__device__ __forceinline__ void encode_record(const float (&v)[NC], const LayoutK &lay,
+                                              const uint32_t *keys, uint32_t slot,
+                                              uint32_t (&r)[NW32]) {
+  uint32_t gmax[NC];
+  uint32_t run = 0u;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const CompK &k = lay.c[c];
+    gmax[c] = 0u;
+    if (k.kind == K_FLOAT && k.shared) {
+      const uint32_t e = f32_exp(v[c]);
+      run = k.gfirst ? e : (run > e ? run : e);
+      gmax[c] = run;
+    }
+  }
+#pragma unroll
+  for (int c = NC - 1; c >= 0; --c) {
+    const CompK &k = lay.c[c];
+    if (k.kind == K_FLOAT && k.shared) {
+      run = k.glast ? gmax[c] : run;
+      gmax[c] = run;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const CompK &k = lay.c[c];
+    if (k.kind == K_FIXED) {
+      const float u = k.dither ? noise_u(slot, keys[c]) : 0.0f;
+      put_field(r, k.off, k.nbits, enc_fixed(v[c], k, u));
+    } else if (k.kind == K_FLOAT) {
+      if (k.shared) {
+        if (k.gfirst) put_field(r, k.eoff, k.exp_bits, shared_exp_field(gmax[c], k));
+        put_field(r, k.off, k.nbits, enc_shared(v[c], gmax[c], k));
+      } else {
+        uint32_t e;
+        const uint32_t d = enc_float(v[c], k, e);
+        put_field(r, k.eoff, k.exp_bits, e);
+        put_field(r, k.off, k.nbits, d);
+      }
+    } else if (k.kind == K_F32) {
+      put_field(r, k.off, 32u, __float_as_uint(v[c]));
+    } else {
+      put_field(r, k.off, k.nbits, cvt_sat(v[c], k.is_signed));
+    }
+  }
+}
+
+}  // namespace aq

@@ -1,0 +1,337 @@
+// MLS-MPM substep kernels for gfx950.
+//
+// Physics follows MPMSolver of the reference (mpm_exps/engine/mpm_solver.py):
+//   stencil base/weights  :599-606, :676-678
+//   stress + affine       :608-613, compute_cauchy :1012-1031,
+//                         polar_decompose2d (TS/python/taichi/lang/linalg_impl.py:9-26)
+//   P2G scatter           :615-622
+//   grid update           :633-665 (separating walls; sticky walls zero the node)
+//   G2P gather + update   :682-694, :702 (J), :750-758 (F)
+// with the reference's operand order; the file is compiled with
+// -ffp-contract=off, so per-particle results equal the CPU restatement's and
+// only the order of the P2G float atomics differs.
+//
+// Grid: n^DIM nodes in blocks of 64 (4x4x4 in 3D, 8x8 in 2D), block-major,
+// one 16-byte (float4) or 32-byte (double4) record per node:
+// (m v_x, m v_y, m v_z, m) after P2G, (v_x, v_y, v_z, m) after the grid update.
+// A grid block of 64 nodes is one wavefront in the grid kernels.
+#pragma once
+
+#include "aq_codec.hpp"
+
+namespace aq {
+
+template <class T> struct Vec4;
+template <> struct Vec4<float> { using type = float4; };
+template <> struct Vec4<double> { using type = double4; };
+template <class T> using vec4_t = typename Vec4<T>::type;
+
+template <class T> struct ParamsT {
+  T dx, inv_dx, dt, p_mass, stress_k, la, two_mu, la_0, g_dt, c4, half_pm, g_pm, eps_m;
+  int32_t n, nb, bound, boundary;  // cells per axis, blocks per axis, wall, kind
+};
+
+template <int DIM> struct BlockShape;
+template <> struct BlockShape<3> { static constexpr int E = 4, LOG = 2; };
+template <> struct BlockShape<2> { static constexpr int E = 8, LOG = 3; };
+
+template <int DIM, int MAT> struct Comps {
+  static constexpr int CX = 0, CV = DIM, CC = 2 * DIM, CJ = 2 * DIM + DIM * DIM;
+  static constexpr int NC = 2 * DIM + DIM * DIM + (MAT == AQ_FLUID ? 1 : DIM * DIM);
+};
+
+// ---------------------------------------------------------------------------
+// state access: AQ_STORE_QUANT (word planes + layout) or plain value planes
+// ---------------------------------------------------------------------------
+struct StateK {
+  void *ptr;
+  int64_t cap;        // plane stride
+  int32_t nw;         // quant: record words
+  uint32_t g2p_mask;  // quant: words holding x and J/F (what G2P reads)
+};
+
+template <int NW32>
+__device__ __forceinline__ void load_words(const StateK &s, int64_t p, uint32_t mask,
+                                           uint32_t (&r)[NW32]) {
+  const uint64_t *w = (const uint64_t *)s.ptr;
+#pragma unroll
+  for (int i = 0; i < NW32 / 2; ++i) {
+    r[2 * i] = 0u;
+    r[2 * i + 1] = 0u;
+    if (i < s.nw && ((mask >> i) & 1u)) {
+      const uint64_t x = __builtin_nontemporal_load(w + (int64_t)i * s.cap + p);
+      r[2 * i] = (uint32_t)x;
+      r[2 * i + 1] = (uint32_t)(x >> 32);
+    }
+  }
+}
+
+template <int NW32>
+__device__ __forceinline__ void store_words(const StateK &s, int64_t p, const uint32_t (&r)[NW32]) {
+  uint64_t *w = (uint64_t *)s.ptr;
+#pragma unroll
+  for (int i = 0; i < NW32 / 2; ++i)
+    if (i < s.nw)
+      __builtin_nontemporal_store((uint64_t)r[2 * i] | ((uint64_t)r[2 * i + 1] << 32),
+                                  w + (int64_t)i * s.cap + p);
+}
+
+// Load components [c0, c1) of particle p as T.
+template <int ST, int NW32, int NC, class T>
+__device__ __forceinline__ void load_state(const StateK &s, const LayoutK &lay, int64_t p,
+                                           uint32_t mask, T (&v)[NC]) {
+  if constexpr (ST == AQ_STORE_QUANT) {
+    uint32_t r[NW32];
+    load_words(s, p, mask, r);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = (T)decode_comp(r, lay.c[c]);
+  } else {
+    const T *vals = (const T *)s.ptr;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = vals[(int64_t)c * s.cap + p];
+  }
+}
+
+template <int ST, int NW32, int NC, class T>
+__device__ __forceinline__ void store_state(const StateK &s, const LayoutK &lay,
+                                            const uint32_t *keys, int64_t p, T (&v)[NC],
+                                            uint32_t (&r)[NW32 > 0 ? NW32 : 1]) {
+  if constexpr (ST == AQ_STORE_QUANT) {
+    float f[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) f[c] = (float)v[c];
+#pragma unroll
+    for (int i = 0; i < NW32; ++i) r[i] = 0u;
+    encode_record<NC>(f, lay, keys, (uint32_t)p, r);
+    store_words(s, p, r);
+  } else {
+    T *vals = (T *)s.ptr;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) vals[(int64_t)c * s.cap + p] = v[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// stencil
+// ---------------------------------------------------------------------------
+template <class T> __device__ __forceinline__ T tfloor(T x);
+template <> __device__ __forceinline__ float tfloor<float>(float x) { return floorf(x); }
+template <> __device__ __forceinline__ double tfloor<double>(double x) { return floor(x); }
+template <class T> __device__ __forceinline__ T tmax(T a, T b);
+template <> __device__ __forceinline__ float tmax<float>(float a, float b) { return fmaxf(a, b); }
+template <> __device__ __forceinline__ double tmax<double>(double a, double b) { return fmax(a, b); }
+template <class T> __device__ __forceinline__ T tmin(T a, T b);
+template <> __device__ __forceinline__ float tmin<float>(float a, float b) { return fminf(a, b); }
+template <> __device__ __forceinline__ double tmin<double>(double a, double b) { return fmin(a, b); }
+template <class T> __device__ __forceinline__ T tsqrt(T a);
+template <> __device__ __forceinline__ float tsqrt<float>(float a) { return sqrtf(a); }
+template <> __device__ __forceinline__ double tsqrt<double>(double a) { return sqrt(a); }
+
+// base = floor(x * inv_dx - 0.5) clamped to [0, n-3] (NaN -> 0); fx; weights.
+template <class T>
+__device__ __forceinline__ int stencil_1d(T xs, const ParamsT<T> &P, T &fx, T (&w)[3]) {
+  T t = xs * P.inv_dx;
+  t = t - (T)0.5;
+  t = tmax<T>(t, (T)0);
+  t = tmin<T>(t, (T)(P.n - 3));
+  const int b = (int)tfloor<T>(t);
+  fx = xs * P.inv_dx - (T)b;
+  const T a = (T)1.5 - fx;
+  w[0] = (T)0.5 * (a * a);
+  const T bb = fx - (T)1;
+  w[1] = (T)0.75 - bb * bb;
+  const T c = fx - (T)0.5;
+  w[2] = (T)0.5 * (c * c);
+  return b;
+}
+
+// Node offsets of the three stencil nodes per axis in the blocked layout.
+template <int DIM>
+__device__ __forceinline__ void axis_offsets(int base, int axis, int nb, int32_t (&o)[3]) {
+  constexpr int E = BlockShape<DIM>::E, LG = BlockShape<DIM>::LOG;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int c = base + i;
+    const int blk = c >> LG, loc = c & (E - 1);
+    int32_t bs, ls;
+    if constexpr (DIM == 3) {
+      bs = axis == 0 ? nb * nb : (axis == 1 ? nb : 1);
+      ls = axis == 0 ? 16 : (axis == 1 ? 4 : 1);
+    } else {
+      bs = axis == 0 ? nb : 1;
+      ls = axis == 0 ? 8 : 1;
+    }
+    o[i] = blk * bs * 64 + loc * ls;
+  }
+}
+
+template <int DIM>
+__device__ __forceinline__ int32_t block_of_base(const int (&base)[3], int nb) {
+  constexpr int LG = BlockShape<DIM>::LOG;
+  if constexpr (DIM == 3)
+    return ((base[0] >> LG) * nb + (base[1] >> LG)) * nb + (base[2] >> LG);
+  else
+    return (base[0] >> LG) * nb + (base[1] >> LG);
+}
+
+// compute_cauchy * stress coefficient + p_mass * C (mpm_solver.py:608-613, 1012-1031)
+template <int DIM, int MAT, class T, int NC>
+__device__ __forceinline__ void affine_of(const T (&v)[NC], const ParamsT<T> &P,
+                                          T (&aff)[DIM * DIM]) {
+  using CI = Comps<DIM, MAT>;
+  T cauchy[DIM * DIM];
+#pragma unroll
+  for (int i = 0; i < DIM * DIM; ++i) cauchy[i] = (T)0;
+  if constexpr (MAT == AQ_FLUID) {
+    const T sig = P.la * (v[CI::CJ] - (T)1.0);
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) cauchy[i * DIM + i] = sig;
+  } else {
+    static_assert(DIM == 2, "fixed-corotated elasticity is 2D");
+    const T F00 = v[CI::CJ + 0], F01 = v[CI::CJ + 1], F10 = v[CI::CJ + 2], F11 = v[CI::CJ + 3];
+    const T J = F00 * F11 - F01 * F10;
+    const T x = F00 + F11, y = F10 - F01;
+    const T sc = (T)1.0 / tsqrt<T>(x * x + y * y);
+    const T c = x * sc, sn = y * sc;
+    const T A00 = P.two_mu * (F00 - c), A01 = P.two_mu * (F01 - (-sn));
+    const T A10 = P.two_mu * (F10 - sn), A11 = P.two_mu * (F11 - c);
+    const T P00 = A00 * F00 + A01 * F01, P01 = A00 * F10 + A01 * F11;
+    const T P10 = A10 * F00 + A11 * F01, P11 = A10 * F10 + A11 * F11;
+    const T dg = (P.la_0 * J) * (J - (T)1);
+    const T og = (((T)0 * P.la_0) * J) * (J - (T)1);
+    cauchy[0] = P00 + dg;
+    cauchy[1] = P01 + og;
+    cauchy[2] = P10 + og;
+    cauchy[3] = P11 + dg;
+  }
+#pragma unroll
+  for (int i = 0; i < DIM * DIM; ++i) aff[i] = P.stress_k * cauchy[i] + P.p_mass * v[CI::CC + i];
+}
+
+template <class T> __device__ __forceinline__ void atomic_add(T *a, T v) { unsafeAtomicAdd(a, v); }
+
+// P2G of one particle with global atomics.  Returns the block of its base.
+template <int DIM, int MAT, class T, int NC>
+__device__ __forceinline__ int32_t p2g_particle(const T (&v)[NC], const ParamsT<T> &P,
+                                                vec4_t<T> *__restrict__ grid) {
+  using CI = Comps<DIM, MAT>;
+  int base[3] = {0, 0, 0};
+  T fx[3] = {0, 0, 0}, w[3][3];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) base[a] = stencil_1d<T>(v[CI::CX + a], P, fx[a], w[a]);
+  T aff[DIM * DIM];
+  affine_of<DIM, MAT, T>(v, P, aff);
+  T mv[3];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) mv[a] = P.p_mass * v[CI::CV + a];
+  int32_t o0[3], o1[3], o2[3] = {0, 0, 0};
+  axis_offsets<DIM>(base[0], 0, P.nb, o0);
+  axis_offsets<DIM>(base[1], 1, P.nb, o1);
+  if constexpr (DIM == 3) axis_offsets<DIM>(base[2], 2, P.nb, o2);
+  constexpr int NK = DIM == 3 ? 3 : 1;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int k = 0; k < NK; ++k) {
+        const int off[3] = {i, j, k};
+        T dpos[3];
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) dpos[a] = ((T)off[a] - fx[a]) * P.dx;
+        T weight = (T)1.0;
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) weight *= w[a][off[a]];
+        T *node = (T *)(grid + (o0[i] + o1[j] + o2[k]));
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) {
+          T ad = aff[a * DIM + 0] * dpos[0];
+#pragma unroll
+          for (int b = 1; b < DIM; ++b) ad = ad + aff[a * DIM + b] * dpos[b];
+          atomic_add<T>(node + a, weight * (mv[a] + ad));
+        }
+        atomic_add<T>(node + 3, weight * P.p_mass);
+      }
+  return block_of_base<DIM>(base, P.nb);
+}
+
+// G2P of one particle: new state from old x, J/F and the grid velocities.
+template <int DIM, int MAT, class T, int NC>
+__device__ __forceinline__ void g2p_particle(const T (&v)[NC], const ParamsT<T> &P,
+                                             const vec4_t<T> *__restrict__ grid, T (&out)[NC]) {
+  using CI = Comps<DIM, MAT>;
+  int base[3] = {0, 0, 0};
+  T fx[3] = {0, 0, 0}, w[3][3];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) base[a] = stencil_1d<T>(v[CI::CX + a], P, fx[a], w[a]);
+  int32_t o0[3], o1[3], o2[3] = {0, 0, 0};
+  axis_offsets<DIM>(base[0], 0, P.nb, o0);
+  axis_offsets<DIM>(base[1], 1, P.nb, o1);
+  if constexpr (DIM == 3) axis_offsets<DIM>(base[2], 2, P.nb, o2);
+  T nv[3] = {0, 0, 0}, nC[DIM * DIM];
+#pragma unroll
+  for (int i = 0; i < DIM * DIM; ++i) nC[i] = (T)0;
+  constexpr int NK = DIM == 3 ? 3 : 1;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int k = 0; k < NK; ++k) {
+        const int off[3] = {i, j, k};
+        T dpos[3];
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) dpos[a] = (T)off[a] - fx[a];
+        const vec4_t<T> nd = grid[o0[i] + o1[j] + o2[k]];
+        const T g[3] = {nd.x, nd.y, nd.z};
+        T weight = (T)1.0;
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) weight *= w[a][off[a]];
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) nv[a] = nv[a] + weight * g[a];
+        const T cw = P.c4 * weight;
+#pragma unroll
+        for (int a = 0; a < DIM; ++a)
+#pragma unroll
+          for (int b = 0; b < DIM; ++b) nC[a * DIM + b] = nC[a * DIM + b] + cw * (g[a] * dpos[b]);
+      }
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    out[CI::CX + a] = v[CI::CX + a] + P.dt * nv[a];
+    out[CI::CV + a] = nv[a];
+  }
+#pragma unroll
+  for (int i = 0; i < DIM * DIM; ++i) out[CI::CC + i] = nC[i];
+  if constexpr (MAT == AQ_FLUID) {
+    T tr = nC[0];
+#pragma unroll
+    for (int a = 1; a < DIM; ++a) tr = tr + nC[a * DIM + a];
+    out[CI::CJ] = ((T)1.0 + P.dt * tr) * v[CI::CJ];
+  } else {
+    T M[DIM * DIM];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a)
+#pragma unroll
+      for (int b = 0; b < DIM; ++b) M[a * DIM + b] = (a == b ? (T)1 : (T)0) + P.dt * nC[a * DIM + b];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a)
+#pragma unroll
+      for (int b = 0; b < DIM; ++b) {
+        T acc = M[a * DIM + 0] * v[CI::CJ + 0 * DIM + b];
+#pragma unroll
+        for (int k = 1; k < DIM; ++k) acc = acc + M[a * DIM + k] * v[CI::CJ + k * DIM + b];
+        out[CI::CJ + a * DIM + b] = acc;
+      }
+  }
+}
+
+// Thread-block index remapped so each XCD (blocks are dealt round-robin to
+// the 8 XCDs) gets a contiguous range of particles: spatially sorted
+// particles then hit grid nodes cached in that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nblocks) {
+  const uint32_t per = nblocks / 8u;
+  if (bid >= per * 8u) return bid;  // tail keeps its id
+  return (bid & 7u) * per + (bid >> 3);
+}
+
+}  // namespace aq

@@ -1,0 +1,387 @@
+"""MLS-MPM solver over quantized particle records: the drop-in for the
+reference's MPMConfig / MPMSolver operator API on the substep path.
+
+Mirrors mpm_exps/mpm_config.py:1-53 and mpm_exps/engine/mpm_solver.py
+(MPMSolver.__init__ :19-122, init_quantaichi :364-455, reshape_params
+:522-535, initialize :564-576, build_pid :578-588, p2g_bls :590-622,
+substep :624-631, grid_op :633-665, g2p_bls :667-758, detect_bound* and
+get_limits :944-997, compute_ek :1043-1052, add_cube :1328-1342).
+Every phase runs in libaqmpm on the GPU; the state lives in torch tensors.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .fields import comps_array, plan
+from .quant import PlainF32, place_record, quant
+
+
+class MPMConfig:
+    """Defaults of mpm_exps/mpm_config.py:1-53 (fields the substep path reads)."""
+
+    def __init__(self):
+        self.n_grid = 128
+        self.n_particles = 9000
+        self.dim = 2
+        self.dt = 1e-4
+        self.p_rho = 1
+        self.p_vol = (1 / self.n_grid * 0.5) ** self.dim
+        self.E, self.nu = 0.1e4, 0.2
+        self.la = 100000
+        self.quantized_properties = None
+        self.quant_bits = None
+        self.quant_ranges = None
+        self.use_quantaichi = False
+        self.use_bitpack = False
+        self.use_fluid = True
+        self.use_dithering = True
+        self.bound = None
+        self.lb = None
+        self.ub = None
+        self.g = 9.8
+        self.compute_fp = 'f64'
+        self.size = 1.0
+        self.softer = False
+        # additions: wall kind and dithering seed
+        self.sticky = False
+        self.seed = 0
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def component_names(dim, fluid):
+    names = [f"x{a}" for a in range(dim)] + [f"v{a}" for a in range(dim)]
+    names += [f"C{a}{b}" for a in range(dim) for b in range(dim)]
+    names += ["J"] if fluid else [f"F{a}{b}" for a in range(dim) for b in range(dim)]
+    return names
+
+
+def reference_layout(cfg):
+    """The record init_quantaichi builds (mpm_solver.py:364-455): every
+    component a signed fixed-point type of quant_bits[c] bits and range
+    quant_ranges[c] (J with offset 1.0), in one bit_pack when use_bitpack,
+    else the bit_struct words of :411-444.  Returns (containers, places)."""
+    dim, fluid = cfg.dim, cfg.use_fluid
+    names = component_names(dim, fluid)
+    bits = list(np.asarray(cfg.quant_bits).reshape(-1))
+    ranges = list(np.asarray(cfg.quant_ranges, dtype=np.float64).reshape(-1))
+    if len(bits) != len(names) or len(ranges) != len(names):
+        raise ValueError(f"need {len(names)} bits and ranges")
+    types = {}
+    for k, nm in enumerate(names):
+        types[nm] = quant.fixed(frac=int(bits[k]), range=float(ranges[k]), signed=True,
+                                offset=1.0 if nm == "J" else 0.0, use_dithering=cfg.use_dithering)
+    if cfg.use_bitpack:
+        groups = [("pack", 0, names)]
+    elif fluid and dim == 3:
+        groups = [("struct", 64, ["x0", "x1", "x2"]), ("struct", 64, ["v0", "v1", "v2"]),
+                  ("struct", 64, ["C00", "C01", "C02", "C10"]), ("struct", 64, ["C11", "C12", "C20", "C21"]),
+                  ("struct", 32, ["C22", "J"])]
+    elif not fluid and dim == 2:
+        groups = [("struct", 64, ["x0", "x1", "v0"]), ("struct", 64, ["v1", "C00", "C01", "C10", "C11"]),
+                  ("struct", 64, ["F00", "F01", "F10"]), ("struct", 32, ["F11"])]
+    else:
+        groups = [("pack", 0, names)]
+    return types, groups
+
+
+def plan_state(names, types, groups, shared=()):
+    """Place the state components: groups = [(kind, num_bits, [names])];
+    names listed in `shared` prefixes form one shared-exponent group per field.
+    Returns canonical-order components and the record words."""
+    containers, places, owner = [], [], []
+    gid = 0
+    for ci, (kind, nb, members) in enumerate(groups):
+        containers.append((_lib.AQ_BIT_PACK if kind == "pack" else _lib.AQ_BIT_STRUCT, nb))
+        cur_field, cur_gid = None, -1
+        for nm in members:
+            rec = place_record(types[nm])
+            fieldname = nm[0]
+            sg = -1
+            if fieldname in shared and rec["kind"] == _lib.AQ_FLOAT:
+                if cur_field != fieldname:
+                    cur_field, cur_gid = fieldname, gid
+                    gid += 1
+                sg = cur_gid
+            else:
+                cur_field = None
+            rec.update(container=ci, shared_group=sg)
+            places.append(rec)
+            owner.append(nm)
+    comps, nw = plan(containers, places)
+    by = dict(zip(owner, comps))
+    missing = [nm for nm in names if nm not in by]
+    if missing:
+        raise ValueError(f"components not placed: {missing}")
+    return [by[nm] for nm in names], nw
+
+
+class MPMSolver:
+    """MLS-MPM on one GPU with quantized (or plain fp32/fp64) particle state.
+
+    Quantized storage: `types` maps component names (x0, v1, C01, J, F10, ...)
+    to quant types, `groups` lists containers [(kind, bits, names)], and
+    `shared` names the fields whose members share one exponent.  Without
+    `types`, cfg.use_quantaichi selects the reference's fixed-point scheme
+    (cfg.quant_bits / cfg.quant_ranges) and otherwise plain storage in
+    cfg.compute_fp precision."""
+
+    def __init__(self, cfg, types=None, groups=None, shared=(), capacity=None, device="cuda"):
+        self.param = cfg
+        self.dim = cfg.dim
+        self.fluid = bool(cfg.use_fluid)
+        self.n_particles = int(cfg.n_particles)
+        self.capacity = int(capacity or self.n_particles)
+        self.n_grid = int(cfg.n_grid)
+        self.dx = 1 / self.n_grid
+        self.inv_dx = float(self.n_grid)
+        self.dt = cfg.dt
+        self.p_vol = cfg.p_vol
+        self.p_mass = cfg.p_vol * cfg.p_rho
+        self.E, self.nu = cfg.E, cfg.nu
+        self.mu_0 = self.E / (2 * (1 + self.nu))
+        self.la_0 = self.E * self.nu / ((1 + self.nu) * (1 - 2 * self.nu))
+        if cfg.softer:
+            self.mu_0 *= 0.3
+            self.la_0 *= 0.3
+        self.device = device
+        self.names = component_names(self.dim, self.fluid)
+        self.ncomp = len(self.names)
+        if types is None and cfg.use_quantaichi:
+            types, groups = reference_layout(cfg)
+        if types is not None:
+            self.storage = _lib.AQ_STORE_QUANT
+            if groups is None:
+                groups = [("pack", 0, self.names)]
+            self.comps, self.nwords = plan_state(self.names, types, groups, tuple(shared))
+            self.state = torch.zeros((self.nwords, self.capacity), dtype=torch.int64, device=device)
+        else:
+            self.comps, self.nwords = [], 0
+            f64 = str(cfg.compute_fp) in ("f64", "float64")
+            self.storage = _lib.AQ_STORE_F64 if f64 else _lib.AQ_STORE_F32
+            self.state = torch.zeros((self.ncomp, self.capacity),
+                                     dtype=torch.float64 if f64 else torch.float32, device=device)
+        self.ids = torch.arange(self.capacity, dtype=torch.int32, device=device)
+        self.current_idx = 0
+        self.step_count = 0
+        bound = cfg.bound if cfg.bound is not None else [3] * self.dim
+        self.bound = int(bound[0])
+        self._make_sim(cfg)
+        self.init_limits()
+
+    # ---- handle -----------------------------------------------------------
+    def _make_sim(self, cfg):
+        d = _lib.SimDesc()
+        d.dim, d.material = self.dim, _lib.AQ_FLUID if self.fluid else _lib.AQ_ELASTIC
+        d.n_grid = self.n_grid
+        d.boundary = _lib.AQ_STICKY if getattr(cfg, "sticky", False) else _lib.AQ_SEPARATING
+        d.bound = self.bound
+        d.storage = self.storage
+        d.ncomp = self.ncomp
+        d.nwords = self.nwords
+        d.capacity = self.capacity
+        d.dx, d.inv_dx, d.dt, d.p_vol = self.dx, self.inv_dx, self.dt, self.p_vol
+        d.p_mass, d.gravity = self.p_mass, cfg.g
+        d.la, d.mu_0, d.la_0 = cfg.la, self.mu_0, self.la_0
+        d.seed = int(getattr(cfg, "seed", 0))
+        for k, c in enumerate(self.comps):
+            for key, val in c.items():
+                setattr(d.comps[k], key, val)
+        self._desc = d
+        L = _lib.lib()
+        nbytes = L.aq_sim_workspace_bytes(ctypes.byref(d))
+        if nbytes < 0:
+            _lib.check(-1)
+        self.workspace = torch.empty(int(nbytes), dtype=torch.uint8, device=self.device)
+        h = ctypes.c_void_p()
+        _lib.check(L.aq_sim_create(ctypes.byref(d), _ptr(self.workspace), nbytes, ctypes.byref(h)))
+        self._h = h
+        self._carr = comps_array(self.comps) if self.comps else None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().aq_sim_destroy(h)
+            except Exception:
+                pass
+
+    # ---- state in and out -------------------------------------------------
+    def set_state(self, vals, n=None):
+        """vals: [ncomp, n] float array (canonical component order)."""
+        vals = np.asarray(vals, dtype=np.float64)
+        n = vals.shape[1] if n is None else n
+        if n > self.capacity:
+            raise ValueError("more particles than capacity")
+        self.current_idx = n
+        if self.storage == _lib.AQ_STORE_QUANT:
+            v = torch.as_tensor(np.ascontiguousarray(vals[:, :n], dtype=np.float32), device=self.device)
+            tmp = torch.zeros((self.nwords, n), dtype=torch.int64, device=self.device)
+            _lib.check(_lib.lib().aq_pack(self._carr, self.ncomp, self.nwords, _ptr(v), None,
+                                          self._desc.seed, 0xFFFFFFFF, _ptr(tmp), n, _stream()))
+            self.state[:, :n] = tmp
+        else:
+            self.state[:, :n] = torch.as_tensor(vals[:, :n], dtype=self.state.dtype, device=self.device)
+        self.ids[:n] = torch.arange(n, dtype=torch.int32, device=self.device)
+        torch.cuda.current_stream().synchronize()
+
+    def set_words(self, words):
+        """Load raw records [nwords, n] (quantized storage)."""
+        w = torch.as_tensor(np.asarray(words).view(np.int64), device=self.device)
+        n = w.shape[1]
+        self.state[:, :n] = w
+        self.current_idx = n
+        self.ids[:n] = torch.arange(n, dtype=torch.int32, device=self.device)
+
+    def get_words(self, original_order=False):
+        n = self.current_idx
+        w = self.state[:, :n]
+        if original_order:
+            w = w[:, self._inverse_perm()]
+        return w.contiguous().cpu().numpy().view(np.uint64)
+
+    def _inverse_perm(self):
+        n = self.current_idx
+        ids = self.ids[:n].long()
+        inv = torch.empty_like(ids)
+        inv[ids] = torch.arange(n, device=ids.device)
+        return inv
+
+    def get_state(self, original_order=True):
+        """Decoded state [ncomp, n] as float64 numpy (particle order of set_state)."""
+        n = self.current_idx
+        if self.storage == _lib.AQ_STORE_QUANT:
+            w = self.state[:, :n].contiguous()
+            vals = torch.empty((self.ncomp, n), dtype=torch.float32, device=self.device)
+            _lib.check(_lib.lib().aq_unpack(self._carr, self.ncomp, self.nwords, _ptr(w), _ptr(vals), n,
+                                            _stream()))
+        else:
+            vals = self.state[:, :n]
+        if original_order:
+            vals = vals[:, self._inverse_perm()]
+        return vals.double().cpu().numpy()
+
+    # ---- scene setup (mpm_solver.py:564-576, 1328-1342, 1561-1589) --------
+    def initialize(self):
+        self.current_idx = 0
+        self._pending = []
+
+    def add_cube(self, low_corner, box_size, num, velocity=None, seed=0, lattice=False):
+        """Particles in a box: uniform random (seeded numpy generator) or, with
+        lattice=True, the regular grid of fill_in_box_2duniform (:1561-1571)."""
+        d = self.dim
+        if lattice:
+            m = int(round(num ** (1.0 / d)))
+            g = np.stack(np.meshgrid(*[np.arange(m)] * d, indexing="ij"), -1).reshape(-1, d)[:, ::-1]
+            x = np.asarray(low_corner) + g / m * np.asarray(box_size)
+        else:
+            rng = np.random.default_rng(seed)
+            x = np.asarray(low_corner) + rng.random((num, d)) * np.asarray(box_size)
+        vals = np.zeros((self.ncomp, x.shape[0]))
+        vals[:d] = x.T
+        if velocity is not None:
+            vals[d:2 * d] = np.asarray(velocity, dtype=np.float64).reshape(d, 1)
+        if self.fluid:
+            vals[-1] = 1.0
+        else:
+            vals[-d * d:] = np.eye(d).reshape(-1, 1)
+        self._pending = getattr(self, "_pending", []) + [vals]
+        allv = np.concatenate(self._pending, axis=1)
+        self.set_state(allv)
+
+    # ---- the substep (mpm_solver.py:624-631) ------------------------------
+    def substep(self, s=0, t=None):
+        _lib.check(_lib.lib().aq_substep(self._h, _ptr(self.state), self.current_idx,
+                                         self.step_count, _stream()))
+        self.step_count += 1
+
+    def build_pid(self, s=0):
+        _lib.check(_lib.lib().aq_sort(self._h, _ptr(self.state), self.current_idx, _ptr(self.ids),
+                                      _stream()))
+
+    def p2g_bls(self, s=0, t=None):
+        _lib.check(_lib.lib().aq_p2g(self._h, _ptr(self.state), self.current_idx, _stream()))
+
+    def grid_op(self):
+        _lib.check(_lib.lib().aq_grid_op(self._h, _stream()))
+
+    def g2p_bls(self, s=0, t=None):
+        _lib.check(_lib.lib().aq_g2p(self._h, _ptr(self.state), self.current_idx, self.step_count,
+                                     _stream()))
+        self.step_count += 1
+
+    def advance(self, nsteps, sort_every=0):
+        """nsteps substeps with G2P fused into the next P2G."""
+        _lib.check(_lib.lib().aq_advance(self._h, _ptr(self.state), self.current_idx, int(nsteps),
+                                         self.step_count, int(sort_every), _ptr(self.ids), _stream()))
+        self.step_count += int(nsteps)
+
+    def grid_velocity(self):
+        """Last grid [nodes, 4] (v_x, v_y, v_z, m) in the blocked node order."""
+        g = ctypes.c_void_p()
+        nodes = ctypes.c_int64()
+        _lib.check(_lib.lib().aq_sim_grid(self._h, ctypes.byref(g), ctypes.byref(nodes)))
+        dt = torch.float64 if self.storage == _lib.AQ_STORE_F64 else torch.float32
+        esz = 8 if dt == torch.float64 else 4
+        off = g.value - self.workspace.data_ptr()
+        nbytes = nodes.value * 4 * esz
+        torch.cuda.current_stream().synchronize()
+        return self.workspace[off:off + nbytes].view(dt).reshape(nodes.value, 4).cpu().numpy()
+
+    def grid_dense(self):
+        """Grid [d+1, n^d] in dense (i, j[, k]) order: velocity components then mass."""
+        g = self.grid_velocity()
+        n, d = self.n_grid, self.dim
+        e = 4 if d == 3 else 8
+        nb = n // e
+        idx = np.indices((n,) * d).reshape(d, -1)
+        if d == 3:
+            blk = ((idx[0] // 4) * nb + idx[1] // 4) * nb + idx[2] // 4
+            loc = (idx[0] % 4) * 16 + (idx[1] % 4) * 4 + idx[2] % 4
+        else:
+            blk = (idx[0] // 8) * nb + idx[1] // 8
+            loc = (idx[0] % 8) * 8 + idx[1] % 8
+        node = blk * 64 + loc
+        out = np.zeros((d + 1, n ** d))
+        for a in range(d):
+            out[a] = g[node, a]
+        out[d] = g[node, 3]
+        return out
+
+    # ---- loss and ranges -------------------------------------------------
+    def compute_ek(self):
+        out = torch.zeros(1, dtype=torch.float64, device=self.device)
+        _lib.check(_lib.lib().aq_energy(self._h, _ptr(self.state), self.current_idx, _ptr(out),
+                                        _stream()))
+        return float(out.item())
+
+    compute_loss = compute_ek
+
+    def init_limits(self):
+        self._mxmn = torch.empty(2 * self.ncomp, dtype=torch.float32, device=self.device)
+        self._mxmn[:self.ncomp] = -math.inf
+        self._mxmn[self.ncomp:] = math.inf
+
+    def detect_bound_kernel(self, s=0):
+        _lib.check(_lib.lib().aq_update_limits(self._h, _ptr(self.state), self.current_idx,
+                                               _ptr(self._mxmn), _stream()))
+
+    def detect_bound(self):
+        m = self._mxmn.cpu().numpy().astype(np.float64)
+        self.limits = np.maximum(np.abs(m[:self.ncomp]), np.abs(m[self.ncomp:]))
+        return self.limits
+
+    def get_limits(self):
+        """get_limits (mpm_solver.py:987-997): x placeholder 2, then max|v|, max|C|, max|J| or max|F|."""
+        lim = self.detect_bound()
+        d = self.dim
+        return np.concatenate([[2.0] * d, lim[d:]])

@@ -1,0 +1,215 @@
+"""Benchmark of the quantized MLS-MPM substep on one MI355X.
+
+Workload: 3D weakly compressible fluid (J-based EOS, bulk modulus 400),
+1,048,576 particles uniformly at random (seed 0) in [0.1, 0.5]^3, 128^3 grid,
+dt = 2e-4, gravity 9.8, separating walls 3 cells thick; state stored as the
+32-byte quantized record of autoquantizer_amd.layouts.fluid_3d (x 3 x fixed21,
+v 3 x fixed16, C 9 x shared-exponent float(8, 10), J fixed16, dithered),
+f32 compute.  A step is one substep of all particles (P2G, grid update, G2P;
+G2P fused with the next P2G).  Printed: one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-substeps/s + scheme-search time, quantized vs fp32 MPM, 1 MI355X"
+HBM_PEAK = 8.0e12
+
+
+def make_solver(storage, n, n_grid, seed=0):
+    import autoquantizer_amd as aq
+    from autoquantizer_amd import layouts
+    cfg = aq.MPMConfig()
+    cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.la, cfg.g = 3, True, n_grid, 2e-4, 400.0, 9.8
+    cfg.p_vol = (1 / n_grid * 0.5) ** 3
+    cfg.n_particles = n
+    cfg.seed = 1234
+    if storage == "quant":
+        names, t, g, sh = layouts.fluid_3d()
+        return aq.MPMSolver(cfg, types=t, groups=g, shared=sh)
+    cfg.compute_fp = storage
+    return aq.MPMSolver(cfg)
+
+
+def initial_state(n, seed=0):
+    rng = np.random.default_rng(seed)
+    vals = np.zeros((16, n))
+    vals[:3] = rng.uniform(0.1, 0.5, (3, n))
+    vals[15] = 1.0
+    return vals
+
+
+def timed_run(sim, steps, sort_every):
+    """Wall time of `steps` substeps (synchronised both sides)."""
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.advance(steps, sort_every=sort_every)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def kernel_times(sim, steps, sort_every):
+    """Per-kernel event timing over `steps` substeps (aq_advance_timed)."""
+    import torch
+    from autoquantizer_amd import _lib
+    ms = (ctypes.c_float * 4)()
+    launches = (ctypes.c_int32 * 4)()
+    act = ctypes.c_int64()
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(_lib.lib().aq_advance_timed(sim._h, ctypes.c_void_p(sim.state.data_ptr()), sim.current_idx,
+                                           steps, sim.step_count, sort_every,
+                                           ctypes.c_void_p(sim.ids.data_ptr()), ms, launches,
+                                           ctypes.byref(act), stream))
+    sim.step_count += steps
+    return list(ms), list(launches), act.value
+
+
+def field_errors(a, b):
+    """Per-field RMS and max |a - b| over particles; a, b: [16, n]."""
+    out = {}
+    for name, sl in (("x", slice(0, 3)), ("v", slice(3, 6)), ("C", slice(6, 15)), ("J", slice(15, 16))):
+        d = a[sl] - b[sl]
+        out[name] = {"rms": float(np.sqrt(np.mean(d * d))), "max": float(np.abs(d).max()),
+                     "ref_rms": float(np.sqrt(np.mean(b[sl] ** 2)))}
+    return out
+
+
+def cpu_baseline(n, n_grid, target_s=10.0):
+    """The C oracle (the port of the reference substep) on the host's cores, on
+    the same quantized workload, for a bounded number of substeps."""
+    from oracle import orc
+    from autoquantizer_amd import layouts
+    from autoquantizer_amd.mpm import plan_state
+    names, t, g, sh = layouts.fluid_3d()
+    comps, nw = plan_state(names, t, g, sh)
+    oc = [dict(kind=c["kind"], nbits=c["num_bits"], is_signed=c["is_signed"], bit_offset=c["bit_offset"],
+               exp_bits=c["exp_bits"], exp_offset=c["exp_offset"], shared=c["shared"],
+               dither=c["use_dithering"], scale=c["scale"], offset=c["offset"]) for c in comps]
+    s = orc.mpm_params(3, orc.FLUID, n_grid, 2e-4, la=400.0)
+    threads = min(os.cpu_count() or 1, 16)
+    words = orc.pack(oc, nw, initial_state(n).astype(np.float32), seed=1234, step=0)
+    t0 = time.perf_counter()
+    orc.run_quant(s, oc, nw, words, 1, seed=1234, step0=0, nthreads=threads)
+    t1 = time.perf_counter() - t0
+    k = int(max(1, min(20, target_s / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    orc.run_quant(s, oc, nw, words, k, seed=1234, step0=1, nthreads=threads)
+    tk = time.perf_counter() - t0
+    return {"value": n * k / tk, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} particles, {k} quantized substeps after 1 untimed (128^3 grid), C oracle "
+                      f"with OpenMP, {tk:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--particles", type=int, default=1 << 20)
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--sort-every", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the substep has no multi-GPU path",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    n, K, W = args.particles, args.steps, args.warmup
+    init = initial_state(n)
+
+    results = {}
+    for storage in ("quant", "f32"):
+        sim = make_solver(storage, n, args.grid)
+        sim.set_state(init)
+        if W:
+            sim.advance(W, sort_every=args.sort_every)
+        sim.set_state(init)
+        sim.step_count = 0
+        el = timed_run(sim, K, args.sort_every)
+        final = sim.get_state()
+        # event-timed repeat from the same initial state, for the per-kernel roofline
+        sim.set_state(init)
+        sim.step_count = 0
+        ms, launches, act = kernel_times(sim, K, args.sort_every)
+        results[storage] = dict(elapsed=el, final=final, ms=ms, launches=launches, act=act,
+                                nwords=sim.nwords, g2p_words=None)
+        if storage == "quant":
+            mask = 0
+            for c in list(range(3)) + [15]:
+                cc = sim.comps[c]
+                for b in range(cc["bit_offset"], cc["bit_offset"] + cc["num_bits"]):
+                    mask |= 1 << (b >> 6)
+            results[storage]["g2p_words"] = bin(mask).count("1")
+        del sim
+        torch.cuda.empty_cache()
+
+    q, f = results["quant"], results["f32"]
+    value = n * K / q["elapsed"]
+    fp32_value = n * K / f["elapsed"]
+
+    # roofline of the dominant kernel: the fused G2P->P2G launch
+    avg_s = q["ms"][0] / 1e3 / max(q["launches"][0], 1)
+    per_particle = q["g2p_words"] * 8 + q["nwords"] * 8  # words G2P reads + whole record written
+    per_node = 32  # grid velocity read once (16 B) + accumulated momentum/mass written once (16 B)
+    alg_bytes = n * per_particle + q["act"] * per_node
+    achieved = alg_bytes / avg_s
+    f_avg = f["ms"][0] / 1e3 / max(f["launches"][0], 1)
+    f_bytes = n * (4 * 4 + 16 * 4) + f["act"] * per_node  # x, J read; 16 floats written
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": q["elapsed"] * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"3D fluid MLS-MPM, {n} particles uniform in [0.1,0.5]^3 (seed 0), "
+                               f"{args.grid}^3 grid, dt 2e-4, quantized 32 B/particle record, dithered",
+                   "particles": n, "grid": args.grid, "sort_every": args.sort_every,
+                   "bytes_per_particle": q["nwords"] * 8, "fp32_bytes_per_particle": 64},
+        "fp32": {"value": fp32_value, "ms_per_step": f["elapsed"] * 1e3 / K,
+                 "fused_kernel_us": f_avg * 1e6, "fused_kernel_GBps": f_bytes / f_avg / 1e9},
+        "speedup_vs_fp32": value / fp32_value,
+        "kernel_ms": {"fused_g2p2g": q["ms"][0], "grid_update": q["ms"][1], "sort": q["ms"][2],
+                      "first_p2g_last_g2p": q["ms"][3], "launches": q["launches"]},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "k_g2p2g<3,fluid,quant>", "kernel_us": avg_s * 1e6,
+                     "algorithmic_bytes": alg_bytes, "active_nodes": q["act"],
+                     "bytes_per_particle": per_particle, "bytes_per_node": per_node},
+        "scheme_search_s": None,
+    }
+    if not args.no_fp64:
+        sim = make_solver("f64", n, args.grid)
+        sim.set_state(init)
+        sim.advance(K, sort_every=args.sort_every)
+        ref = sim.get_state()
+        del sim
+        out["error_vs_fp64"] = {"quant": field_errors(q["final"], ref), "fp32": field_errors(f["final"], ref)}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n, args.grid)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,214 @@
+/*
+ * aq_mpm.h -- C ABI of libaqmpm.so, the MI355X (gfx950) implementation of the
+ * quantized MLS-MPM substep of Hanke98/AutoQuantizer.
+ *
+ * Paths in citations are relative to the reference root; TS/ is
+ * taichi_submission/ (the reference's patched Taichi v0.8.4).
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative AQ_ERR_* code; the
+ *    message of the last error on the calling thread is aq_last_error().
+ *    No C++ exception crosses the ABI.
+ *  - Device pointers are caller-owned (the Python layer passes torch tensor
+ *    storage); the library never allocates device memory.  A simulation's
+ *    grid, flags and sort buffers live in a caller-provided workspace of
+ *    aq_sim_workspace_bytes() bytes.
+ *  - `stream` is a hipStream_t (NULL = default stream).  Nothing synchronises
+ *    implicitly; work is enqueued on `stream` in call order.
+ *  - Particle state is stored as word planes: plane w of a record lives at
+ *    words[w * capacity + p].  Plain storage uses value planes
+ *    vals[c * capacity + p] (float or double).
+ *  - A handle (aq_sim) is not thread-safe; use one stream per handle.
+ */
+#ifndef AQ_MPM_H
+#define AQ_MPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AQ_OK 0
+#define AQ_ERR_INVALID (-1)  /* bad argument or layout */
+#define AQ_ERR_HIP (-2)      /* a HIP runtime call failed */
+#define AQ_ERR_CAPACITY (-3) /* size beyond what the handle or workspace holds */
+
+#define AQ_MAX_COMPS 32
+#define AQ_MAX_WORDS 16
+
+/* Quant type kinds (TS/python/taichi/lang/quant_impl.py:13-73,
+   TS/taichi/ir/type.h:207-293). */
+enum aq_kind {
+  AQ_FIXED = 0, /* ti.quant.fixed: custom float without exponent, scale+offset */
+  AQ_INT = 1,   /* ti.quant.int: custom int                                     */
+  AQ_FLOAT = 2, /* ti.quant.float: exponent + digits (shared or not)            */
+  AQ_F32 = 3    /* plain IEEE binary32 member (unquantized)                      */
+};
+
+/* One scalar member of a particle record after placement.
+   Replaces the (CustomIntType/CustomFloatType, bit_offset, word_offset,
+   exp_snode) tuple the reference keeps per place SNode
+   (TS/taichi/ir/snode.h:121-145, TS/taichi/struct/struct_llvm.cpp:71-116). */
+typedef struct aq_component {
+  int32_t kind;       /* enum aq_kind */
+  int32_t num_bits;   /* digit/int bits incl. sign; 32 for AQ_F32 */
+  int32_t is_signed;
+  int32_t bit_offset; /* absolute bit offset in the record, LSB first */
+  int32_t exp_bits;   /* AQ_FLOAT: exponent width (<= 8) */
+  int32_t exp_offset; /* AQ_FLOAT: absolute bit offset of the exponent field */
+  int32_t shared;     /* AQ_FLOAT: exponent shared by all members with this exp_offset */
+  int32_t use_dithering;
+  double scale;       /* AQ_FIXED: range/2^(bits-1) signed, range/2^bits unsigned */
+  double offset;      /* AQ_FIXED: offset */
+} aq_component;
+
+/* Container kinds of a placement (TS/python/taichi/lang/snode.py:108-180). */
+enum aq_container_kind { AQ_BIT_STRUCT = 0, AQ_BIT_PACK = 1 };
+
+/* One member to place.  `container` indexes the containers array;
+   `shared_group` >= 0 puts the member in a shared-exponent placement group
+   (SNode.place(..., shared_exponent=True)); members of one group must be
+   consecutive in the places array and use the same exponent width. */
+typedef struct aq_place {
+  int32_t kind, num_bits, is_signed, exp_bits, use_dithering;
+  int32_t container;
+  int32_t shared_group;
+  int32_t pad;
+  double scale, offset;
+} aq_place;
+
+typedef struct aq_container {
+  int32_t kind;     /* enum aq_container_kind */
+  int32_t num_bits; /* AQ_BIT_STRUCT: 8, 16, 32 or 64; ignored for AQ_BIT_PACK */
+} aq_container;
+
+const char *aq_last_error(void);
+/* Library version string and the GPU architecture it was built for. */
+const char *aq_version(void);
+
+/* Lay out places into a particle record of u64 words.  Follows
+   struct_llvm.cpp:71-116 (offsets accumulate in placement order; bit_pack
+   word = offset/64, members may straddle words; bit_struct members must fit
+   the container) and snode_expr_utils.cpp:44-89 (a custom-float exponent is
+   placed before its first user; a shared group has one exponent).
+   Containers are aligned to their size (bit_pack and bit_struct(64): one
+   word).  Unlike the reference, a bit_pack may hold custom floats with
+   exponents (codegen_llvm_quant.cpp:766-767 forbids them there).
+   out[i] describes places[i]; *nwords receives the record size. */
+int aq_layout_plan(const aq_container *containers, int ncontainers,
+                   const aq_place *places, int nplaces, aq_component *out,
+                   int *nwords);
+
+/* ---- codec: replaces the quantized load/store codegen ---------------------
+   aq_pack: bit_pack_assign of whole records
+   (TS/taichi/codegen/codegen_llvm_quant.cpp:884-952 via
+   TS/python/taichi/lang/impl.py:753-756): vals[c * n + p] (device, f32)
+   encoded with float_to_custom_int (:44-80) / shared exponents (:347-499)
+   and written whole.  noise: NULL = counter-based noise of (seed, step,
+   p, c); else device [ncomp][n] uniforms in [0, 1).
+   aq_unpack: load_custom_float / load_custom_int (:759-784, :529-621). */
+int aq_pack(const aq_component *comps, int ncomp, int nwords, const float *vals,
+            const float *noise, uint64_t seed, uint32_t step, uint64_t *words,
+            int64_t n, void *stream);
+int aq_unpack(const aq_component *comps, int ncomp, int nwords,
+              const uint64_t *words, float *vals, int64_t n, void *stream);
+/* Masked store of the members selected by `mask` (bit c = component c) into
+   records idx[k] (k < count); vals[j * count + k] is the value of the j-th
+   selected member (increasing component order).  A non-shared member is a
+   masked store (TS/taichi/codegen/codegen_llvm.cpp:1270-1300); a shared
+   exponent group with a selected member is re-encoded, its unselected
+   members reconstructed from the record first (codegen_llvm_quant.cpp:347-383).
+   Selecting all members of one container is bit_pack_assign
+   (TS/python/taichi/lang/impl.py:753-756).  idx, vals: device arrays;
+   duplicate indices race. */
+int aq_store_members(const aq_component *comps, int ncomp, int nwords,
+                     uint64_t *words, int64_t n, uint32_t mask, const int64_t *idx,
+                     const float *vals, int64_t count, uint64_t seed,
+                     uint32_t step, void *stream);
+
+/* ---- MLS-MPM simulation: replaces MPMSolver's kernels ---------------------
+   (mpm_exps/engine/mpm_solver.py:19-122, 578-758). */
+enum aq_material { AQ_FLUID = 0, AQ_ELASTIC = 1 };
+enum aq_boundary { AQ_SEPARATING = 0, AQ_STICKY = 1 };
+enum aq_storage { AQ_STORE_QUANT = 0, AQ_STORE_F32 = 1, AQ_STORE_F64 = 2 };
+
+typedef struct aq_sim_desc {
+  int32_t dim;      /* 2 or 3 */
+  int32_t material; /* AQ_FLUID (J, sigma = la (J-1) I) or AQ_ELASTIC (2D fixed-corotated) */
+  int32_t n_grid;   /* cells per axis; multiple of 8 (2D) or 4 (3D) */
+  int32_t boundary; /* enum aq_boundary */
+  int32_t bound;    /* wall thickness: lb = bound, ub = n_grid - bound */
+  int32_t storage;  /* enum aq_storage */
+  int32_t ncomp;    /* number of state components (2d + d^2 + 1 or 2d + 2d^2) */
+  int32_t nwords;   /* AQ_STORE_QUANT: record size in u64 words */
+  int64_t capacity; /* particle capacity = plane stride */
+  double dx, inv_dx, dt, p_vol, p_mass, gravity;
+  double la;        /* fluid bulk modulus (MPMSolver.la) */
+  double mu_0, la_0;/* elastic Lame parameters */
+  uint64_t seed;    /* dithering seed */
+  /* AQ_STORE_QUANT: components in the order of g2p_bls's bit_pack_assign
+     (mpm_solver.py:700-744): x[d], v[d], C[d*d] row major, J or F[d*d]. */
+  aq_component comps[AQ_MAX_COMPS];
+} aq_sim_desc;
+
+typedef struct aq_sim aq_sim;
+
+/* Workspace for the grid (two node buffers), block flags and the sort. */
+int64_t aq_sim_workspace_bytes(const aq_sim_desc *desc);
+int aq_sim_create(const aq_sim_desc *desc, void *workspace, int64_t workspace_bytes,
+                  aq_sim **out);
+int aq_sim_destroy(aq_sim *sim);
+
+/* The reference substep (mpm_solver.py:624-631): grid.deactivate_all,
+   build_pid, p2g_bls, grid_op, g2p_bls, as separate launches.  `state` is
+   the words (AQ_STORE_QUANT) or value planes; n <= capacity. `step` keys the
+   dithering noise of the G2P encode. */
+int aq_substep(aq_sim *sim, void *state, int64_t n, uint32_t step, void *stream);
+int aq_p2g(aq_sim *sim, const void *state, int64_t n, void *stream);
+int aq_grid_op(aq_sim *sim, void *stream);
+int aq_g2p(aq_sim *sim, void *state, int64_t n, uint32_t step, void *stream);
+
+/* nsteps substeps with G2P of substep t fused with P2G of substep t+1
+   (one launch reads the state once, writes it once and scatters the next
+   grid).  Same results as nsteps aq_substep calls up to the order of the
+   P2G float atomics.  Particles are re-sorted by grid block every
+   `sort_every` substeps (0 = never); ids (device int32[capacity] or NULL) is
+   permuted along. */
+int aq_advance(aq_sim *sim, void *state, int64_t n, int nsteps, uint32_t step0,
+               int sort_every, int32_t *ids, void *stream);
+
+/* aq_advance with HIP events recorded on `stream` around every launch:
+   ms[0] fused G2P->P2G kernels, ms[1] grid updates, ms[2] sorts, ms[3] the
+   first P2G and last G2P (summed milliseconds); launches[0..3] the launch
+   counts per category; *active_nodes the grid nodes in active blocks of the
+   last grid.  Synchronises `stream`.  For measurement; the reference's
+   counterpart is its kernel profiler (TS/taichi/program/kernel_profiler.h,
+   TS/taichi/backends/cuda/cuda_profiler.cpp:10-44). */
+int aq_advance_timed(aq_sim *sim, void *state, int64_t n, int nsteps, uint32_t step0,
+                     int sort_every, int32_t *ids, float *ms, int32_t *launches,
+                     int64_t *active_nodes, void *stream);
+
+/* Sort particles by grid block of their stencil base (replaces build_pid's
+   per-block lists, mpm_solver.py:578-588, node_dynamic.h:61-90); permutes
+   the state and ids in place through the workspace. */
+int aq_sort(aq_sim *sim, void *state, int64_t n, int32_t *ids, void *stream);
+
+/* Per-component max and min over particles, accumulated into
+   mx_mn[0..ncomp) (max) and mx_mn[ncomp..2 ncomp) (min), device f32;
+   replaces update_limits_* (mpm_solver.py:960-979). */
+int aq_update_limits(aq_sim *sim, const void *state, int64_t n, float *mx_mn,
+                     void *stream);
+/* compute_ek (mpm_solver.py:1043-1052) reduced in f64 into *out (device). */
+int aq_energy(aq_sim *sim, const void *state, int64_t n, double *out, void *stream);
+
+/* Device pointer of the grid node buffer holding the last grid velocities
+   (float4 or double4 per node, blocked layout) and its node count; for tests
+   and the adjoint. */
+int aq_sim_grid(aq_sim *sim, void **grid, int64_t *nodes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AQ_MPM_H */

@@ -16,21 +16,29 @@
 #define SFX f32
 #define FLOOR floorf
 #define SQRT sqrtf
+#define FMAX fmaxf
+#define FMIN fminf
 #include "orc_mpm_impl.h"
 #undef REAL
 #undef SFX
 #undef FLOOR
 #undef SQRT
+#undef FMAX
+#undef FMIN
 
 #define REAL double
 #define SFX f64
 #define FLOOR floor
 #define SQRT sqrt
+#define FMAX fmax
+#define FMIN fmin
 #include "orc_mpm_impl.h"
 #undef REAL
 #undef SFX
 #undef FLOOR
 #undef SQRT
+#undef FMAX
+#undef FMIN
 
 int orc_num_comps(const orc_mpm *s) {
   int d = s->dim;
@@ -42,94 +50,70 @@ static int64_t num_nodes(const orc_mpm *s) {
   return s->dim == 3 ? n * n * n : n * n;
 }
 
-/* P2G split over threads: each thread scatters a contiguous particle range
-   into a private grid, the grids are then summed node by node.  Used only to
-   time the CPU baseline; the single-thread path is the parity oracle. */
-#define DEFINE_PAR_P2G(REALT, SFXN)                                               \
-  static void par_p2g_##SFXN(const orc_mpm *s, const REALT *st, int64_t n,        \
-                             REALT *gm, REALT *gv, int nthreads) {                \
-    int64_t nodes = num_nodes(s);                                                 \
-    int d = s->dim;                                                               \
-    if (nthreads <= 1) {                                                          \
-      orc_p2g_##SFXN(s, st, n, gm, gv);                                           \
-      return;                                                                     \
-    }                                                                             \
-    REALT *priv = (REALT *)calloc((size_t)nthreads * (size_t)nodes * (size_t)(d + 1), \
-                                  sizeof(REALT));                                 \
-    _Pragma("omp parallel num_threads(nthreads)") {                               \
-      int t = 0, nt = 1;                                                          \
-      SET_T                                                                       \
-      int64_t lo = n * t / nt, hi = n * (t + 1) / nt;                             \
-      REALT *pm = priv + (size_t)t * (size_t)nodes * (size_t)(d + 1);             \
-      REALT *pv = pm + nodes;                                                     \
-      p2g_range_##SFXN(s, st, n, lo, hi, pm, pv);                                 \
-    }                                                                             \
-    _Pragma("omp parallel for num_threads(nthreads)")                             \
-    for (int64_t i = 0; i < nodes; i++) {                                         \
-      for (int c = 0; c <= d; c++) {                                              \
-        REALT acc = 0;                                                            \
-        for (int t = 0; t < nthreads; t++)                                        \
-          acc += priv[(size_t)t * (size_t)nodes * (size_t)(d + 1) + (size_t)c * nodes + i]; \
-        if (c == 0) gm[i] += acc; else gv[(int64_t)(c - 1) * nodes + i] += acc;   \
-      }                                                                           \
-    }                                                                             \
-    free(priv);                                                                   \
-  }
-
 #ifdef _OPENMP
-#define SET_T t = omp_get_thread_num(); nt = omp_get_num_threads();
+#define THREAD_ID() omp_get_thread_num()
+#define THREAD_COUNT() omp_get_num_threads()
 #else
-#define SET_T
+#define THREAD_ID() 0
+#define THREAD_COUNT() 1
 #endif
 
-/* P2G over particles [lo, hi): the oracle body with a sub-range.  Implemented
-   by offsetting: the phase functions take (state, n) with stride n, so a
-   range is handled by a view that keeps the stride. */
-#define DEFINE_RANGE(REALT, SFXN)                                                 \
-  static void p2g_range_##SFXN(const orc_mpm *s, const REALT *st, int64_t n,      \
-                               int64_t lo, int64_t hi, REALT *gm, REALT *gv) {    \
-    int nc = orc_num_comps(s);                                                    \
-    int64_t m = hi - lo;                                                          \
-    if (m <= 0) return;                                                           \
-    REALT *tmp = (REALT *)malloc(sizeof(REALT) * (size_t)nc * (size_t)m);         \
-    for (int c = 0; c < nc; c++)                                                  \
-      memcpy(tmp + (size_t)c * m, st + (size_t)c * n + lo, sizeof(REALT) * (size_t)m); \
-    orc_p2g_##SFXN(s, tmp, m, gm, gv);                                            \
-    free(tmp);                                                                    \
+/* Multi-threaded phases, used only to time the CPU baseline (the
+   single-thread path is the parity oracle).  P2G: each thread scatters a
+   contiguous particle range into a private grid kept across calls; the
+   private grids are then summed node by node. */
+static void *g_priv = NULL;
+static size_t g_priv_bytes = 0;
+
+static void *priv_buffer(size_t bytes) {
+  if (bytes > g_priv_bytes) {
+    free(g_priv);
+    g_priv = malloc(bytes);
+    g_priv_bytes = g_priv ? bytes : 0;
+  }
+  return g_priv;
+}
+
+#define DEFINE_PAR(REALT, SFXN)                                                             \
+  static void par_p2g_##SFXN(const orc_mpm *s, const REALT *st, int64_t n, REALT *gm,       \
+                             REALT *gv, int nthreads) {                                     \
+    const int64_t nodes = num_nodes(s);                                                     \
+    const int d = s->dim;                                                                   \
+    if (nthreads <= 1) {                                                                    \
+      orc_p2g_##SFXN(s, st, n, gm, gv);                                                     \
+      return;                                                                               \
+    }                                                                                       \
+    const size_t per = (size_t)nodes * (size_t)(d + 1);                                     \
+    REALT *priv = (REALT *)priv_buffer((size_t)nthreads * per * sizeof(REALT));             \
+    _Pragma("omp parallel num_threads(nthreads)") {                                         \
+      const int t = THREAD_ID(), nt = THREAD_COUNT();                                       \
+      REALT *pm = priv + (size_t)t * per;                                                   \
+      memset(pm, 0, per * sizeof(REALT));                                                   \
+      p2g_span_##SFXN(s, st, n, n * t / nt, n * (t + 1) / nt, pm, pm + nodes);              \
+    }                                                                                       \
+    _Pragma("omp parallel for num_threads(nthreads)")                                       \
+    for (int64_t i = 0; i < nodes; i++) {                                                   \
+      for (int c = 0; c <= d; c++) {                                                        \
+        REALT acc = 0;                                                                      \
+        for (int t = 0; t < nthreads; t++) acc += priv[(size_t)t * per + (size_t)c * nodes + i]; \
+        if (c == 0) gm[i] += acc; else gv[(int64_t)(c - 1) * nodes + i] += acc;             \
+      }                                                                                     \
+    }                                                                                       \
+  }                                                                                         \
+  static void par_g2p_##SFXN(const orc_mpm *s, const REALT *st, const REALT *gv, REALT *out, \
+                             int64_t n, int nthreads) {                                     \
+    if (nthreads <= 1) {                                                                    \
+      orc_g2p_##SFXN(s, st, gv, out, n);                                                    \
+      return;                                                                               \
+    }                                                                                       \
+    _Pragma("omp parallel num_threads(nthreads)") {                                         \
+      const int t = THREAD_ID(), nt = THREAD_COUNT();                                       \
+      g2p_span_##SFXN(s, st, gv, out, n, n * t / nt, n * (t + 1) / nt);                     \
+    }                                                                                       \
   }
 
-DEFINE_RANGE(float, f32)
-DEFINE_RANGE(double, f64)
-DEFINE_PAR_P2G(float, f32)
-DEFINE_PAR_P2G(double, f64)
-
-#define DEFINE_PAR_G2P(REALT, SFXN)                                               \
-  static void par_g2p_##SFXN(const orc_mpm *s, const REALT *st, const REALT *gv,  \
-                             REALT *out, int64_t n, int nthreads) {               \
-    if (nthreads <= 1) {                                                          \
-      orc_g2p_##SFXN(s, st, gv, out, n);                                          \
-      return;                                                                     \
-    }                                                                             \
-    int nc = orc_num_comps(s);                                                    \
-    _Pragma("omp parallel num_threads(nthreads)") {                               \
-      int t = 0, nt = 1;                                                          \
-      SET_T                                                                       \
-      int64_t lo = n * t / nt, hi = n * (t + 1) / nt, m = hi - lo;                \
-      if (m > 0) {                                                                \
-        REALT *a = (REALT *)malloc(sizeof(REALT) * (size_t)nc * (size_t)m * 2);   \
-        REALT *b = a + (size_t)nc * m;                                            \
-        for (int c = 0; c < nc; c++)                                              \
-          memcpy(a + (size_t)c * m, st + (size_t)c * n + lo, sizeof(REALT) * (size_t)m); \
-        orc_g2p_##SFXN(s, a, gv, b, m);                                           \
-        for (int c = 0; c < nc; c++)                                              \
-          memcpy(out + (size_t)c * n + lo, b + (size_t)c * m, sizeof(REALT) * (size_t)m); \
-        free(a);                                                                  \
-      }                                                                           \
-    }                                                                             \
-  }
-
-DEFINE_PAR_G2P(float, f32)
-DEFINE_PAR_G2P(double, f64)
+DEFINE_PAR(float, f32)
+DEFINE_PAR(double, f64)
 
 void orc_run_plain_f32(const orc_mpm *s, float *state, int64_t n, int nsteps,
                        float *grid_m, float *grid_v, int nthreads) {
@@ -165,8 +149,8 @@ void orc_run_quant_f32(const orc_mpm *s, const orc_comp *comps, int nwords,
                        float *scratch, int nthreads) {
   int nc = orc_num_comps(s);
   float *st = scratch, *out = scratch + (size_t)nc * (size_t)n;
+  const int nt = nthreads < 1 ? 1 : nthreads;
   for (int t = 0; t < nsteps; t++) {
-    int nt = nthreads < 1 ? 1 : nthreads;
     _Pragma("omp parallel for num_threads(nt)")
     for (int k = 0; k < nt; k++)
       orc_unpack_range_f32(comps, nc, nwords, words, st, n, n * k / nt, n * (k + 1) / nt);

@@ -21,11 +21,10 @@
 static int FN(base_of)(REAL xs, REAL inv_dx, int n) {
   REAL t = xs * inv_dx;               /* x * inv_dx */
   t = t - (REAL)0.5;                  /* - 0.5 */
-  int b = (int)FLOOR(t);              /* floor(...).cast(int) */
-  if (t != t) b = 0;
-  if (b < 0) b = 0;
-  if (b > n - 3) b = n - 3;
-  return b;
+  /* clamp before floor (NaN -> 0): identical to floor-then-clamp for finite x */
+  t = FMAX(t, (REAL)0);
+  t = FMIN(t, (REAL)(n - 3));
+  return (int)FLOOR(t);               /* floor(...).cast(int) */
 }
 
 static void FN(weights)(REAL fx, REAL w[3]) {
@@ -79,11 +78,13 @@ static void FN(affine)(const orc_mpm *s, const REAL *st, int64_t n, int64_t p,
     aff[i] = k * cauchy[i] + pm * st[(int64_t)(cC + i) * n + p];
 }
 
-void FN(orc_p2g)(const orc_mpm *s, const REAL *st, int64_t n, REAL *gm, REAL *gv) {
+/* P2G of particles [lo, hi) of a state with plane stride n */
+static void FN(p2g_span)(const orc_mpm *s, const REAL *st, int64_t n, int64_t lo, int64_t hi,
+                         REAL *gm, REAL *gv) {
   const int d = s->dim, ng = s->n_grid;
   const int64_t nodes = d == 3 ? (int64_t)ng * ng * ng : (int64_t)ng * ng;
   const REAL inv_dx = (REAL)s->inv_dx, dx = (REAL)s->dx, pm = (REAL)s->p_mass;
-  for (int64_t p = 0; p < n; p++) {
+  for (int64_t p = lo; p < hi; p++) {
     int base[3] = {0, 0, 0};
     REAL fx[3] = {0, 0, 0}, w[3][3];
     for (int a = 0; a < d; a++) {
@@ -115,6 +116,10 @@ void FN(orc_p2g)(const orc_mpm *s, const REAL *st, int64_t n, REAL *gm, REAL *gv
           gm[node] += weight * pm;
         }
   }
+}
+
+void FN(orc_p2g)(const orc_mpm *s, const REAL *st, int64_t n, REAL *gm, REAL *gv) {
+  FN(p2g_span)(s, st, n, 0, n, gm, gv);
 }
 
 void FN(orc_grid_op)(const orc_mpm *s, const REAL *gm, REAL *gv) {
@@ -153,13 +158,15 @@ void FN(orc_grid_op)(const orc_mpm *s, const REAL *gm, REAL *gv) {
   }
 }
 
-void FN(orc_g2p)(const orc_mpm *s, const REAL *st, const REAL *gv, REAL *out, int64_t n) {
+/* G2P of particles [lo, hi) of a state with plane stride n */
+static void FN(g2p_span)(const orc_mpm *s, const REAL *st, const REAL *gv, REAL *out, int64_t n,
+                         int64_t lo, int64_t hi) {
   const int d = s->dim, ng = s->n_grid;
   const int64_t nodes = d == 3 ? (int64_t)ng * ng * ng : (int64_t)ng * ng;
   const REAL inv_dx = (REAL)s->inv_dx, dt = (REAL)s->dt;
   const REAL c4 = (REAL)(4 * s->inv_dx);
   const int cC = 2 * d, cJF = 2 * d + d * d;
-  for (int64_t p = 0; p < n; p++) {
+  for (int64_t p = lo; p < hi; p++) {
     int base[3] = {0, 0, 0};
     REAL fx[3] = {0, 0, 0}, w[3][3];
     for (int a = 0; a < d; a++) {
@@ -212,6 +219,10 @@ void FN(orc_g2p)(const orc_mpm *s, const REAL *st, const REAL *gv, REAL *out, in
         }
     }
   }
+}
+
+void FN(orc_g2p)(const orc_mpm *s, const REAL *st, const REAL *gv, REAL *out, int64_t n) {
+  FN(g2p_span)(s, st, gv, out, n, 0, n);
 }
 
 double FN(orc_energy)(const orc_mpm *s, const REAL *st, int64_t n) {

@@ -1,0 +1,119 @@
+"""CPU tests of the C ABI: the library loads, exports every function declared
+in include/aq_mpm.h, and the host-side layout planner follows the
+reference's placement rules (TS/taichi/struct/struct_llvm.cpp:71-116,
+TS/taichi/program/snode_expr_utils.cpp:44-89).  No GPU is touched."""
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(ROOT, "include", "aq_mpm.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(aq_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_header():
+    from autoquantizer_amd import _lib
+    L = _lib.lib()
+    names = declared_functions()
+    assert len(names) >= 18
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(_lib.EXPORTS) == set(names)
+    assert b"gfx950" in L.aq_version()
+
+
+def q():
+    import autoquantizer_amd as aq
+    return aq
+
+
+def test_plan_bit_pack_assign_layout():
+    """test_bit_pack_assign.py:12-17: fixed21 x3 + int15 in a bit_pack: offsets
+    0, 21, 42, 63 (the int straddles into word 1); two words."""
+    aq = q()
+    from autoquantizer_amd import _lib
+    t = aq.quant.fixed(frac=21, range=2)
+    comps, nw = aq.plan([(_lib.AQ_BIT_PACK, 0)],
+                        [dict(t.place_record(), container=0, shared_group=-1)] * 3 +
+                        [dict(kind=_lib.AQ_INT, num_bits=15, is_signed=0, exp_bits=0, use_dithering=0,
+                              scale=1.0, offset=0.0, container=0, shared_group=-1)])
+    assert [c["bit_offset"] for c in comps] == [0, 21, 42, 63]
+    assert nw == 2
+    assert comps[0]["scale"] == 2 / 2 ** 20
+
+
+def test_plan_shared_exponent_first():
+    """Shared exponent placed before its first user; members follow."""
+    aq = q()
+    from autoquantizer_amd import _lib
+    ft = aq.quant.float(exp=6, frac=13)
+    recs = [dict(ft.place_record(), container=0, shared_group=0) for _ in range(4)]
+    comps, nw = aq.plan([(_lib.AQ_BIT_STRUCT, 64)], recs)
+    assert [c["exp_offset"] for c in comps] == [0] * 4
+    assert [c["bit_offset"] for c in comps] == [6, 19, 32, 45]
+    assert all(c["shared"] == 1 for c in comps)
+    assert nw == 1
+
+
+def test_plan_non_shared_floats_own_exponent():
+    aq = q()
+    from autoquantizer_amd import _lib
+    ft = aq.quant.float(exp=6, frac=13)
+    recs = [dict(ft.place_record(), container=0, shared_group=-1) for _ in range(2)]
+    comps, nw = aq.plan([(_lib.AQ_BIT_STRUCT, 64)], recs)
+    assert [(c["exp_offset"], c["bit_offset"]) for c in comps] == [(0, 6), (19, 25)]
+    assert all(c["shared"] == 0 for c in comps)
+
+
+def test_plan_bit_struct_overflow_rejected():
+    aq = q()
+    from autoquantizer_amd import _lib
+    t = aq.quant.fixed(frac=20)
+    with pytest.raises(_lib.AQError):
+        aq.plan([(_lib.AQ_BIT_STRUCT, 32)], [dict(t.place_record(), container=0, shared_group=-1)] * 2)
+
+
+def test_plan_containers_align():
+    """bit_struct(32) then bit_struct(64): the 64-bit container starts on a word."""
+    aq = q()
+    from autoquantizer_amd import _lib
+    t = aq.quant.fixed(frac=16)
+    recs = [dict(t.place_record(), container=0, shared_group=-1),
+            dict(t.place_record(), container=1, shared_group=-1)]
+    comps, nw = aq.plan([(_lib.AQ_BIT_STRUCT, 32), (_lib.AQ_BIT_STRUCT, 64)], recs)
+    assert [c["bit_offset"] for c in comps] == [0, 64]
+    assert nw == 2
+
+
+def test_config_layouts():
+    """The two benchmark layouts: 2D elastic 24 B, 3D fluid 32 B per particle."""
+    from autoquantizer_amd.mpm import component_names, plan_state
+    from autoquantizer_amd import layouts
+    names, types, groups, shared = layouts.elastic_2d()
+    comps, nw = plan_state(names, types, groups, shared)
+    assert nw == 3
+    names, types, groups, shared = layouts.fluid_3d()
+    comps, nw = plan_state(names, types, groups, shared)
+    assert nw == 4
+    assert comps[0]["bit_offset"] == 0 and comps[2]["bit_offset"] == 42
+
+
+def test_sim_desc_validation():
+    """Bad descriptors are rejected on the host with a message."""
+    import ctypes
+    from autoquantizer_amd import _lib
+    L = _lib.lib()
+    d = _lib.SimDesc()
+    d.dim, d.material, d.n_grid, d.ncomp, d.storage = 3, _lib.AQ_ELASTIC, 64, 24, _lib.AQ_STORE_F32
+    assert L.aq_sim_workspace_bytes(ctypes.byref(d)) < 0
+    assert b"2D" in L.aq_last_error()
+    d.material, d.ncomp = _lib.AQ_FLUID, 16
+    d.n_grid = 66
+    assert L.aq_sim_workspace_bytes(ctypes.byref(d)) < 0
+    d.n_grid, d.capacity = 64, 1000
+    assert L.aq_sim_workspace_bytes(ctypes.byref(d)) > 64 ** 3 * 16 * 2

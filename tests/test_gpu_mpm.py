@@ -1,0 +1,244 @@
+"""GPU parity of the MPM substep kernels with the C oracle.
+
+- P2G grid (mass, momentum) within f32 accumulation-order tolerance;
+- G2P from the GPU's own grid: bit-exact with the oracle's G2P on that grid
+  (both unfused, same operand order);
+- whole quantized substeps: fixed-point codes within one code of the
+  oracle's, shared-exponent members within a few digits;
+- the fused G2P->P2G path against separate substeps;
+- the plain fp32 path against the reference regression test_mpm88.py:97-104;
+- sort, limits and energy.
+"""
+import numpy as np
+import pytest
+from pytest import approx
+
+from gpu_util import state_layout, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def scene(orc, kind, n, seed=0, storage="quant"):
+    """Random blob scenes for the two benchmark materials."""
+    _torch()
+    import autoquantizer_amd as aq
+    from autoquantizer_amd import layouts
+    cfg = aq.MPMConfig()
+    rng = np.random.default_rng(seed)
+    if kind == "elastic_2d":
+        cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.E, cfg.nu, cfg.sticky = 2, False, 128, 1e-4, 1e4, 0.2, True
+        names, t, g, sh = layouts.elastic_2d()
+        s = orc.mpm_params(2, orc.ELASTIC, 128, 1e-4, E=1e4, nu=0.2, boundary=orc.STICKY, gravity=9.8)
+        lo, hi = 0.3, 0.5
+    else:
+        cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.la = 3, True, 64, 2e-4, 400.0
+        names, t, g, sh = layouts.fluid_3d()
+        s = orc.mpm_params(3, orc.FLUID, 64, 2e-4, la=400.0, gravity=9.8)
+        lo, hi = 0.1, 0.5
+    cfg.p_vol = (1 / cfg.n_grid * 0.5) ** cfg.dim
+    cfg.n_particles = n
+    cfg.compute_fp = "f32"
+    cfg.seed = 11
+    if storage == "quant":
+        sim = aq.MPMSolver(cfg, types=t, groups=g, shared=sh)
+    else:
+        cfg.compute_fp = storage
+        sim = aq.MPMSolver(cfg)
+    d = cfg.dim
+    vals = np.zeros((len(names), n))
+    vals[:d] = rng.uniform(lo, hi, (d, n))
+    vals[d:2 * d] = rng.uniform(-1, 1, (d, n))
+    vals[2 * d:2 * d + d * d] = rng.uniform(-3, 3, (d * d, n))
+    if kind == "elastic_2d":
+        vals[-4:] = np.eye(2).reshape(-1, 1) + rng.uniform(-0.05, 0.05, (4, n))
+    else:
+        vals[-1] = rng.uniform(0.95, 1.05, n)
+    sim.set_state(vals)
+    return sim, s
+
+
+def dense_to_oracle(g, d):
+    return np.ascontiguousarray(g[d]), np.ascontiguousarray(g[:d])
+
+
+@pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d"])
+def test_p2g_grid(orc, kind):
+    print(f"[p2g] {kind}", flush=True)
+    sim, s = scene(orc, kind, 4000, seed=1)
+    st = sim.get_state().astype(np.float32)
+    sim.p2g_bls()
+    g = sim.grid_dense()
+    gm, gv = orc.grids(s)
+    orc.p2g(s, st, gm, gv)
+    d = s.dim
+    assert g[d] == approx(gm, rel=2e-5, abs=1e-6 * s.p_mass)
+    mom_scale = np.abs(gv).max()
+    assert np.abs(g[:d] - gv).max() <= 2e-5 * mom_scale
+
+
+@pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d"])
+def test_g2p_exact_on_gpu_grid(orc, kind):
+    """G2P is per-particle deterministic: with the GPU grid as input the oracle
+    G2P + encode gives the same words as the GPU's G2P."""
+    print(f"[g2p] {kind}", flush=True)
+    sim, s = scene(orc, kind, 4000, seed=2)
+    st = sim.get_state(original_order=False).astype(np.float32)
+    sim.p2g_bls()
+    sim.grid_op()
+    g = sim.grid_dense()
+    sim.g2p_bls()
+    words_gpu = sim.get_words()
+    gv = np.ascontiguousarray(g[:s.dim]).astype(np.float32)
+    out = orc.g2p(s, st, gv)
+    words_orc = orc.pack(to_oracle(sim.comps), sim.nwords, out, seed=11, step=0)
+    assert np.array_equal(words_gpu, words_orc)
+
+
+@pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d"])
+def test_grid_op_exact(orc, kind):
+    """Grid update on the GPU's P2G grid equals the oracle's grid_op exactly."""
+    sim, s = scene(orc, kind, 3000, seed=3)
+    sim.p2g_bls()
+    g0 = sim.grid_dense()
+    sim.grid_op()
+    g1 = sim.grid_dense()
+    d = s.dim
+    gm = np.ascontiguousarray(g0[d]).astype(np.float32)
+    gv = np.ascontiguousarray(g0[:d]).astype(np.float32)
+    orc.grid_op(s, gm, gv)
+    # only nodes in active blocks are updated on the GPU; all touched nodes are active
+    touched = g0[d] != 0
+    assert np.array_equal(g1[:d][:, touched].astype(np.float32), gv[:, touched])
+
+
+def code_diff(orc, comps, nw, a_words, b_words):
+    """Per-component decoded difference in units of the fixed-point step."""
+    oc = to_oracle(comps)
+    a = orc.unpack(oc, nw, a_words)
+    b = orc.unpack(oc, nw, b_words)
+    return a, b
+
+
+@pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d"])
+def test_substeps_vs_oracle(orc, kind):
+    """10 quantized substeps on the GPU and in the oracle from the same words:
+    fixed-point members agree to within 2 codes, custom floats to 1e-3 of the
+    field's magnitude (only the P2G summation order differs)."""
+    print(f"[substeps] {kind}", flush=True)
+    sim, s = scene(orc, kind, 3000, seed=4)
+    w0 = sim.get_words()
+    for _ in range(10):
+        sim.substep()
+    wg = sim.get_words()
+    wo = w0.copy()
+    oc = to_oracle(sim.comps)
+    orc.run_quant(s, oc, sim.nwords, wo, 10, seed=11, step0=0)
+    a = orc.unpack(oc, sim.nwords, wg).astype(np.float64)
+    b = orc.unpack(oc, sim.nwords, wo).astype(np.float64)
+    for k, c in enumerate(sim.comps):
+        if c["kind"] == 0:
+            assert np.abs(a[k] - b[k]).max() <= 2.0 * c["scale"] + 1e-12, (k, c)
+        else:
+            tol = 1e-3 * max(np.abs(b[k]).max(), 1e-6)
+            assert np.abs(a[k] - b[k]).max() <= tol, k
+
+
+def test_fused_advance_matches_substeps(orc):
+    """advance(n) (G2P fused with the next P2G) vs n separate substeps."""
+    sim_a, s = scene(orc, "fluid_3d", 5000, seed=5)
+    sim_b, _ = scene(orc, "fluid_3d", 5000, seed=5)
+    for _ in range(8):
+        sim_a.substep()
+    sim_b.advance(8)
+    a = sim_a.get_state()
+    b = sim_b.get_state()
+    for k, c in enumerate(sim_a.comps):
+        if c["kind"] == 0:
+            assert np.abs(a[k] - b[k]).max() <= 2.0 * c["scale"] + 1e-12, k
+        else:
+            assert np.abs(a[k] - b[k]).max() <= 1e-3 * max(np.abs(a[k]).max(), 1e-6), k
+
+
+def test_fused_advance_with_sort(orc):
+    """Sorting between fused substeps permutes particles but not physics."""
+    sim_a, s = scene(orc, "fluid_3d", 5000, seed=6)
+    sim_b, _ = scene(orc, "fluid_3d", 5000, seed=6)
+    sim_a.advance(6)
+    sim_b.advance(6, sort_every=2)
+    a = sim_a.get_state()
+    b = sim_b.get_state()
+    for k, c in enumerate(sim_a.comps):
+        if c["kind"] == 0:
+            assert np.abs(a[k] - b[k]).max() <= 3.0 * c["scale"] + 1e-12, k
+
+
+def test_sort_is_permutation(orc):
+    torch = _torch()
+    sim, s = scene(orc, "fluid_3d", 7000, seed=7)
+    w0 = sim.get_words(original_order=True)
+    sim.build_pid()
+    ids = sim.ids[:7000].cpu().numpy()
+    assert np.array_equal(np.sort(ids), np.arange(7000))
+    assert np.array_equal(sim.get_words(original_order=True), w0)
+    # sorted by grid block of the stencil base
+    st = sim.get_state(original_order=False)
+    base = np.clip(np.floor(st[:3] * 64 - 0.5), 0, 61).astype(int) // 4
+    key = (base[0] * 16 + base[1]) * 16 + base[2]
+    assert np.all(np.diff(key) >= 0)
+
+
+def test_mpm88_regression_gpu():
+    """test_mpm88.py:9-104 on the plain fp32 GPU path."""
+    _torch()
+    import autoquantizer_amd as aq
+    cfg = aq.MPMConfig()
+    cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.la = 2, True, 128, 2e-4, 400.0
+    cfg.p_vol = (1 / 128 * 0.5) ** 2
+    N = 64
+    n = N * N
+    cfg.n_particles = n
+    cfg.compute_fp = "f32"
+    sim = aq.MPMSolver(cfg)
+    i = np.arange(n)
+    vals = np.zeros((9, n))
+    vals[0] = (i % N) / N * 0.4 + 0.2
+    vals[1] = i / N / N * 0.4 + 0.05
+    vals[3] = -3.0
+    vals[8] = 1.0
+    sim.set_state(vals)
+    sim.advance(500)
+    pos = sim.get_state()[:2].T.copy()
+    pos[:, 1] *= 2
+    regression = [0.31722742, 0.15826741, 0.10224003, 0.07810827]
+    for k in range(4):
+        assert (pos ** (k + 1)).mean() == approx(regression[k], rel=1e-2)
+
+
+def test_plain_f64_matches_oracle(orc):
+    """FP64 plain storage: 20 substeps agree with the oracle to 1e-9 relative."""
+    sim, s = scene(orc, "fluid_3d", 3000, seed=8, storage="f64")
+    st = sim.get_state()
+    for _ in range(20):
+        sim.substep()
+    ref = st.copy()
+    orc.run_plain(s, ref, 20)
+    got = sim.get_state()
+    scale = np.abs(ref).max(axis=1, keepdims=True) + 1e-30
+    assert (np.abs(got - ref) / scale).max() < 1e-9
+
+
+def test_limits_and_energy(orc):
+    sim, s = scene(orc, "fluid_3d", 9000, seed=9)
+    st = sim.get_state()
+    sim.detect_bound_kernel()
+    lim = sim.detect_bound()
+    assert lim == approx(np.maximum(np.abs(st.max(axis=1)), np.abs(st.min(axis=1))), rel=1e-6)
+    e = sim.compute_ek()
+    assert e == approx(orc.energy(s, st.astype(np.float32)), rel=1e-6)
