@@ -89,10 +89,10 @@ def lib():
     L.aq_sim_workspace_bytes.argtypes = [P]
     L.aq_sim_create.argtypes = [P, P, i64, ctypes.POINTER(P)]
     L.aq_sim_destroy.argtypes = [P]
-    L.aq_substep.argtypes = [P, P, i64, u32, P]
+    L.aq_substep.argtypes = [P, P, i64, P, u32, P]
     L.aq_p2g.argtypes = [P, P, i64, P]
     L.aq_grid_op.argtypes = [P, P]
-    L.aq_g2p.argtypes = [P, P, i64, u32, P]
+    L.aq_g2p.argtypes = [P, P, i64, P, u32, P]
     L.aq_advance.argtypes = [P, P, i64, i32, u32, i32, P, P]
     L.aq_sort.argtypes = [P, P, i64, P, P]
     L.aq_update_limits.argtypes = [P, P, i64, P, P]

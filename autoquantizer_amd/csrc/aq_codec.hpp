@@ -66,27 +66,28 @@ __device__ __forceinline__ uint32_t get_field(const uint32_t (&r)[NW32], uint32_
   return v & lowmask(nb);
 }
 
-// OR a field into a record whose field bits are zero.
-template <int NW32>
-__device__ __forceinline__ void put_field(uint32_t (&r)[NW32], uint32_t off, uint32_t nb,
-                                          uint32_t v) {
+// Records are encoded in a per-thread row of LDS words (rows of an odd
+// number of words, so the 64 lanes of a wavefront hit distinct banks): a
+// field at a uniform bit offset becomes one or two ds_or_b32 at a uniform
+// word index, instead of a read-modify-write of a register array at a
+// runtime index.
+// OR a field into a row whose field bits are zero.
+__device__ __forceinline__ void row_put(uint32_t *row, uint32_t off, uint32_t nb, uint32_t v) {
   const uint32_t i = off >> 5, b = off & 31u;
   v &= lowmask(nb);
-  r[i] |= v << b;
-  if (b + nb > 32u) r[i + 1] |= v >> (32u - b);
+  atomicOr(row + i, v << b);
+  if (b + nb > 32u) atomicOr(row + i + 1, v >> (32u - b));
 }
 
-// Replace a field (masked store).
-template <int NW32>
-__device__ __forceinline__ void set_field(uint32_t (&r)[NW32], uint32_t off, uint32_t nb,
-                                          uint32_t v) {
+// Replace a field of a row (masked store).
+__device__ __forceinline__ void row_set(uint32_t *row, uint32_t off, uint32_t nb, uint32_t v) {
   const uint32_t i = off >> 5, b = off & 31u;
   const uint32_t m = lowmask(nb);
   v &= m;
-  r[i] = (r[i] & ~(m << b)) | (v << b);
+  row[i] = (row[i] & ~(m << b)) | (v << b);
   if (b + nb > 32u) {
     const uint32_t s = 32u - b;
-    r[i + 1] = (r[i + 1] & ~(m >> s)) | (v >> s);
+    row[i + 1] = (row[i + 1] & ~(m >> s)) | (v >> s);
   }
 }
 
@@ -207,26 +208,65 @@ __device__ __forceinline__ float dec_shared(uint32_t e, uint32_t q, const CompK 
   return __uint_as_float(fb | sign);
 }
 
-// Decode component c of a record.
-template <int NW32>
-__device__ __forceinline__ float decode_comp(const uint32_t (&r)[NW32], const CompK &c) {
-  if (c.kind == K_FIXED) return dec_fixed(get_field(r, c.off, c.nbits), c);
-  if (c.kind == K_F32) return __uint_as_float(get_field(r, c.off, 32u));
+// Decode a member given a field reader get(off, nbits).
+template <class G>
+__device__ __forceinline__ float decode_with(G &&get, const CompK &c) {
+  if (c.kind == K_FIXED) return dec_fixed(get(c.off, c.nbits), c);
+  if (c.kind == K_F32) return __uint_as_float(get(c.off, 32u));
   if (c.kind == K_INT) {
-    const uint32_t q = get_field(r, c.off, c.nbits);
+    const uint32_t q = get(c.off, c.nbits);
     return c.is_signed ? (float)sext(q, c.nbits) : (float)q;
   }
-  const uint32_t e = get_field(r, c.eoff, c.exp_bits);
-  const uint32_t q = get_field(r, c.off, c.nbits);
+  const uint32_t e = get(c.eoff, c.exp_bits);
+  const uint32_t q = get(c.off, c.nbits);
   return c.shared ? dec_shared(e, q, c) : dec_float(e, q, c);
 }
 
+// Decode component c of a record held in registers.
+template <int NW32>
+__device__ __forceinline__ float decode_comp(const uint32_t (&r)[NW32], const CompK &c) {
+  return decode_with([&](uint32_t off, uint32_t nb) { return get_field(r, off, nb); }, c);
+}
+
+// Field of a record held in an LDS row (one or two ds_read_b32 at a uniform index).
+__device__ __forceinline__ uint32_t row_get(const uint32_t *row, uint32_t off, uint32_t nb) {
+  const uint32_t i = off >> 5, b = off & 31u;
+  const uint32_t v = (b + nb > 32u) ? __builtin_amdgcn_alignbit(row[i + 1], row[i], b) : (row[i] >> b);
+  return v & lowmask(nb);
+}
+
+__device__ __forceinline__ float decode_row(const uint32_t *row, const CompK &c) {
+  return decode_with([&](uint32_t off, uint32_t nb) { return row_get(row, off, nb); }, c);
+}
+
+// Encode one member (fixed, own-exponent float, f32 or int) into a row.
+__device__ __forceinline__ void encode_member(float v, const CompK &k, float u, uint32_t *row, bool masked) {
+  if (k.kind == K_FIXED) {
+    const uint32_t q = enc_fixed(v, k, u);
+    if (masked) row_set(row, k.off, k.nbits, q); else row_put(row, k.off, k.nbits, q);
+  } else if (k.kind == K_FLOAT) {
+    uint32_t e;
+    const uint32_t d = enc_float(v, k, e);
+    if (masked) {
+      row_set(row, k.eoff, k.exp_bits, e);
+      row_set(row, k.off, k.nbits, d);
+    } else {
+      row_put(row, k.eoff, k.exp_bits, e);
+      row_put(row, k.off, k.nbits, d);
+    }
+  } else if (k.kind == K_F32) {
+    if (masked) row_set(row, k.off, 32u, __float_as_uint(v)); else row_put(row, k.off, 32u, __float_as_uint(v));
+  } else {
+    const uint32_t q = cvt_sat(v, k.is_signed);
+    if (masked) row_set(row, k.off, k.nbits, q); else row_put(row, k.off, k.nbits, q);
+  }
+}
+
 // Encode NC values (compile-time count, components lay.c[0..NC)) into a
-// zeroed record.  keys[c] is the dithering key of component c.
-template <int NC, int NW32>
+// zeroed LDS row.  keys[c] is the dithering key of component c.
+template <int NC>
 __device__ __forceinline__ void encode_record(const float (&v)[NC], const LayoutK &lay,
-                                              const uint32_t *keys, uint32_t slot,
-                                              uint32_t (&r)[NW32]) {
+                                              const uint32_t *keys, uint32_t slot, uint32_t *row) {
   uint32_t gmax[NC];
   uint32_t run = 0u;
 #pragma unroll
@@ -250,23 +290,12 @@ __device__ __forceinline__ void encode_record(const float (&v)[NC], const Layout
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const CompK &k = lay.c[c];
-    if (k.kind == K_FIXED) {
-      const float u = k.dither ? noise_u(slot, keys[c]) : 0.0f;
-      put_field(r, k.off, k.nbits, enc_fixed(v[c], k, u));
-    } else if (k.kind == K_FLOAT) {
-      if (k.shared) {
-        if (k.gfirst) put_field(r, k.eoff, k.exp_bits, shared_exp_field(gmax[c], k));
-        put_field(r, k.off, k.nbits, enc_shared(v[c], gmax[c], k));
-      } else {
-        uint32_t e;
-        const uint32_t d = enc_float(v[c], k, e);
-        put_field(r, k.eoff, k.exp_bits, e);
-        put_field(r, k.off, k.nbits, d);
-      }
-    } else if (k.kind == K_F32) {
-      put_field(r, k.off, 32u, __float_as_uint(v[c]));
+    if (k.kind == K_FLOAT && k.shared) {
+      if (k.gfirst) row_put(row, k.eoff, k.exp_bits, shared_exp_field(gmax[c], k));
+      row_put(row, k.off, k.nbits, enc_shared(v[c], gmax[c], k));
     } else {
-      put_field(r, k.off, k.nbits, cvt_sat(v[c], k.is_signed));
+      const float u = (k.kind == K_FIXED && k.dither) ? noise_u(slot, keys[c]) : 0.0f;
+      encode_member(v[c], k, u, row, false);
     }
   }
 }

@@ -122,50 +122,44 @@ static int nw32_bucket(int nw) { return nw <= 2 ? 4 : nw <= 4 ? 8 : nw <= 8 ? 16
 // ---------------------------------------------------------------------------
 // codec kernels
 // ---------------------------------------------------------------------------
+// Per-thread LDS rows for encoding (odd stride: distinct banks per lane).
+template <int NW32> struct Rows {
+  static constexpr int RS = NW32 + 1;
+};
+
 template <int NW32>
 __global__ void __launch_bounds__(256) k_pack(LayoutK lay, KeysK keys, const float *__restrict__ vals,
                                               const float *__restrict__ noise, uint64_t *__restrict__ words,
                                               int64_t n) {
+  __shared__ uint32_t rows[256 * Rows<NW32>::RS];
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  uint32_t r[NW32];
+  uint32_t *row = rows + threadIdx.x * Rows<NW32>::RS;
 #pragma unroll
-  for (int i = 0; i < NW32; ++i) r[i] = 0u;
+  for (int i = 0; i < NW32; ++i) row[i] = 0u;
   uint32_t gmax = 0u;
-  // generic component count: shared-group maxima need the whole group first
   for (int c = 0; c < lay.nc; ++c) {
     const CompK &k = lay.c[c];
     const float v = vals[(int64_t)c * n + p];
-    if (k.kind == K_FIXED) {
-      float u = 0.0f;
-      if (k.dither) u = noise ? noise[(int64_t)c * n + p] : noise_u((uint32_t)p, keys.k[c]);
-      put_field(r, k.off, k.nbits, enc_fixed(v, k, u));
-    } else if (k.kind == K_FLOAT) {
-      if (k.shared) {
-        if (k.gfirst) {
-          gmax = 0u;
-          for (int j = c; j < lay.nc; ++j) {
-            const uint32_t e = f32_exp(vals[(int64_t)j * n + p]);
-            gmax = e > gmax ? e : gmax;
-            if (lay.c[j].glast) break;
-          }
-          put_field(r, k.eoff, k.exp_bits, shared_exp_field(gmax, k));
+    if (k.kind == K_FLOAT && k.shared) {
+      if (k.gfirst) {
+        gmax = 0u;
+        for (int j = c; j < lay.nc; ++j) {
+          const uint32_t e = f32_exp(vals[(int64_t)j * n + p]);
+          gmax = e > gmax ? e : gmax;
+          if (lay.c[j].glast) break;
         }
-        put_field(r, k.off, k.nbits, enc_shared(v, gmax, k));
-      } else {
-        uint32_t e;
-        const uint32_t d = enc_float(v, k, e);
-        put_field(r, k.eoff, k.exp_bits, e);
-        put_field(r, k.off, k.nbits, d);
+        row_put(row, k.eoff, k.exp_bits, shared_exp_field(gmax, k));
       }
-    } else if (k.kind == K_F32) {
-      put_field(r, k.off, 32u, __float_as_uint(v));
+      row_put(row, k.off, k.nbits, enc_shared(v, gmax, k));
     } else {
-      put_field(r, k.off, k.nbits, cvt_sat(v, k.is_signed));
+      float u = 0.0f;
+      if (k.kind == K_FIXED && k.dither) u = noise ? noise[(int64_t)c * n + p] : noise_u((uint32_t)p, keys.k[c]);
+      encode_member(v, k, u, row, false);
     }
   }
   for (int i = 0; i < lay.nw; ++i)
-    words[(int64_t)i * n + p] = (uint64_t)r[2 * i] | ((uint64_t)r[2 * i + 1] << 32);
+    words[(int64_t)i * n + p] = (uint64_t)row[2 * i] | ((uint64_t)row[2 * i + 1] << 32);
 }
 
 template <int NW32>
@@ -186,18 +180,22 @@ __global__ void __launch_bounds__(256) k_unpack(LayoutK lay, const uint64_t *__r
 
 // Masked store of the components selected by `mask` (bit c = component c)
 // into existing records: vals[j * count + t] is the value of the j-th
-// selected component for record idx[t].  Unselected members of a
-// shared-exponent group that has a selected member are reconstructed from
-// the record and re-encoded with the group's new exponent.
+// selected component for record idx[t].  Every member of a shared-exponent
+// group with a selected member is re-encoded with the group's new exponent,
+// the unselected ones from their decoded stored values.
 template <int NW32>
 __global__ void __launch_bounds__(256) k_store_members(LayoutK lay, KeysK keys, uint32_t mask,
                                                        uint64_t *__restrict__ words, int64_t n,
                                                        const int64_t *__restrict__ idx,
                                                        const float *__restrict__ vals, int64_t count) {
+  __shared__ uint32_t rows[256 * Rows<NW32>::RS];
+  __shared__ float vvs[256 * (AQ_MAX_COMPS + 1)];
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= count) return;
   const int64_t p = idx[t];
   if (p < 0 || p >= n) return;
+  uint32_t *row = rows + threadIdx.x * Rows<NW32>::RS;
+  float *vv = vvs + threadIdx.x * (AQ_MAX_COMPS + 1);
   uint32_t r[NW32];
 #pragma unroll
   for (int i = 0; i < NW32; ++i) r[i] = 0u;
@@ -206,57 +204,51 @@ __global__ void __launch_bounds__(256) k_store_members(LayoutK lay, KeysK keys, 
     r[2 * i] = (uint32_t)x;
     r[2 * i + 1] = (uint32_t)(x >> 32);
   }
+#pragma unroll
+  for (int i = 0; i < NW32; ++i) row[i] = r[i];
+  // member values: new where selected, stored otherwise
   int j = 0;
   for (int c = 0; c < lay.nc; ++c) {
-    if (!((mask >> c) & 1u)) continue;
-    const CompK &k = lay.c[c];
-    const float v = vals[(int64_t)j * count + t];
-    ++j;
-    if (k.kind == K_FLOAT && k.shared) continue;  // groups below
-    if (k.kind == K_FIXED) {
-      const float u = k.dither ? noise_u((uint32_t)p, keys.k[c]) : 0.0f;
-      set_field(r, k.off, k.nbits, enc_fixed(v, k, u));
-    } else if (k.kind == K_FLOAT) {
-      uint32_t e;
-      const uint32_t d = enc_float(v, k, e);
-      set_field(r, k.eoff, k.exp_bits, e);
-      set_field(r, k.off, k.nbits, d);
-    } else if (k.kind == K_F32) {
-      set_field(r, k.off, 32u, __float_as_uint(v));
+    if ((mask >> c) & 1u) {
+      vv[c] = vals[(int64_t)j * count + t];
+      ++j;
     } else {
-      set_field(r, k.off, k.nbits, cvt_sat(v, k.is_signed));
+      vv[c] = decode_comp(r, lay.c[c]);
     }
   }
-  // shared-exponent groups touched by the mask
-  for (int g0 = 0; g0 < lay.nc; ++g0) {
-    const CompK &k0 = lay.c[g0];
-    if (!(k0.kind == K_FLOAT && k0.shared && k0.gfirst)) continue;
-    int g1 = g0;
-    while (!lay.c[g1].glast) ++g1;
-    uint32_t gm = 0u;
-    for (int c = g0; c <= g1; ++c) gm |= (mask >> c) & 1u;
-    if (!gm) continue;
-    float gv[AQ_MAX_COMPS];
-    uint32_t emax = 0u;
-    for (int c = g0; c <= g1; ++c) {
-      float v;
-      if ((mask >> c) & 1u) {
-        int jj = 0;
-        for (int q = 0; q < c; ++q) jj += (mask >> q) & 1u;
-        v = vals[(int64_t)jj * count + t];
-      } else {
-        v = decode_comp(r, lay.c[c]);
-      }
-      gv[c - g0] = v;
-      const uint32_t e = f32_exp(v);
-      emax = e > emax ? e : emax;
+  // members to write: selected ones and whole groups touched by them
+  uint32_t touched = mask;
+  for (int c = 0; c < lay.nc; ++c) {
+    const CompK &k = lay.c[c];
+    if (k.kind == K_FLOAT && k.shared && ((mask >> c) & 1u)) {
+      int g0 = c, g1 = c;
+      while (!lay.c[g0].gfirst) --g0;
+      while (!lay.c[g1].glast) ++g1;
+      for (int q = g0; q <= g1; ++q) touched |= 1u << q;
     }
-    set_field(r, k0.eoff, k0.exp_bits, shared_exp_field(emax, k0));
-    for (int c = g0; c <= g1; ++c)
-      set_field(r, lay.c[c].off, lay.c[c].nbits, enc_shared(gv[c - g0], emax, lay.c[c]));
+  }
+  uint32_t gmax = 0u;
+  for (int c = 0; c < lay.nc; ++c) {
+    if (!((touched >> c) & 1u)) continue;
+    const CompK &k = lay.c[c];
+    if (k.kind == K_FLOAT && k.shared) {
+      if (k.gfirst) {
+        gmax = 0u;
+        for (int q = c; q < lay.nc; ++q) {
+          const uint32_t e = f32_exp(vv[q]);
+          gmax = e > gmax ? e : gmax;
+          if (lay.c[q].glast) break;
+        }
+        row_set(row, k.eoff, k.exp_bits, shared_exp_field(gmax, k));
+      }
+      row_set(row, k.off, k.nbits, enc_shared(vv[c], gmax, k));
+    } else {
+      const float u = (k.kind == K_FIXED && k.dither) ? noise_u((uint32_t)p, keys.k[c]) : 0.0f;
+      encode_member(vv[c], k, u, row, true);
+    }
   }
   for (int i = 0; i < lay.nw; ++i)
-    words[(int64_t)i * n + p] = (uint64_t)r[2 * i] | ((uint64_t)r[2 * i + 1] << 32);
+    words[(int64_t)i * n + p] = (uint64_t)row[2 * i] | ((uint64_t)row[2 * i + 1] << 32);
 }
 
 template <template <int> class K, class... A>
@@ -402,26 +394,27 @@ extern "C" int aq_layout_plan(const aq_container *cs, int ncont, const aq_place 
 template <int DIM, int MAT, int ST, int NW32, class T>
 __global__ void __launch_bounds__(256) k_p2g(StateK s, LayoutK lay, ParamsT<T> P, int64_t n,
                                              vec4_t<T> *__restrict__ grid, uint32_t *__restrict__ flags) {
+  __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
   const int64_t p = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= n) return;
   constexpr int NC = Comps<DIM, MAT>::NC;
   T v[NC];
-  load_state<ST, NW32, NC, T>(s, lay, p, 0xFFFFFFFFu, v);
-  const int32_t blk = p2g_particle<DIM, MAT, T>(v, P, grid);
-  flags[blk] = 1u;
+  load_state<ST, NW32, NC, T>(s, lay, p, 0xFFFFFFFFu, rows + threadIdx.x * RowsOf<ST, NW32>::RS, v);
+  p2g_particle<DIM, MAT, T>(v, P, grid, flags);
 }
 
 template <int DIM, int MAT, int ST, int NW32, class T>
 __global__ void __launch_bounds__(256) k_g2p(StateK s, LayoutK lay, KeysK keys, ParamsT<T> P, int64_t n,
                                              const vec4_t<T> *__restrict__ grid) {
+  __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
   const int64_t p = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= n) return;
   constexpr int NC = Comps<DIM, MAT>::NC;
+  uint32_t *row = rows + threadIdx.x * RowsOf<ST, NW32>::RS;
   T v[NC], out[NC];
-  load_state<ST, NW32, NC, T>(s, lay, p, s.g2p_mask, v);
+  load_state<ST, NW32, NC, T>(s, lay, p, s.g2p_mask, row, v);
   g2p_particle<DIM, MAT, T>(v, P, grid, out);
-  uint32_t r[NW32 > 0 ? NW32 : 1];
-  store_state<ST, NW32, NC, T>(s, lay, keys.k, p, out, r);
+  store_state<ST, NW32, NC, T>(s, lay, keys.k, p, out, row);
 }
 
 // G2P of substep t (reading grid_v) fused with P2G of substep t+1
@@ -430,26 +423,255 @@ template <int DIM, int MAT, int ST, int NW32, class T>
 __global__ void __launch_bounds__(256) k_g2p2g(StateK s, LayoutK lay, KeysK keys, ParamsT<T> P, int64_t n,
                                                const vec4_t<T> *__restrict__ grid_v,
                                                vec4_t<T> *__restrict__ grid_s, uint32_t *__restrict__ flags_s) {
+  __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
   const int64_t p = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= n) return;
   constexpr int NC = Comps<DIM, MAT>::NC;
+  uint32_t *row = rows + threadIdx.x * RowsOf<ST, NW32>::RS;
   T v[NC], out[NC];
-  load_state<ST, NW32, NC, T>(s, lay, p, s.g2p_mask, v);
+  load_state<ST, NW32, NC, T>(s, lay, p, s.g2p_mask, row, v);
   g2p_particle<DIM, MAT, T>(v, P, grid_v, out);
-  uint32_t r[NW32 > 0 ? NW32 : 1];
-  store_state<ST, NW32, NC, T>(s, lay, keys.k, p, out, r);
+  store_state<ST, NW32, NC, T>(s, lay, keys.k, p, out, row);
   if constexpr (ST == AQ_STORE_QUANT) {
     // the next P2G sees what storage holds: decode the record just written
 #pragma unroll
-    for (int c = 0; c < NC; ++c) out[c] = (T)decode_comp(r, lay.c[c]);
+    for (int c = 0; c < NC; ++c) out[c] = (T)decode_row(row, lay.c[c]);
   }
-  const int32_t blk = p2g_particle<DIM, MAT, T>(out, P, grid_s);
-  flags_s[blk] = 1u;
+  p2g_particle<DIM, MAT, T>(out, P, grid_s, flags_s);
 }
 
-// Grid update (mpm_solver.py:633-665) on one 64-node block per wavefront.
-// A block is active when a particle based in it or in a lower neighbour
-// block scattered this substep.  Optionally zeroes the blocks of another
+// ---- LDS-tiled kernels over work items ---------------------------------------
+// A work item is up to 256 particles whose stencil base was in one grid block
+// at the last aq_sort: int4 (block, first particle, count, 0).  The item's
+// tile covers the block, the 2-node stencil reach and a 1-cell drift margin
+// on each side: TE = E + 4 nodes per axis starting at block_origin - 1.
+// G2P reads grid velocities staged in LDS.  P2G accumulates into an LDS tile
+// in 64-bit fixed point (integer LDS atomics; float LDS atomics are ~30x
+// slower on this chip): per item and component the contributions are scaled
+// by 2^k with 2^(61-k) above the item's summed contribution bound, so the
+// integer sums cannot overflow and are exact sums of the rounded terms
+// (quantum 2^-61 of the bound); each non-empty node is then flushed once with
+// global float atomics.  A particle whose stencil leaves the tile (it drifted
+// more than a cell since the sort), or an item with a non-finite bound,
+// reads and scatters through global memory.
+template <int DIM> struct Tile {
+  static constexpr int E = BlockShape<DIM>::E;
+  static constexpr int TE = E + 4;
+  static constexpr int TN = DIM == 3 ? TE * TE * TE : TE * TE;
+  __device__ static int index(int x, int y, int z) { return DIM == 3 ? (x * TE + y) * TE + z : x * TE + y; }
+};
+
+template <int DIM>
+__device__ __forceinline__ void block_coords(int32_t b, int nb, int (&bc)[3]) {
+  if constexpr (DIM == 3) {
+    bc[0] = b / (nb * nb);
+    bc[1] = (b / nb) % nb;
+    bc[2] = b % nb;
+  } else {
+    bc[0] = b / nb;
+    bc[1] = b % nb;
+    bc[2] = 0;
+  }
+}
+
+template <int DIM>
+__device__ __forceinline__ int64_t node_index(int i, int j, int k, int nb) {
+  constexpr int LG = BlockShape<DIM>::LOG, E = BlockShape<DIM>::E;
+  if constexpr (DIM == 3) {
+    const int64_t blk = ((int64_t)(i >> LG) * nb + (j >> LG)) * nb + (k >> LG);
+    return blk * 64 + (((i & (E - 1)) << 4) | ((j & (E - 1)) << 2) | (k & (E - 1)));
+  } else {
+    const int64_t blk = (int64_t)(i >> LG) * nb + (j >> LG);
+    return blk * 64 + (((i & (E - 1)) << 3) | (j & (E - 1)));
+  }
+}
+
+template <int DIM>
+__device__ __forceinline__ void tile_coords(int q, int (&t)[3]) {
+  constexpr int TE = Tile<DIM>::TE;
+  if constexpr (DIM == 3) {
+    t[2] = q % TE;
+    t[1] = (q / TE) % TE;
+    t[0] = q / (TE * TE);
+  } else {
+    t[1] = q % TE;
+    t[0] = q / TE;
+    t[2] = 0;
+  }
+}
+
+template <class T> __device__ __forceinline__ T tldexp(T x, int k);
+template <> __device__ __forceinline__ float tldexp<float>(float x, int k) { return ldexpf(x, k); }
+template <> __device__ __forceinline__ double tldexp<double>(double x, int k) { return ldexp(x, k); }
+template <class T> __device__ __forceinline__ T trint(T x);
+template <> __device__ __forceinline__ float trint<float>(float x) { return rintf(x); }
+template <> __device__ __forceinline__ double trint<double>(double x) { return rint(x); }
+template <class T> __device__ __forceinline__ int texp(T x);  // x = m 2^e, 0.5 <= |m| < 1
+template <> __device__ __forceinline__ int texp<float>(float x) { int e; frexpf(x, &e); return e; }
+template <> __device__ __forceinline__ int texp<double>(double x) { int e; frexp(x, &e); return e; }
+
+// round(c * 2^k) as a two's-complement int64, for |c * 2^k| < 2^62, from
+// exact steps: |f| = hi * 2^32 + lo with hi < 2^30, and lo = |f| - hi * 2^32
+// holds only bits of |f| below 2^32 (representable in T).
+template <class T> __device__ __forceinline__ uint64_t to_fixed(T c, int k) {
+  const T f = trint<T>(tldexp<T>(c, k));
+  const T a = tabs<T>(f);
+  const T hi = tfloor<T>(tldexp<T>(a, -32));
+  const T lo = a - hi * (T)4294967296.0;
+  const uint64_t u = ((uint64_t)(uint32_t)hi << 32) + (uint64_t)(uint32_t)lo;
+  return f < (T)0 ? (uint64_t)0 - u : u;
+}
+
+// q * 2^-k rounded once to double, then to T.
+template <class T> __device__ __forceinline__ T from_fixed(uint64_t q, int k) {
+  const double d = (double)(int32_t)((int64_t)q >> 32) * 4294967296.0 + (double)(uint32_t)q;
+  return (T)ldexp(d, -k);
+}
+
+template <class T> __device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+
+template <int DIM, int MAT, int ST, int NW32, class T, bool DO_G2P, bool DO_P2G>
+__global__ void __launch_bounds__(256) k_tiled(StateK s, LayoutK lay, KeysK keys, ParamsT<T> P, int64_t n,
+                                               const int4 *__restrict__ items, const int32_t *__restrict__ nitems,
+                                               const vec4_t<T> *__restrict__ grid_v, vec4_t<T> *__restrict__ grid_s,
+                                               uint32_t *__restrict__ flags_s) {
+  using TL = Tile<DIM>;
+  constexpr int TN = TL::TN, TE = TL::TE, NC = Comps<DIM, MAT>::NC;
+  __shared__ vec4_t<T> tv[DO_G2P ? TN : 1];
+  __shared__ unsigned long long ts[DO_P2G ? 4 : 1][DO_P2G ? TN : 1];
+  __shared__ T bpart[4][4];
+  __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  uint32_t *row = rows + tid * RowsOf<ST, NW32>::RS;
+  if constexpr (DO_P2G)
+    for (int q = tid; q < TN; q += 256) ts[0][q] = ts[1][q] = ts[2][q] = ts[3][q] = 0ull;
+  const int ni = *nitems;
+  // each XCD (blocks dealt round-robin to 8 XCDs) walks a contiguous item range
+  const int xcd = (int)(blockIdx.x & 7u);
+  const int per = (int)(gridDim.x >> 3);
+  const int lo = (int)((int64_t)ni * xcd / 8), hi = (int)((int64_t)ni * (xcd + 1) / 8);
+  for (int it = lo + (int)(blockIdx.x >> 3); it < hi; it += per) {
+    const int4 item = items[it];
+    int bc[3], o[3];
+    block_coords<DIM>(item.x, P.nb, bc);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) o[a] = a < DIM ? bc[a] * TL::E - 1 : 0;
+    if constexpr (DO_G2P) {
+      for (int q = tid; q < TN; q += 256) {
+        int t[3];
+        tile_coords<DIM>(q, t);
+        const int gi = o[0] + t[0], gj = o[1] + t[1], gk = o[2] + t[2];
+        vec4_t<T> val;
+        val.x = val.y = val.z = val.w = (T)0;
+        const bool in = gi >= 0 && gi < P.n && gj >= 0 && gj < P.n && (DIM == 2 || (gk >= 0 && gk < P.n));
+        if (in) val = grid_v[node_index<DIM>(gi, gj, gk, P.nb)];
+        tv[q] = val;
+      }
+    }
+    __syncthreads();  // tile staged; previous item's flush (and zeroing) done
+    const bool has = tid < item.z && (int64_t)item.y + tid < n;
+    const int64_t p = (int64_t)item.y + tid;
+    T out[NC];
+    if (has) {
+      if constexpr (DO_G2P) {
+        T v[NC];
+        load_state<ST, NW32, NC, T>(s, lay, p, s.g2p_mask, row, v);
+        const Stencil<DIM, T> S = stencil_of<DIM, MAT, T>(v, P);
+        int rel[3] = {0, 0, 0};
+        bool inside = true;
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) {
+          rel[a] = S.base[a] - o[a];
+          inside = inside && rel[a] >= 0 && rel[a] + 2 < TE;
+        }
+        if (inside) {
+          g2p_apply<DIM, MAT, T>(v, P, S, [&](int i, int j, int k) { return tv[TL::index(rel[0] + i, rel[1] + j, rel[2] + k)]; }, out);
+        } else {
+          g2p_particle<DIM, MAT, T>(v, P, grid_v, out);
+        }
+        store_state<ST, NW32, NC, T>(s, lay, keys.k, p, out, row);
+        if constexpr (ST == AQ_STORE_QUANT && DO_P2G) {
+          // the next P2G sees what storage holds: decode the record just written
+#pragma unroll
+          for (int c = 0; c < NC; ++c) out[c] = (T)decode_row(row, lay.c[c]);
+        }
+      } else {
+        load_state<ST, NW32, NC, T>(s, lay, p, 0xFFFFFFFFu, row, out);
+      }
+    }
+    if constexpr (DO_P2G) {
+      Stencil<DIM, T> S;
+      P2GPrep<DIM, T> R;
+      int rel[3] = {0, 0, 0};
+      bool inside = false;
+      T B[4] = {(T)0, (T)0, (T)0, (T)0};
+      if (has) {
+        S = stencil_of<DIM, MAT, T>(out, P);
+        R = p2g_prepare<DIM, MAT, T>(out, P);
+        inside = true;
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) {
+          rel[a] = S.base[a] - o[a];
+          inside = inside && rel[a] >= 0 && rel[a] + 2 < TE;
+        }
+        if (inside) p2g_bounds<DIM, T>(R, P, S, B);
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) B[c] = wave_sum<T>(B[c]);
+      if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) bpart[wave][c] = B[c];
+      }
+      __syncthreads();  // item bounds complete
+      int k[4];
+      bool finite = true;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const T sum = ((bpart[0][c] + bpart[1][c]) + bpart[2][c]) + bpart[3][c];
+        finite = finite && sum <= (T)3.0e38;
+        k[c] = 61 - texp<T>(sum * ((T)1 + (T)1 / (T)1024));
+      }
+      if (has) {
+        if (inside && finite) {
+          p2g_emit<DIM, T>(R, P, S, [&](int i, int j, int kk, const T (&mom)[3], T m) {
+            const int q = TL::index(rel[0] + i, rel[1] + j, rel[2] + kk);
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) atomicAdd(&ts[a][q], (unsigned long long)to_fixed<T>(mom[a], k[a]));
+            atomicAdd(&ts[3][q], (unsigned long long)to_fixed<T>(m, k[3]));
+          });
+        } else {
+          p2g_particle<DIM, MAT, T>(out, P, grid_s, flags_s);
+        }
+      }
+      __syncthreads();  // tile sums complete
+      for (int q = tid; q < TN; q += 256) {
+        const uint64_t qm = ts[3][q], qx = ts[0][q], qy = ts[1][q], qz = DIM == 3 ? ts[2][q] : 0ull;
+        if ((qm | qx | qy | qz) != 0ull) {
+          int t[3];
+          tile_coords<DIM>(q, t);
+          const int gi = o[0] + t[0], gj = o[1] + t[1], gk = o[2] + t[2];
+          const int64_t node = node_index<DIM>(gi, gj, gk, P.nb);
+          T *nd = (T *)(grid_s + node);
+          atomic_add<T>(nd + 0, from_fixed<T>(qx, k[0]));
+          atomic_add<T>(nd + 1, from_fixed<T>(qy, k[1]));
+          if (DIM == 3) atomic_add<T>(nd + 2, from_fixed<T>(qz, k[2]));
+          atomic_add<T>(nd + 3, from_fixed<T>(qm, k[3]));
+          flags_s[node >> 6] = 1u;
+          ts[0][q] = ts[1][q] = ts[2][q] = ts[3][q] = 0ull;
+        }
+      }
+    } else {
+      __syncthreads();  // G2P done with the tile before the next item stages it
+    }
+  }
+}
+
+// Grid update (mpm_solver.py:633-665) on one 64-node block per wavefront,
+// for the blocks flagged by P2G.  Optionally zeroes the blocks of another
 // grid that were active (the grid consumed by the previous G2P).
 template <int DIM, class T>
 __global__ void __launch_bounds__(256) k_grid_update(ParamsT<T> P, int32_t nblocks, vec4_t<T> *__restrict__ grid,
@@ -472,29 +694,11 @@ __global__ void __launch_bounds__(256) k_grid_update(ParamsT<T> P, int32_t nbloc
       oact[b] = 0u;
     }
   }
+  if (!flags[b]) return;
+  if (lane == 0) act[b] = 1u;
   const int nb = P.nb;
   int bc[3];
-  if constexpr (DIM == 3) {
-    bc[0] = b / (nb * nb);
-    bc[1] = (b / nb) % nb;
-    bc[2] = b % nb;
-  } else {
-    bc[0] = b / nb;
-    bc[1] = b % nb;
-    bc[2] = 0;
-  }
-  uint32_t a = 0u;
-  constexpr int ND = DIM == 3 ? 8 : 4;
-#pragma unroll
-  for (int d = 0; d < ND; ++d) {
-    const int di = (d >> 0) & 1, dj = (d >> 1) & 1, dk = DIM == 3 ? ((d >> 2) & 1) : 0;
-    const int i = bc[0] - di, j = bc[1] - dj, k = bc[2] - dk;
-    if (i < 0 || j < 0 || k < 0) continue;
-    const int32_t nbid = DIM == 3 ? (i * nb + j) * nb + k : i * nb + j;
-    a |= flags[nbid];
-  }
-  if (!a) return;
-  if (lane == 0) act[b] = 1u;
+  block_coords<DIM>(b, nb, bc);
   int I[3];
   if constexpr (DIM == 3) {
     I[0] = bc[0] * 4 + (lane >> 4);
@@ -528,6 +732,47 @@ __global__ void __launch_bounds__(256) k_grid_update(ParamsT<T> P, int32_t nbloc
   grid[(int64_t)b * 64 + lane] = nd;
 }
 
+// Work items from the block counts of a sort: items per block = ceil(count / 256);
+// one workgroup scans them (item_off[b], total in item_off[m]), then each
+// block writes its items.
+__global__ void __launch_bounds__(1024) k_items_scan(const uint32_t *__restrict__ counts,
+                                                     int32_t *__restrict__ item_off, int32_t m) {
+  __shared__ int32_t part[1024];
+  const int t = threadIdx.x;
+  const int per = (m + 1023) / 1024;
+  const int lo = t * per, hi = lo + per < m ? lo + per : m;
+  int32_t sum = 0;
+  for (int i = lo; i < hi; ++i) sum += (int32_t)((counts[i] + 255u) >> 8);
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int32_t x = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += x;
+    __syncthreads();
+  }
+  int32_t run = part[t] - sum;
+  for (int i = lo; i < hi; ++i) {
+    item_off[i] = run;
+    run += (int32_t)((counts[i] + 255u) >> 8);
+  }
+  if (t == 1023) item_off[m] = part[1023];
+}
+
+__global__ void __launch_bounds__(256) k_items_fill(const uint32_t *__restrict__ counts,
+                                                    const uint32_t *__restrict__ offsets,
+                                                    const int32_t *__restrict__ item_off, int4 *__restrict__ items,
+                                                    int32_t m) {
+  const int32_t b = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (b >= m) return;
+  const uint32_t c = counts[b];
+  const int32_t base = item_off[b];
+  for (uint32_t j = 0; j * 256u < c; ++j) {
+    const uint32_t left = c - j * 256u;
+    items[base + (int32_t)j] = make_int4(b, (int)(offsets[b] + j * 256u), (int)(left < 256u ? left : 256u), 0);
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) k_grid_clear(int32_t nblocks, vec4_t<T> *__restrict__ grid,
                                                     uint32_t *__restrict__ flags, uint32_t *__restrict__ act) {
@@ -557,20 +802,16 @@ template <int DIM, int MAT, int ST, int NW32, class T>
 __global__ void __launch_bounds__(256) k_sort_keys(StateK s, LayoutK lay, ParamsT<T> P, int64_t n,
                                                    uint32_t *__restrict__ key, uint32_t *__restrict__ rank,
                                                    uint32_t *__restrict__ counts) {
+  __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
+  constexpr int NC = Comps<DIM, MAT>::NC;
+  T v[NC];
+  load_state<ST, NW32, NC, T>(s, lay, p, s.g2p_mask, rows + threadIdx.x * RowsOf<ST, NW32>::RS, v);
   int base[3] = {0, 0, 0};
   T fx, w[3];
-  if constexpr (ST == AQ_STORE_QUANT) {
-    uint32_t r[NW32];
-    load_words(s, p, s.g2p_mask, r);
 #pragma unroll
-    for (int a = 0; a < DIM; ++a) base[a] = stencil_1d<T>((T)decode_comp(r, lay.c[a]), P, fx, w);
-  } else {
-    const T *vals = (const T *)s.ptr;
-#pragma unroll
-    for (int a = 0; a < DIM; ++a) base[a] = stencil_1d<T>(vals[(int64_t)a * s.cap + p], P, fx, w);
-  }
+  for (int a = 0; a < DIM; ++a) base[a] = stencil_1d<T>(v[a], P, fx, w);
   const uint32_t k = (uint32_t)block_of_base<DIM>(base, P.nb);
   key[p] = k;
   rank[p] = atomicAdd(counts + k, 1u);
@@ -628,11 +869,12 @@ template <int DIM, int MAT, int ST, int NW32, class T>
 __global__ void __launch_bounds__(256) k_limits(StateK s, LayoutK lay, int64_t n, float *__restrict__ mxmn) {
   constexpr int NC = Comps<DIM, MAT>::NC;
   __shared__ float smx[NC][8], smn[NC][8];
+  __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   T v[NC];
   float mx[NC], mn[NC];
   if (p < n) {
-    load_state<ST, NW32, NC, T>(s, lay, p, 0xFFFFFFFFu, v);
+    load_state<ST, NW32, NC, T>(s, lay, p, 0xFFFFFFFFu, rows + threadIdx.x * RowsOf<ST, NW32>::RS, v);
 #pragma unroll
     for (int c = 0; c < NC; ++c) mx[c] = mn[c] = (float)v[c];
   } else {
@@ -675,11 +917,12 @@ __global__ void __launch_bounds__(256) k_energy(StateK s, LayoutK lay, ParamsT<T
   constexpr int NC = Comps<DIM, MAT>::NC;
   using CI = Comps<DIM, MAT>;
   __shared__ double part[8];
+  __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double e = 0.0;
   if (p < n) {
     T v[NC];
-    load_state<ST, NW32, NC, T>(s, lay, p, 0xFFFFFFFFu, v);
+    load_state<ST, NW32, NC, T>(s, lay, p, 0xFFFFFFFFu, rows + threadIdx.x * RowsOf<ST, NW32>::RS, v);
     T v2 = (T)0;
 #pragma unroll
     for (int a = 0; a < DIM; ++a) v2 = v2 + v[CI::CV + a] * v[CI::CV + a];
@@ -718,6 +961,13 @@ struct aq_sim {
   uint32_t *skey, *srank, *scount, *soff;
   void *tmp_state;
   int32_t *tmp_ids;
+  int4 *items;
+  int32_t *item_off;  // [nblocks + 1]; item_off[nblocks] = item count
+  int32_t max_items;
+  bool have_items;    // items describe the current order of `items_state`
+  const void *items_state;
+  int64_t items_n;
+  bool dither_any;
   int64_t ws_bytes;
 };
 
@@ -732,15 +982,21 @@ static int check_desc(const aq_sim_desc *d) {
   const int e = d->dim == 3 ? 4 : 8;
   if (d->n_grid < 8 || d->n_grid % e) return fail(AQ_ERR_INVALID, "n_grid must be a multiple of the block edge");
   if (d->dim == 3 && d->n_grid > 1024) return fail(AQ_ERR_INVALID, "n_grid too large");
+  if (d->dim == 2 && d->n_grid > 32768) return fail(AQ_ERR_INVALID, "n_grid too large");
   if (d->storage < AQ_STORE_QUANT || d->storage > AQ_STORE_F64) return fail(AQ_ERR_INVALID, "bad storage");
   const int nc = 2 * d->dim + d->dim * d->dim + (d->material == AQ_FLUID ? 1 : d->dim * d->dim);
   if (d->ncomp != nc) return fail(AQ_ERR_INVALID, "ncomp does not match dim/material");
   if (d->capacity < 0 || d->capacity > 0x7FFFFFFFll) return fail(AQ_ERR_INVALID, "capacity out of range");
   if (d->boundary != AQ_SEPARATING && d->boundary != AQ_STICKY) return fail(AQ_ERR_INVALID, "bad boundary");
+  if (d->bound < 0 || 2 * d->bound >= d->n_grid) return fail(AQ_ERR_INVALID, "bad wall thickness");
   return AQ_OK;
 }
 
-static void ws_plan(const aq_sim_desc *d, int64_t *offs /*[10]*/, int64_t *total) {
+static int32_t max_items_of(const aq_sim_desc *d, int64_t nblocks) {
+  return (int32_t)((d->capacity + 255) / 256 + nblocks);
+}
+
+static void ws_plan(const aq_sim_desc *d, int64_t *offs /*[11]*/, int64_t *total) {
   const int64_t n = d->n_grid;
   const int64_t nodes = d->dim == 3 ? n * n * n : n * n;
   const int64_t nblocks = nodes / 64;
@@ -749,21 +1005,23 @@ static void ws_plan(const aq_sim_desc *d, int64_t *offs /*[10]*/, int64_t *total
   const int64_t state_bytes = d->storage == AQ_STORE_QUANT ? cap * 8 * d->nwords
                               : cap * d->ncomp * (d->storage == AQ_STORE_F64 ? 8 : 4);
   int64_t o = 0;
-  offs[0] = o; o = align256(o + nodes * node_bytes);  // grid 0
-  offs[1] = o; o = align256(o + nodes * node_bytes);  // grid 1
-  offs[2] = o; o = align256(o + nblocks * 4 * 4);     // flags0, flags1, act0, act1
-  offs[3] = o; o = align256(o + cap * 4);             // sort keys
-  offs[4] = o; o = align256(o + cap * 4);             // sort ranks
-  offs[5] = o; o = align256(o + (nblocks + 1) * 4);   // counts
-  offs[6] = o; o = align256(o + (nblocks + 1) * 4);   // offsets
-  offs[7] = o; o = align256(o + state_bytes);         // sort temp state
-  offs[8] = o; o = align256(o + cap * 4);             // sort temp ids
+  offs[0] = o; o = align256(o + nodes * node_bytes);          // grid 0
+  offs[1] = o; o = align256(o + nodes * node_bytes);          // grid 1
+  offs[2] = o; o = align256(o + nblocks * 4 * 4);             // flags0, flags1, act0, act1
+  offs[3] = o; o = align256(o + cap * 4);                     // sort keys
+  offs[4] = o; o = align256(o + cap * 4);                     // sort ranks
+  offs[5] = o; o = align256(o + (nblocks + 1) * 4);           // counts
+  offs[6] = o; o = align256(o + (nblocks + 1) * 4);           // offsets
+  offs[7] = o; o = align256(o + state_bytes);                 // sort temp state
+  offs[8] = o; o = align256(o + cap * 4);                     // sort temp ids
+  offs[9] = o; o = align256(o + (int64_t)max_items_of(d, nblocks) * 16);  // work items
+  offs[10] = o; o = align256(o + (nblocks + 1) * 4);          // item offsets
   *total = o;
 }
 
 extern "C" int64_t aq_sim_workspace_bytes(const aq_sim_desc *d) {
   if (check_desc(d)) return -1;
-  int64_t offs[10], total;
+  int64_t offs[11], total;
   ws_plan(d, offs, &total);
   return total;
 }
@@ -795,17 +1053,19 @@ extern "C" int aq_sim_create(const aq_sim_desc *d, void *ws, int64_t ws_bytes, a
   int rc = check_desc(d);
   if (rc) return rc;
   if (!out) return fail(AQ_ERR_INVALID, "null output");
-  int64_t offs[10], total;
+  int64_t offs[11], total;
   ws_plan(d, offs, &total);
   if (!ws || ws_bytes < total) return fail(AQ_ERR_CAPACITY, "workspace too small");
   aq_sim *s = new aq_sim();
   s->d = *d;
+  s->dither_any = false;
   if (d->storage == AQ_STORE_QUANT) {
     rc = build_layout(d->comps, d->ncomp, d->nwords, &s->lay);
     if (rc) {
       delete s;
       return rc;
     }
+    for (int c = 0; c < d->ncomp; ++c) s->dither_any = s->dither_any || s->lay.c[c].dither;
   } else {
     std::memset(&s->lay, 0, sizeof(s->lay));
   }
@@ -829,11 +1089,18 @@ extern "C" int aq_sim_create(const aq_sim_desc *d, void *ws, int64_t ws_bytes, a
   s->soff = (uint32_t *)(base + offs[6]);
   s->tmp_state = base + offs[7];
   s->tmp_ids = (int32_t *)(base + offs[8]);
+  s->items = (int4 *)(base + offs[9]);
+  s->item_off = (int32_t *)(base + offs[10]);
+  s->max_items = max_items_of(d, s->nblocks);
+  s->have_items = false;
+  s->items_state = nullptr;
+  s->items_n = -1;
   s->ws_bytes = ws_bytes;
   s->dirty[0] = s->dirty[1] = false;
   s->cur = 0;
   // grids, flags and activity start at zero
   hipError_t e = hipMemset(ws, 0, (size_t)offs[3]);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     delete s;
     return fail(AQ_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
@@ -847,11 +1114,14 @@ extern "C" int aq_sim_destroy(aq_sim *s) {
   return AQ_OK;
 }
 
-static StateK make_state(const aq_sim *s, const void *state) {
+static StateK make_state(const aq_sim *s, const void *state, const int32_t *ids) {
   StateK k;
   k.ptr = (void *)state;
   k.cap = s->d.capacity;
+  k.ids = ids;
   k.nw = s->d.nwords;
+  k.dither_any = s->dither_any ? 1 : 0;
+  k.pad = 0;
   uint32_t m = 0u;
   if (s->d.storage == AQ_STORE_QUANT) {
     const int d = s->d.dim;
@@ -917,12 +1187,22 @@ template <> const ParamsT<double> &params_of<double>(const aq_sim *s) { return s
 
 static unsigned pblocks(int64_t n) { return (unsigned)((n + 255) / 256); }
 static unsigned gblocks(const aq_sim *s) { return (unsigned)((s->nblocks + 3) / 4); }
+// persistent grid of the tiled kernels: a multiple of 8 (one share per XCD)
+static unsigned tblocks(const aq_sim *s, int64_t n) {
+  int64_t want = (n + 255) / 256 + 64;
+  if (want > 4096) want = 4096;
+  return (unsigned)((want + 7) / 8 * 8);
+}
 
 static int check_n(const aq_sim *s, const void *state, int64_t n) {
   if (!s) return fail(AQ_ERR_INVALID, "null handle");
   if (n < 0 || n > s->d.capacity) return fail(AQ_ERR_CAPACITY, "particle count exceeds capacity");
   if (n && !state) return fail(AQ_ERR_INVALID, "null state");
   return AQ_OK;
+}
+
+static bool items_valid(const aq_sim *s, const void *state, int64_t n) {
+  return s->have_items && s->items_state == state && s->items_n == n;
 }
 
 static int clear_grid(aq_sim *s, int g, hipStream_t st) {
@@ -938,13 +1218,20 @@ static int clear_grid(aq_sim *s, int g, hipStream_t st) {
   return AQ_OK;
 }
 
+// P2G into grid g (tiled when the last sort describes this state).
 static int launch_p2g(aq_sim *s, const void *state, int64_t n, int g, hipStream_t st) {
-  StateK sk = make_state(s, state);
   if (n == 0) return AQ_OK;
+  StateK sk = make_state(s, state, nullptr);
+  const bool tiled = items_valid(s, state, n);
   dispatch(s, [&](auto tag_) {
     TAG_TYPES(tag_)
-    hipLaunchKernelGGL((k_p2g<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, s->lay,
-                       params_of<T>(s), n, (vec4_t<T> *)s->grid[g], s->flags[g]);
+    if (tiled)
+      hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, false, true>), dim3(tblocks(s, n)), dim3(256), 0, st, sk,
+                         s->lay, KeysK{}, params_of<T>(s), n, s->items, s->item_off + s->nblocks,
+                         (const vec4_t<T> *)nullptr, (vec4_t<T> *)s->grid[g], s->flags[g]);
+    else
+      hipLaunchKernelGGL((k_p2g<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, s->lay,
+                         params_of<T>(s), n, (vec4_t<T> *)s->grid[g], s->flags[g]);
   });
   LAUNCH_CHECK();
   return AQ_OK;
@@ -974,14 +1261,40 @@ static int launch_grid_update(aq_sim *s, int g, int other, hipStream_t st) {
   return AQ_OK;
 }
 
-static int launch_g2p(aq_sim *s, void *state, int64_t n, int g, uint32_t step, hipStream_t st) {
-  StateK sk = make_state(s, state);
+static int launch_g2p(aq_sim *s, void *state, int64_t n, const int32_t *ids, int g, uint32_t step,
+                      hipStream_t st) {
   if (n == 0) return AQ_OK;
+  StateK sk = make_state(s, state, ids);
   KeysK keys = make_keys(s->d.seed, step);
+  const bool tiled = items_valid(s, state, n);
   dispatch(s, [&](auto tag_) {
     TAG_TYPES(tag_)
-    hipLaunchKernelGGL((k_g2p<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, s->lay, keys,
-                       params_of<T>(s), n, (const vec4_t<T> *)s->grid[g]);
+    if (tiled)
+      hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, true, false>), dim3(tblocks(s, n)), dim3(256), 0, st, sk,
+                         s->lay, keys, params_of<T>(s), n, s->items, s->item_off + s->nblocks,
+                         (const vec4_t<T> *)s->grid[g], (vec4_t<T> *)nullptr, (uint32_t *)nullptr);
+    else
+      hipLaunchKernelGGL((k_g2p<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, s->lay, keys,
+                         params_of<T>(s), n, (const vec4_t<T> *)s->grid[g]);
+  });
+  LAUNCH_CHECK();
+  return AQ_OK;
+}
+
+static int launch_g2p2g(aq_sim *s, void *state, int64_t n, const int32_t *ids, int gv, int gs, uint32_t step,
+                        hipStream_t st) {
+  StateK sk = make_state(s, state, ids);
+  KeysK keys = make_keys(s->d.seed, step);
+  const bool tiled = items_valid(s, state, n);
+  dispatch(s, [&](auto tag_) {
+    TAG_TYPES(tag_)
+    if (tiled)
+      hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, true, true>), dim3(tblocks(s, n)), dim3(256), 0, st, sk,
+                         s->lay, keys, params_of<T>(s), n, s->items, s->item_off + s->nblocks,
+                         (const vec4_t<T> *)s->grid[gv], (vec4_t<T> *)s->grid[gs], s->flags[gs]);
+    else
+      hipLaunchKernelGGL((k_g2p2g<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, s->lay, keys,
+                         params_of<T>(s), n, (const vec4_t<T> *)s->grid[gv], (vec4_t<T> *)s->grid[gs], s->flags[gs]);
   });
   LAUNCH_CHECK();
   return AQ_OK;
@@ -1003,17 +1316,10 @@ extern "C" int aq_grid_op(aq_sim *s, void *stream) {
   return launch_grid_update(s, s->cur, -1, (hipStream_t)stream);
 }
 
-extern "C" int aq_g2p(aq_sim *s, void *state, int64_t n, uint32_t step, void *stream) {
+extern "C" int aq_g2p(aq_sim *s, void *state, int64_t n, const int32_t *ids, uint32_t step, void *stream) {
   int rc = check_n(s, state, n);
   if (rc) return rc;
-  return launch_g2p(s, state, n, s->cur, step, (hipStream_t)stream);
-}
-
-extern "C" int aq_substep(aq_sim *s, void *state, int64_t n, uint32_t step, void *stream) {
-  int rc;
-  if ((rc = aq_p2g(s, state, n, stream))) return rc;
-  if ((rc = aq_grid_op(s, stream))) return rc;
-  return aq_g2p(s, state, n, step, stream);
+  return launch_g2p(s, state, n, ids, s->cur, step, (hipStream_t)stream);
 }
 
 extern "C" int aq_sort(aq_sim *s, void *state, int64_t n, int32_t *ids, void *stream) {
@@ -1021,7 +1327,7 @@ extern "C" int aq_sort(aq_sim *s, void *state, int64_t n, int32_t *ids, void *st
   if (rc) return rc;
   if (n == 0) return AQ_OK;
   hipStream_t st = (hipStream_t)stream;
-  StateK sk = make_state(s, state);
+  StateK sk = make_state(s, state, nullptr);
   hipLaunchKernelGGL(k_fill_u32, dim3((s->nblocks + 1 + 255) / 256), dim3(256), 0, st, s->scount, 0u,
                      (int64_t)s->nblocks + 1);
   LAUNCH_CHECK();
@@ -1056,7 +1362,23 @@ extern "C" int aq_sort(aq_sim *s, void *state, int64_t n, int32_t *ids, void *st
   HIP_TRY(hipMemcpy2DAsync(state, (size_t)cap * esz, s->tmp_state, (size_t)cap * esz, (size_t)n * esz,
                            (size_t)planes, hipMemcpyDeviceToDevice, st));
   if (ids) HIP_TRY(hipMemcpyAsync(ids, s->tmp_ids, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+  // work items: up to 256 particles of one block each
+  hipLaunchKernelGGL(k_items_scan, dim3(1), dim3(1024), 0, st, s->scount, s->item_off, s->nblocks);
+  hipLaunchKernelGGL(k_items_fill, dim3((s->nblocks + 255) / 256), dim3(256), 0, st, s->scount, s->soff,
+                     s->item_off, s->items, s->nblocks);
+  LAUNCH_CHECK();
+  s->have_items = true;
+  s->items_state = state;
+  s->items_n = n;
   return AQ_OK;
+}
+
+extern "C" int aq_substep(aq_sim *s, void *state, int64_t n, int32_t *ids, uint32_t step, void *stream) {
+  int rc;
+  if ((rc = aq_sort(s, state, n, ids, stream))) return rc;
+  if ((rc = aq_p2g(s, state, n, stream))) return rc;
+  if ((rc = aq_grid_op(s, stream))) return rc;
+  return aq_g2p(s, state, n, ids, step, stream);
 }
 
 // Event pairs around launches, summed per category when the call ends.
@@ -1066,27 +1388,31 @@ struct Timer {
   std::vector<hipEvent_t> a, b;
   std::vector<int> cat;
   int launches[NCAT] = {0, 0, 0, 0};
+  bool ok = true;
   void begin(int c) {
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventRecord(e0, st);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ok = ok && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
+         hipEventRecord(e0, st) == hipSuccess;
     a.push_back(e0);
     b.push_back(e1);
     cat.push_back(c);
     launches[c]++;
   }
-  void end() { hipEventRecord(b.back(), st); }
-  void collect(float *ms) {
-    hipStreamSynchronize(st);
+  void end() { ok = ok && hipEventRecord(b.back(), st) == hipSuccess; }
+  // Per-category sums in ms; false when an event call failed.
+  bool collect(float *ms) {
+    ok = ok && hipStreamSynchronize(st) == hipSuccess;
     for (int c = 0; c < NCAT; ++c) ms[c] = 0.0f;
     for (size_t i = 0; i < a.size(); ++i) {
       float t = 0.0f;
-      hipEventElapsedTime(&t, a[i], b[i]);
+      if (ok) ok = hipEventElapsedTime(&t, a[i], b[i]) == hipSuccess;
       ms[cat[i]] += t;
-      hipEventDestroy(a[i]);
-      hipEventDestroy(b[i]);
+      if (a[i]) (void)hipEventDestroy(a[i]);
+      if (b[i]) (void)hipEventDestroy(b[i]);
     }
+    a.clear();
+    b.clear();
+    return ok;
   }
 };
 
@@ -1120,22 +1446,14 @@ static int advance_impl(aq_sim *s, void *state, int64_t n, int nsteps, uint32_t 
       TIMED(tm, Timer::SORT, rc = aq_sort(s, state, n, ids, st));
       if (rc) return rc;
     }
-    StateK sk = make_state(s, state);
-    KeysK keys = make_keys(s->d.seed, step0 + (uint32_t)t);
-    if (tm) tm->begin(Timer::FUSED);
-    dispatch(s, [&](auto tag_) {
-    TAG_TYPES(tag_)
-      hipLaunchKernelGGL((k_g2p2g<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, s->lay, keys,
-                         params_of<T>(s), n, (const vec4_t<T> *)s->grid[X], (vec4_t<T> *)s->grid[Y], s->flags[Y]);
-    });
-    if (tm) tm->end();
-    LAUNCH_CHECK();
+    TIMED(tm, Timer::FUSED, rc = launch_g2p2g(s, state, n, ids, X, Y, step0 + (uint32_t)t, st));
+    if (rc) return rc;
     s->dirty[Y] = true;
     TIMED(tm, Timer::GRID, rc = launch_grid_update(s, Y, X, st));
     if (rc) return rc;
     std::swap(X, Y);
   }
-  TIMED(tm, Timer::ENDS, rc = launch_g2p(s, state, n, X, step0 + (uint32_t)(nsteps - 1), st));
+  TIMED(tm, Timer::ENDS, rc = launch_g2p(s, state, n, ids, X, step0 + (uint32_t)(nsteps - 1), st));
   if (rc) return rc;
   s->cur = X;
   return AQ_OK;
@@ -1160,11 +1478,12 @@ extern "C" int aq_advance_timed(aq_sim *s, void *state, int64_t n, int nsteps, u
   Timer tm;
   tm.st = st;
   int rc = advance_impl(s, state, n, nsteps, step0, sort_every, ids, st, &tm);
-  tm.collect(ms);
+  const bool timed = tm.collect(ms);
   if (rc) return rc;
+  if (!timed) return fail(AQ_ERR_HIP, "event timing failed");
   for (int c = 0; c < Timer::NCAT; ++c) launches[c] = tm.launches[c];
-  // active grid nodes of the last grid (blocks flagged by its grid update)
-  unsigned long long *cnt = (unsigned long long *)s->scount;  // sort counters are free here
+  // active grid nodes of the last grid (blocks flagged by its P2G)
+  unsigned long long *cnt = (unsigned long long *)s->tmp_ids;  // free after the last sort
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (uint32_t *)cnt, 0u, (int64_t)2);
   hipLaunchKernelGGL(k_count_active, dim3((s->nblocks + 255) / 256), dim3(256), 0, st, s->act[s->cur], s->nblocks,
                      cnt);
@@ -1182,7 +1501,7 @@ extern "C" int aq_update_limits(aq_sim *s, const void *state, int64_t n, float *
   if (!mxmn) return fail(AQ_ERR_INVALID, "null output");
   if (n == 0) return AQ_OK;
   hipStream_t st = (hipStream_t)stream;
-  StateK sk = make_state(s, state);
+  StateK sk = make_state(s, state, nullptr);
   dispatch(s, [&](auto tag_) {
     TAG_TYPES(tag_)
     hipLaunchKernelGGL((k_limits<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, s->lay, n, mxmn);
@@ -1199,7 +1518,7 @@ extern "C" int aq_energy(aq_sim *s, const void *state, int64_t n, double *out, v
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (uint32_t *)out, 0u, (int64_t)2);
   LAUNCH_CHECK();
   if (n == 0) return AQ_OK;
-  StateK sk = make_state(s, state);
+  StateK sk = make_state(s, state, nullptr);
   dispatch(s, [&](auto tag_) {
     TAG_TYPES(tag_)
     hipLaunchKernelGGL((k_energy<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, s->lay,

@@ -45,46 +45,42 @@ template <int DIM, int MAT> struct Comps {
 // ---------------------------------------------------------------------------
 struct StateK {
   void *ptr;
-  int64_t cap;        // plane stride
-  int32_t nw;         // quant: record words
-  uint32_t g2p_mask;  // quant: words holding x and J/F (what G2P reads)
+  int64_t cap;          // plane stride
+  const int32_t *ids;   // particle ids (permuted by sorts) or null: key of the dithering noise
+  int32_t nw;           // quant: record words
+  uint32_t g2p_mask;    // quant: words holding x and J/F (what G2P reads)
+  int32_t dither_any;   // quant: some member dithers (ids are read only then)
+  int32_t pad;
 };
 
-template <int NW32>
-__device__ __forceinline__ void load_words(const StateK &s, int64_t p, uint32_t mask,
-                                           uint32_t (&r)[NW32]) {
-  const uint64_t *w = (const uint64_t *)s.ptr;
-#pragma unroll
-  for (int i = 0; i < NW32 / 2; ++i) {
-    r[2 * i] = 0u;
-    r[2 * i + 1] = 0u;
-    if (i < s.nw && ((mask >> i) & 1u)) {
-      const uint64_t x = __builtin_nontemporal_load(w + (int64_t)i * s.cap + p);
-      r[2 * i] = (uint32_t)x;
-      r[2 * i + 1] = (uint32_t)(x >> 32);
-    }
-  }
+// Noise key of a particle: its id when ids are tracked, else its slot.
+__device__ __forceinline__ uint32_t noise_slot(const StateK &s, int64_t p) {
+  return (s.dither_any && s.ids) ? (uint32_t)s.ids[p] : (uint32_t)p;
 }
 
-template <int NW32>
-__device__ __forceinline__ void store_words(const StateK &s, int64_t p, const uint32_t (&r)[NW32]) {
-  uint64_t *w = (uint64_t *)s.ptr;
-#pragma unroll
-  for (int i = 0; i < NW32 / 2; ++i)
-    if (i < s.nw)
-      __builtin_nontemporal_store((uint64_t)r[2 * i] | ((uint64_t)r[2 * i + 1] << 32),
-                                  w + (int64_t)i * s.cap + p);
-}
+// Per-thread LDS rows of the record codec (odd stride: distinct banks per
+// lane); plain storage needs none.
+template <int ST, int NW32> struct RowsOf {
+  static constexpr int RS = ST == AQ_STORE_QUANT ? NW32 + 1 : 1;
+};
 
-// Load components [c0, c1) of particle p as T.
+// Load particle p (quantized: the words in `mask` go through the thread's
+// LDS row and every member is decoded from it).
 template <int ST, int NW32, int NC, class T>
 __device__ __forceinline__ void load_state(const StateK &s, const LayoutK &lay, int64_t p,
-                                           uint32_t mask, T (&v)[NC]) {
+                                           uint32_t mask, uint32_t *row, T (&v)[NC]) {
   if constexpr (ST == AQ_STORE_QUANT) {
-    uint32_t r[NW32];
-    load_words(s, p, mask, r);
+    const uint64_t *w = (const uint64_t *)s.ptr;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) v[c] = (T)decode_comp(r, lay.c[c]);
+    for (int i = 0; i < NW32 / 2; ++i) {
+      if (i < s.nw && ((mask >> i) & 1u)) {
+        const uint64_t x = __builtin_nontemporal_load(w + (int64_t)i * s.cap + p);
+        row[2 * i] = (uint32_t)x;
+        row[2 * i + 1] = (uint32_t)(x >> 32);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = (T)decode_row(row, lay.c[c]);
   } else {
     const T *vals = (const T *)s.ptr;
 #pragma unroll
@@ -92,18 +88,25 @@ __device__ __forceinline__ void load_state(const StateK &s, const LayoutK &lay, 
   }
 }
 
+// Store particle p; quantized records are encoded in the thread's LDS row,
+// which keeps the new record for a decode that follows.
 template <int ST, int NW32, int NC, class T>
 __device__ __forceinline__ void store_state(const StateK &s, const LayoutK &lay,
                                             const uint32_t *keys, int64_t p, T (&v)[NC],
-                                            uint32_t (&r)[NW32 > 0 ? NW32 : 1]) {
+                                            uint32_t *row) {
   if constexpr (ST == AQ_STORE_QUANT) {
     float f[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) f[c] = (float)v[c];
 #pragma unroll
-    for (int i = 0; i < NW32; ++i) r[i] = 0u;
-    encode_record<NC>(f, lay, keys, (uint32_t)p, r);
-    store_words(s, p, r);
+    for (int i = 0; i < NW32; ++i) row[i] = 0u;
+    encode_record<NC>(f, lay, keys, noise_slot(s, p), row);
+    uint64_t *w = (uint64_t *)s.ptr;
+#pragma unroll
+    for (int i = 0; i < NW32 / 2; ++i)
+      if (i < s.nw)
+        __builtin_nontemporal_store((uint64_t)row[2 * i] | ((uint64_t)row[2 * i + 1] << 32),
+                                    w + (int64_t)i * s.cap + p);
   } else {
     T *vals = (T *)s.ptr;
 #pragma unroll
@@ -210,24 +213,48 @@ __device__ __forceinline__ void affine_of(const T (&v)[NC], const ParamsT<T> &P,
 
 template <class T> __device__ __forceinline__ void atomic_add(T *a, T v) { unsafeAtomicAdd(a, v); }
 
-// P2G of one particle with global atomics.  Returns the block of its base.
+// Per-particle stencil of one particle: base cells, weights, fx.
+template <int DIM, class T> struct Stencil {
+  int base[3];
+  T fx[3];
+  T w[3][3];
+};
+
 template <int DIM, int MAT, class T, int NC>
-__device__ __forceinline__ int32_t p2g_particle(const T (&v)[NC], const ParamsT<T> &P,
-                                                vec4_t<T> *__restrict__ grid) {
+__device__ __forceinline__ Stencil<DIM, T> stencil_of(const T (&v)[NC], const ParamsT<T> &P) {
   using CI = Comps<DIM, MAT>;
-  int base[3] = {0, 0, 0};
-  T fx[3] = {0, 0, 0}, w[3][3];
+  Stencil<DIM, T> S;
+  S.base[1] = S.base[2] = 0;
+  S.fx[1] = S.fx[2] = (T)0;
 #pragma unroll
-  for (int a = 0; a < DIM; ++a) base[a] = stencil_1d<T>(v[CI::CX + a], P, fx[a], w[a]);
+  for (int a = 0; a < DIM; ++a) S.base[a] = stencil_1d<T>(v[CI::CX + a], P, S.fx[a], S.w[a]);
+  return S;
+}
+
+// Per-particle P2G terms (mpm_solver.py:608-613): affine matrix
+// stress * dt-term + p_mass * C, and momentum p_mass * v.
+template <int DIM, class T> struct P2GPrep {
   T aff[DIM * DIM];
-  affine_of<DIM, MAT, T>(v, P, aff);
   T mv[3];
+};
+
+template <int DIM, int MAT, class T, int NC>
+__device__ __forceinline__ P2GPrep<DIM, T> p2g_prepare(const T (&v)[NC], const ParamsT<T> &P) {
+  using CI = Comps<DIM, MAT>;
+  P2GPrep<DIM, T> R;
+  affine_of<DIM, MAT, T>(v, P, R.aff);
+  R.mv[1] = R.mv[2] = (T)0;
 #pragma unroll
-  for (int a = 0; a < DIM; ++a) mv[a] = P.p_mass * v[CI::CV + a];
-  int32_t o0[3], o1[3], o2[3] = {0, 0, 0};
-  axis_offsets<DIM>(base[0], 0, P.nb, o0);
-  axis_offsets<DIM>(base[1], 1, P.nb, o1);
-  if constexpr (DIM == 3) axis_offsets<DIM>(base[2], 2, P.nb, o2);
+  for (int a = 0; a < DIM; ++a) R.mv[a] = P.p_mass * v[CI::CV + a];
+  return R;
+}
+
+// Scatter of one particle (mpm_solver.py:615-622): calls
+// add(i, j, k, w * (m v + A dpos), w * m) for the 3^DIM stencil nodes
+// (i, j, k offsets from the base), in the reference's operand order.
+template <int DIM, class T, class F>
+__device__ __forceinline__ void p2g_emit(const P2GPrep<DIM, T> &R, const ParamsT<T> &P,
+                                         const Stencil<DIM, T> &S, F &&add) {
   constexpr int NK = DIM == 3 ? 3 : 1;
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -238,36 +265,63 @@ __device__ __forceinline__ int32_t p2g_particle(const T (&v)[NC], const ParamsT<
         const int off[3] = {i, j, k};
         T dpos[3];
 #pragma unroll
-        for (int a = 0; a < DIM; ++a) dpos[a] = ((T)off[a] - fx[a]) * P.dx;
+        for (int a = 0; a < DIM; ++a) dpos[a] = ((T)off[a] - S.fx[a]) * P.dx;
         T weight = (T)1.0;
 #pragma unroll
-        for (int a = 0; a < DIM; ++a) weight *= w[a][off[a]];
-        T *node = (T *)(grid + (o0[i] + o1[j] + o2[k]));
+        for (int a = 0; a < DIM; ++a) weight *= S.w[a][off[a]];
+        T mom[3] = {(T)0, (T)0, (T)0};
 #pragma unroll
         for (int a = 0; a < DIM; ++a) {
-          T ad = aff[a * DIM + 0] * dpos[0];
+          T ad = R.aff[a * DIM + 0] * dpos[0];
 #pragma unroll
-          for (int b = 1; b < DIM; ++b) ad = ad + aff[a * DIM + b] * dpos[b];
-          atomic_add<T>(node + a, weight * (mv[a] + ad));
+          for (int b = 1; b < DIM; ++b) ad = ad + R.aff[a * DIM + b] * dpos[b];
+          mom[a] = weight * (R.mv[a] + ad);
         }
-        atomic_add<T>(node + 3, weight * P.p_mass);
+        add(i, j, k, mom, weight * P.p_mass);
       }
-  return block_of_base<DIM>(base, P.nb);
 }
 
-// G2P of one particle: new state from old x, J/F and the grid velocities.
-template <int DIM, int MAT, class T, int NC>
-__device__ __forceinline__ void g2p_particle(const T (&v)[NC], const ParamsT<T> &P,
-                                             const vec4_t<T> *__restrict__ grid, T (&out)[NC]) {
-  using CI = Comps<DIM, MAT>;
-  int base[3] = {0, 0, 0};
-  T fx[3] = {0, 0, 0}, w[3][3];
+template <int DIM, int MAT, class T, int NC, class F>
+__device__ __forceinline__ void p2g_apply(const T (&v)[NC], const ParamsT<T> &P,
+                                          const Stencil<DIM, T> &S, F &&add) {
+  p2g_emit<DIM, T>(p2g_prepare<DIM, MAT, T>(v, P), P, S, add);
+}
+
+template <class T> __device__ __forceinline__ T tabs(T x) { return x < (T)0 ? -x : x; }
+
+// Upper bound of |contribution| of one particle to any of its stencil nodes,
+// per component (momentum x, y, z; mass): max weight * (|m v_a| +
+// sum_b |A_ab| * max |dpos_b|), with a 2^-10 margin for the rounding of the
+// contributions themselves.  NaN/Inf inputs give a non-finite bound.
+template <int DIM, class T>
+__device__ __forceinline__ void p2g_bounds(const P2GPrep<DIM, T> &R, const ParamsT<T> &P,
+                                           const Stencil<DIM, T> &S, T (&B)[4]) {
+  constexpr T margin = (T)1 + (T)1 / (T)1024;
+  T W = (T)1, D[3];
 #pragma unroll
-  for (int a = 0; a < DIM; ++a) base[a] = stencil_1d<T>(v[CI::CX + a], P, fx[a], w[a]);
-  int32_t o0[3], o1[3], o2[3] = {0, 0, 0};
-  axis_offsets<DIM>(base[0], 0, P.nb, o0);
-  axis_offsets<DIM>(base[1], 1, P.nb, o1);
-  if constexpr (DIM == 3) axis_offsets<DIM>(base[2], 2, P.nb, o2);
+  for (int a = 0; a < DIM; ++a) {
+    W *= tmax<T>(tmax<T>(S.w[a][0], S.w[a][1]), S.w[a][2]);
+    D[a] = tmax<T>(tabs<T>(S.fx[a]), tabs<T>((T)2 - S.fx[a])) * P.dx;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    B[a] = (T)0;
+    if (a < DIM) {
+      T acc = tabs<T>(R.mv[a]);
+#pragma unroll
+      for (int b = 0; b < DIM; ++b) acc += tabs<T>(R.aff[a * DIM + b]) * D[b];
+      B[a] = W * acc * margin;
+    }
+  }
+  B[3] = W * P.p_mass * margin;
+}
+
+// G2P of one particle (mpm_solver.py:682-702, 750-758): fetch(i, j, k)
+// returns the grid velocity of stencil node (i, j, k) as a vec4.
+template <int DIM, int MAT, class T, int NC, class F>
+__device__ __forceinline__ void g2p_apply(const T (&v)[NC], const ParamsT<T> &P,
+                                          const Stencil<DIM, T> &S, F &&fetch, T (&out)[NC]) {
+  using CI = Comps<DIM, MAT>;
   T nv[3] = {0, 0, 0}, nC[DIM * DIM];
 #pragma unroll
   for (int i = 0; i < DIM * DIM; ++i) nC[i] = (T)0;
@@ -281,12 +335,12 @@ __device__ __forceinline__ void g2p_particle(const T (&v)[NC], const ParamsT<T> 
         const int off[3] = {i, j, k};
         T dpos[3];
 #pragma unroll
-        for (int a = 0; a < DIM; ++a) dpos[a] = (T)off[a] - fx[a];
-        const vec4_t<T> nd = grid[o0[i] + o1[j] + o2[k]];
+        for (int a = 0; a < DIM; ++a) dpos[a] = (T)off[a] - S.fx[a];
+        const vec4_t<T> nd = fetch(i, j, k);
         const T g[3] = {nd.x, nd.y, nd.z};
         T weight = (T)1.0;
 #pragma unroll
-        for (int a = 0; a < DIM; ++a) weight *= w[a][off[a]];
+        for (int a = 0; a < DIM; ++a) weight *= S.w[a][off[a]];
 #pragma unroll
         for (int a = 0; a < DIM; ++a) nv[a] = nv[a] + weight * g[a];
         const T cw = P.c4 * weight;
@@ -323,6 +377,53 @@ __device__ __forceinline__ void g2p_particle(const T (&v)[NC], const ParamsT<T> 
         out[CI::CJ + a * DIM + b] = acc;
       }
   }
+}
+
+// Flag every grid block that the stencil of `base` touches (the base block
+// and, per axis, the next one when the stencil crosses a block edge).
+template <int DIM>
+__device__ __forceinline__ void mark_stencil_blocks(const int (&base)[3], int nb, uint32_t *flags) {
+  constexpr int LG = BlockShape<DIM>::LOG;
+  int lo[3], hi[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = a < DIM ? base[a] >> LG : 0;
+    hi[a] = a < DIM ? (base[a] + 2) >> LG : 0;
+  }
+  for (int i = lo[0]; i <= hi[0]; ++i)
+    for (int j = lo[1]; j <= hi[1]; ++j)
+      for (int k = lo[2]; k <= hi[2]; ++k)
+        flags[DIM == 3 ? (i * nb + j) * nb + k : i * nb + j] = 1u;
+}
+
+// P2G of one particle with global atomics.
+template <int DIM, int MAT, class T, int NC>
+__device__ __forceinline__ void p2g_particle(const T (&v)[NC], const ParamsT<T> &P,
+                                             vec4_t<T> *__restrict__ grid, uint32_t *__restrict__ flags) {
+  const Stencil<DIM, T> S = stencil_of<DIM, MAT, T>(v, P);
+  int32_t o0[3], o1[3], o2[3] = {0, 0, 0};
+  axis_offsets<DIM>(S.base[0], 0, P.nb, o0);
+  axis_offsets<DIM>(S.base[1], 1, P.nb, o1);
+  if constexpr (DIM == 3) axis_offsets<DIM>(S.base[2], 2, P.nb, o2);
+  p2g_apply<DIM, MAT, T>(v, P, S, [&](int i, int j, int k, const T (&mom)[3], T m) {
+    T *node = (T *)(grid + (o0[i] + o1[j] + o2[k]));
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) atomic_add<T>(node + a, mom[a]);
+    atomic_add<T>(node + 3, m);
+  });
+  mark_stencil_blocks<DIM>(S.base, P.nb, flags);
+}
+
+// G2P of one particle reading the grid from global memory.
+template <int DIM, int MAT, class T, int NC>
+__device__ __forceinline__ void g2p_particle(const T (&v)[NC], const ParamsT<T> &P,
+                                             const vec4_t<T> *__restrict__ grid, T (&out)[NC]) {
+  const Stencil<DIM, T> S = stencil_of<DIM, MAT, T>(v, P);
+  int32_t o0[3], o1[3], o2[3] = {0, 0, 0};
+  axis_offsets<DIM>(S.base[0], 0, P.nb, o0);
+  axis_offsets<DIM>(S.base[1], 1, P.nb, o1);
+  if constexpr (DIM == 3) axis_offsets<DIM>(S.base[2], 2, P.nb, o2);
+  g2p_apply<DIM, MAT, T>(v, P, S, [&](int i, int j, int k) { return grid[o0[i] + o1[j] + o2[k]]; }, out);
 }
 
 // Thread-block index remapped so each XCD (blocks are dealt round-robin to

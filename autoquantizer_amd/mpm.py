@@ -300,7 +300,7 @@ class MPMSolver:
 
     # ---- the substep (mpm_solver.py:624-631) ------------------------------
     def substep(self, s=0, t=None):
-        _lib.check(_lib.lib().aq_substep(self._h, _ptr(self.state), self.current_idx,
+        _lib.check(_lib.lib().aq_substep(self._h, _ptr(self.state), self.current_idx, _ptr(self.ids),
                                          self.step_count, _stream()))
         self.step_count += 1
 
@@ -315,8 +315,8 @@ class MPMSolver:
         _lib.check(_lib.lib().aq_grid_op(self._h, _stream()))
 
     def g2p_bls(self, s=0, t=None):
-        _lib.check(_lib.lib().aq_g2p(self._h, _ptr(self.state), self.current_idx, self.step_count,
-                                     _stream()))
+        _lib.check(_lib.lib().aq_g2p(self._h, _ptr(self.state), self.current_idx, _ptr(self.ids),
+                                     self.step_count, _stream()))
         self.step_count += 1
 
     def advance(self, nsteps, sort_every=0):

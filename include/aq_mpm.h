@@ -161,20 +161,25 @@ int aq_sim_create(const aq_sim_desc *desc, void *workspace, int64_t workspace_by
                   aq_sim **out);
 int aq_sim_destroy(aq_sim *sim);
 
-/* The reference substep (mpm_solver.py:624-631): grid.deactivate_all,
-   build_pid, p2g_bls, grid_op, g2p_bls, as separate launches.  `state` is
-   the words (AQ_STORE_QUANT) or value planes; n <= capacity. `step` keys the
-   dithering noise of the G2P encode. */
-int aq_substep(aq_sim *sim, void *state, int64_t n, uint32_t step, void *stream);
+/* The reference substep (mpm_solver.py:624-631): build_pid (aq_sort),
+   p2g_bls, grid_op, g2p_bls, as separate launches.  `state` is the words
+   (AQ_STORE_QUANT) or value planes; n <= capacity.  `ids` (device
+   int32[capacity] or NULL) are the particles' stable ids, permuted with the
+   state by the sort; the dithering noise of a particle is keyed by its id
+   (by its slot when ids is NULL) and by `step`.
+   aq_p2g and aq_g2p use the LDS-tiled kernels when the last aq_sort on this
+   state and particle count is still current, global-memory kernels
+   otherwise; the results are the same up to the order of float atomics. */
+int aq_substep(aq_sim *sim, void *state, int64_t n, int32_t *ids, uint32_t step, void *stream);
 int aq_p2g(aq_sim *sim, const void *state, int64_t n, void *stream);
 int aq_grid_op(aq_sim *sim, void *stream);
-int aq_g2p(aq_sim *sim, void *state, int64_t n, uint32_t step, void *stream);
+int aq_g2p(aq_sim *sim, void *state, int64_t n, const int32_t *ids, uint32_t step, void *stream);
 
 /* nsteps substeps with G2P of substep t fused with P2G of substep t+1
    (one launch reads the state once, writes it once and scatters the next
    grid).  Same results as nsteps aq_substep calls up to the order of the
    P2G float atomics.  Particles are re-sorted by grid block every
-   `sort_every` substeps (0 = never); ids (device int32[capacity] or NULL) is
+   `sort_every` substeps (0 = never: global-memory kernels); ids are
    permuted along. */
 int aq_advance(aq_sim *sim, void *state, int64_t n, int nsteps, uint32_t step0,
                int sort_every, int32_t *ids, void *stream);

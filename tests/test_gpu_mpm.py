@@ -1,19 +1,24 @@
 """GPU parity of the MPM substep kernels with the C oracle.
 
-- P2G grid (mass, momentum) within f32 accumulation-order tolerance;
-- G2P from the GPU's own grid: bit-exact with the oracle's G2P on that grid
-  (both unfused, same operand order);
-- whole quantized substeps: fixed-point codes within one code of the
-  oracle's, shared-exponent members within a few digits;
-- the fused G2P->P2G path against separate substeps;
+- P2G grid (mass, momentum) within f32 accumulation-order tolerance, for the
+  global-atomic and the LDS-tiled kernels;
+- grid update exact on the GPU's own P2G grid;
+- G2P from the GPU's own grid: bit-exact words with the oracle's G2P + encode
+  on that grid (both unfused, same operand order), global and tiled kernels;
+- whole substeps re-seeded from the GPU state each time: one GPU substep vs
+  one oracle substep from identical words; fixed-point codes within one step,
+  shared-exponent digits within two (only the P2G summation order differs);
+- multi-substep runs (fused G2P->P2G, sorting) against separate substeps,
+  statistically (chaotic amplification of rounding differences);
 - the plain fp32 path against the reference regression test_mpm88.py:97-104;
+- the plain fp64 path against the fp64 oracle;
 - sort, limits and energy.
 """
 import numpy as np
 import pytest
 from pytest import approx
 
-from gpu_util import state_layout, to_oracle
+from gpu_util import to_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -26,7 +31,8 @@ def _torch():
 
 
 def scene(orc, kind, n, seed=0, storage="quant"):
-    """Random blob scenes for the two benchmark materials."""
+    """Random blob scenes for the two benchmark materials (moderate C, F, J so
+    a few substeps stay smooth)."""
     _torch()
     import autoquantizer_amd as aq
     from autoquantizer_amd import layouts
@@ -55,24 +61,43 @@ def scene(orc, kind, n, seed=0, storage="quant"):
     vals = np.zeros((len(names), n))
     vals[:d] = rng.uniform(lo, hi, (d, n))
     vals[d:2 * d] = rng.uniform(-1, 1, (d, n))
-    vals[2 * d:2 * d + d * d] = rng.uniform(-3, 3, (d * d, n))
+    vals[2 * d:2 * d + d * d] = rng.uniform(-0.5, 0.5, (d * d, n))
     if kind == "elastic_2d":
-        vals[-4:] = np.eye(2).reshape(-1, 1) + rng.uniform(-0.05, 0.05, (4, n))
+        vals[-4:] = np.eye(2).reshape(-1, 1) + rng.uniform(-0.01, 0.01, (4, n))
     else:
-        vals[-1] = rng.uniform(0.95, 1.05, n)
+        vals[-1] = rng.uniform(0.99, 1.01, n)
     sim.set_state(vals)
     return sim, s
 
 
-def dense_to_oracle(g, d):
-    return np.ascontiguousarray(g[d]), np.ascontiguousarray(g[:d])
+def oracle_comps(sim):
+    return to_oracle(sim.comps)
+
+
+def step_sizes(sim, dec):
+    """Quantization step of every component at every particle: fixed-point
+    scale, or one digit of the shared-exponent group (the group's largest
+    member's power of two over 2^(digit bits - 1))."""
+    steps = np.zeros_like(dec)
+    for k, c in enumerate(sim.comps):
+        if c["kind"] == 0:
+            steps[k] = c["scale"]
+        else:
+            grp = [j for j, cc in enumerate(sim.comps) if cc["kind"] == 2 and cc["exp_offset"] == c["exp_offset"]]
+            mx = np.abs(dec[grp]).max(axis=0)
+            db = c["num_bits"] - c["is_signed"]
+            steps[k] = np.exp2(np.floor(np.log2(np.maximum(mx, 1e-30)))) * 2.0 ** (1 - db)
+    return steps
 
 
 @pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d"])
-def test_p2g_grid(orc, kind):
-    print(f"[p2g] {kind}", flush=True)
+@pytest.mark.parametrize("tiled", [False, True])
+def test_p2g_grid(orc, kind, tiled):
+    print(f"[p2g] {kind} tiled={tiled}", flush=True)
     sim, s = scene(orc, kind, 4000, seed=1)
-    st = sim.get_state().astype(np.float32)
+    if tiled:
+        sim.build_pid()
+    st = sim.get_state(original_order=False).astype(np.float32)
     sim.p2g_bls()
     g = sim.grid_dense()
     gm, gv = orc.grids(s)
@@ -84,26 +109,31 @@ def test_p2g_grid(orc, kind):
 
 
 @pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d"])
-def test_g2p_exact_on_gpu_grid(orc, kind):
+@pytest.mark.parametrize("tiled", [False, True])
+def test_g2p_exact_on_gpu_grid(orc, kind, tiled):
     """G2P is per-particle deterministic: with the GPU grid as input the oracle
-    G2P + encode gives the same words as the GPU's G2P."""
-    print(f"[g2p] {kind}", flush=True)
+    G2P + encode gives the same words as the GPU's G2P (noise keyed by the
+    particle id, so compared in the original particle order)."""
+    print(f"[g2p] {kind} tiled={tiled}", flush=True)
     sim, s = scene(orc, kind, 4000, seed=2)
-    st = sim.get_state(original_order=False).astype(np.float32)
+    if tiled:
+        sim.build_pid()
+    st = sim.get_state(original_order=True).astype(np.float32)
     sim.p2g_bls()
     sim.grid_op()
     g = sim.grid_dense()
     sim.g2p_bls()
-    words_gpu = sim.get_words()
+    words_gpu = sim.get_words(original_order=True)
     gv = np.ascontiguousarray(g[:s.dim]).astype(np.float32)
     out = orc.g2p(s, st, gv)
-    words_orc = orc.pack(to_oracle(sim.comps), sim.nwords, out, seed=11, step=0)
+    words_orc = orc.pack(oracle_comps(sim), sim.nwords, out, seed=11, step=0)
     assert np.array_equal(words_gpu, words_orc)
 
 
 @pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d"])
 def test_grid_op_exact(orc, kind):
-    """Grid update on the GPU's P2G grid equals the oracle's grid_op exactly."""
+    """Grid update on the GPU's P2G grid equals the oracle's grid_op exactly on
+    every node that received mass."""
     sim, s = scene(orc, kind, 3000, seed=3)
     sim.p2g_bls()
     g0 = sim.grid_dense()
@@ -113,81 +143,79 @@ def test_grid_op_exact(orc, kind):
     gm = np.ascontiguousarray(g0[d]).astype(np.float32)
     gv = np.ascontiguousarray(g0[:d]).astype(np.float32)
     orc.grid_op(s, gm, gv)
-    # only nodes in active blocks are updated on the GPU; all touched nodes are active
     touched = g0[d] != 0
     assert np.array_equal(g1[:d][:, touched].astype(np.float32), gv[:, touched])
 
 
-def code_diff(orc, comps, nw, a_words, b_words):
-    """Per-component decoded difference in units of the fixed-point step."""
-    oc = to_oracle(comps)
-    a = orc.unpack(oc, nw, a_words)
-    b = orc.unpack(oc, nw, b_words)
-    return a, b
-
-
 @pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d"])
-def test_substeps_vs_oracle(orc, kind):
-    """10 quantized substeps on the GPU and in the oracle from the same words:
-    fixed-point members agree to within 2 codes, custom floats to 1e-3 of the
-    field's magnitude (only the P2G summation order differs)."""
+def test_substeps_reseeded(orc, kind):
+    """Each GPU substep (sort + tiled P2G + grid + tiled G2P) from state S_t
+    against one oracle substep from the same words S_t."""
     print(f"[substeps] {kind}", flush=True)
     sim, s = scene(orc, kind, 3000, seed=4)
-    w0 = sim.get_words()
-    for _ in range(10):
+    oc = oracle_comps(sim)
+    worst = 0.0
+    for t in range(5):
+        w0 = sim.get_words(original_order=True)
         sim.substep()
-    wg = sim.get_words()
-    wo = w0.copy()
-    oc = to_oracle(sim.comps)
-    orc.run_quant(s, oc, sim.nwords, wo, 10, seed=11, step0=0)
-    a = orc.unpack(oc, sim.nwords, wg).astype(np.float64)
-    b = orc.unpack(oc, sim.nwords, wo).astype(np.float64)
-    for k, c in enumerate(sim.comps):
-        if c["kind"] == 0:
-            assert np.abs(a[k] - b[k]).max() <= 2.0 * c["scale"] + 1e-12, (k, c)
-        else:
-            tol = 1e-3 * max(np.abs(b[k]).max(), 1e-6)
-            assert np.abs(a[k] - b[k]).max() <= tol, k
+        wg = sim.get_words(original_order=True)
+        wo = w0.copy()
+        orc.run_quant(s, oc, sim.nwords, wo, 1, seed=11, step0=t)
+        a = orc.unpack(oc, sim.nwords, wg).astype(np.float64)
+        b = orc.unpack(oc, sim.nwords, wo).astype(np.float64)
+        steps = step_sizes(sim, b)
+        rel = np.abs(a - b) / steps
+        for k, c in enumerate(sim.comps):
+            lim = 1.0 if c["kind"] == 0 else 2.0
+            assert rel[k].max() <= lim + 1e-9, (t, k, rel[k].max())
+        # nearly all records are identical
+        assert np.mean((wg != wo).any(axis=0)) < 0.05
+        worst = max(worst, float(rel.max()))
+    print("max difference in quantization steps:", worst)
+
+
+def _close_runs(a, b, sim):
+    d = sim.dim
+    assert np.abs(a[:d] - b[:d]).max() < 1e-4
+    for sl in (slice(d, 2 * d), slice(2 * d, 2 * d + d * d), slice(2 * d + d * d, None)):
+        ra = np.sqrt(np.mean((a[sl] - b[sl]) ** 2))
+        assert ra <= 1e-2 * np.sqrt(np.mean(b[sl] ** 2)) + 1e-6
 
 
 def test_fused_advance_matches_substeps(orc):
-    """advance(n) (G2P fused with the next P2G) vs n separate substeps."""
+    """advance(n) (G2P fused with the next P2G, tiled) vs n separate substeps."""
     sim_a, s = scene(orc, "fluid_3d", 5000, seed=5)
     sim_b, _ = scene(orc, "fluid_3d", 5000, seed=5)
     for _ in range(8):
         sim_a.substep()
-    sim_b.advance(8)
-    a = sim_a.get_state()
-    b = sim_b.get_state()
-    for k, c in enumerate(sim_a.comps):
-        if c["kind"] == 0:
-            assert np.abs(a[k] - b[k]).max() <= 2.0 * c["scale"] + 1e-12, k
-        else:
-            assert np.abs(a[k] - b[k]).max() <= 1e-3 * max(np.abs(a[k]).max(), 1e-6), k
+    sim_b.advance(8, sort_every=4)
+    _close_runs(sim_b.get_state(), sim_a.get_state(), sim_a)
 
 
-def test_fused_advance_with_sort(orc):
-    """Sorting between fused substeps permutes particles but not physics."""
+def test_fused_untiled_matches_tiled(orc):
     sim_a, s = scene(orc, "fluid_3d", 5000, seed=6)
     sim_b, _ = scene(orc, "fluid_3d", 5000, seed=6)
     sim_a.advance(6)
     sim_b.advance(6, sort_every=2)
-    a = sim_a.get_state()
-    b = sim_b.get_state()
-    for k, c in enumerate(sim_a.comps):
-        if c["kind"] == 0:
-            assert np.abs(a[k] - b[k]).max() <= 3.0 * c["scale"] + 1e-12, k
+    _close_runs(sim_b.get_state(), sim_a.get_state(), sim_a)
+
+
+def test_fused_elastic(orc):
+    sim_a, s = scene(orc, "elastic_2d", 5000, seed=7)
+    sim_b, _ = scene(orc, "elastic_2d", 5000, seed=7)
+    for _ in range(6):
+        sim_a.substep()
+    sim_b.advance(6, sort_every=3)
+    _close_runs(sim_b.get_state(), sim_a.get_state(), sim_a)
 
 
 def test_sort_is_permutation(orc):
-    torch = _torch()
     sim, s = scene(orc, "fluid_3d", 7000, seed=7)
     w0 = sim.get_words(original_order=True)
     sim.build_pid()
     ids = sim.ids[:7000].cpu().numpy()
     assert np.array_equal(np.sort(ids), np.arange(7000))
     assert np.array_equal(sim.get_words(original_order=True), w0)
-    # sorted by grid block of the stencil base
     st = sim.get_state(original_order=False)
     base = np.clip(np.floor(st[:3] * 64 - 0.5), 0, 61).astype(int) // 4
     key = (base[0] * 16 + base[1]) * 16 + base[2]
@@ -195,7 +223,7 @@ def test_sort_is_permutation(orc):
 
 
 def test_mpm88_regression_gpu():
-    """test_mpm88.py:9-104 on the plain fp32 GPU path."""
+    """test_mpm88.py:9-104 on the plain fp32 GPU path (fused, sorted, tiled)."""
     _torch()
     import autoquantizer_amd as aq
     cfg = aq.MPMConfig()
@@ -213,7 +241,7 @@ def test_mpm88_regression_gpu():
     vals[3] = -3.0
     vals[8] = 1.0
     sim.set_state(vals)
-    sim.advance(500)
+    sim.advance(500, sort_every=8)
     pos = sim.get_state()[:2].T.copy()
     pos[:, 1] *= 2
     regression = [0.31722742, 0.15826741, 0.10224003, 0.07810827]
