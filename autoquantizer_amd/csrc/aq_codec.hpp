@@ -25,7 +25,7 @@ namespace aq {
 
 enum : uint8_t { K_FIXED = AQ_FIXED, K_INT = AQ_INT, K_FLOAT = AQ_FLOAT, K_F32 = AQ_F32 };
 
-// Kernel-side component: 24 bytes, uniform.
+// Kernel-side component: 32 bytes, uniform (one s_load_dwordx8).
 struct CompK {
   uint8_t kind, nbits, is_signed, dither;
   uint8_t exp_bits, shared, gfirst, glast;  // shared groups are contiguous runs
@@ -34,13 +34,43 @@ struct CompK {
   float enc_s;                               // (float)(1.0 / scale)
   float dec_s;                               // (float)scale
   float offs;                                // (float)offset
+  uint32_t pad;
 };
+static_assert(sizeof(CompK) == 32, "CompK is one scalar load");
 
 struct LayoutK {
   int32_t nc;
   int32_t nw;
   CompK c[AQ_MAX_COMPS];
 };
+
+// Record codec of the MPM kernels: component descriptors in device memory,
+// read where they are used (see comp_at), plus per-component bit masks of
+// the record's structure, which stay in scalar registers.
+struct CodecK {
+  const CompK *c;
+  int32_t nc, nw;
+  uint32_t shared_mask;  // bit c: member of a shared-exponent group
+  uint32_t gfirst_mask;  // bit c: first member of its group
+  uint32_t glast_mask;   // bit c: last member of its group
+  uint32_t key_base;     // dithering key of component c: hash32(c ^ key_base)
+};
+
+// Descriptor c, loaded at the point of use: the empty asm hides the pointer's
+// value from the optimiser so the scalar load is not hoisted to the kernel
+// entry (holding 16 descriptors there overflows the scalar registers).
+// (Through the constant address space the uniform load is a scalar load.)
+typedef const __attribute__((address_space(4))) uint32_t ConstU32;
+__device__ __forceinline__ CompK comp_at(const CodecK &cod, int c) {
+  ConstU32 *p = (ConstU32 *)cod.c;
+  asm volatile("" : "+s"(p));
+  uint32_t w[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) w[i] = p[c * 8 + i];
+  CompK k;
+  __builtin_memcpy(&k, w, sizeof(k));
+  return k;
+}
 
 __device__ __forceinline__ uint32_t lowmask(uint32_t nb) {
   return nb >= 32 ? 0xFFFFFFFFu : ((1u << nb) - 1u);
@@ -262,40 +292,63 @@ __device__ __forceinline__ void encode_member(float v, const CompK &k, float u, 
   }
 }
 
-// Encode NC values (compile-time count, components lay.c[0..NC)) into a
-// zeroed LDS row.  keys[c] is the dithering key of component c.
+// Dithering noise of component c of the record with noise key `slot`.
+__device__ __forceinline__ float comp_noise(const CodecK &cod, int c, uint32_t slot) {
+  return noise_u(slot, hash32((uint32_t)c ^ cod.key_base));
+}
+
+// Encode NC values (compile-time count) into a zeroed LDS row and return in
+// dec[] the values the stored fields decode to (the decode of each field's
+// digits, computed from the digits in registers).
 template <int NC>
-__device__ __forceinline__ void encode_record(const float (&v)[NC], const LayoutK &lay,
-                                              const uint32_t *keys, uint32_t slot, uint32_t *row) {
+__device__ __forceinline__ void encode_record(const float (&v)[NC], const CodecK &cod, uint32_t slot,
+                                              uint32_t *row, float (&dec)[NC]) {
+  // largest f32 exponent of each shared group, spread over its members
   uint32_t gmax[NC];
   uint32_t run = 0u;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    const CompK &k = lay.c[c];
     gmax[c] = 0u;
-    if (k.kind == K_FLOAT && k.shared) {
+    if ((cod.shared_mask >> c) & 1u) {
       const uint32_t e = f32_exp(v[c]);
-      run = k.gfirst ? e : (run > e ? run : e);
+      run = ((cod.gfirst_mask >> c) & 1u) ? e : (run > e ? run : e);
       gmax[c] = run;
     }
   }
 #pragma unroll
   for (int c = NC - 1; c >= 0; --c) {
-    const CompK &k = lay.c[c];
-    if (k.kind == K_FLOAT && k.shared) {
-      run = k.glast ? gmax[c] : run;
+    if ((cod.shared_mask >> c) & 1u) {
+      run = ((cod.glast_mask >> c) & 1u) ? gmax[c] : run;
       gmax[c] = run;
     }
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    const CompK &k = lay.c[c];
-    if (k.kind == K_FLOAT && k.shared) {
-      if (k.gfirst) row_put(row, k.eoff, k.exp_bits, shared_exp_field(gmax[c], k));
-      row_put(row, k.off, k.nbits, enc_shared(v[c], gmax[c], k));
+    const CompK k = comp_at(cod, c);
+    if ((cod.shared_mask >> c) & 1u) {
+      const uint32_t e = shared_exp_field(gmax[c], k);
+      if ((cod.gfirst_mask >> c) & 1u) row_put(row, k.eoff, k.exp_bits, e);
+      const uint32_t d = enc_shared(v[c], gmax[c], k);
+      row_put(row, k.off, k.nbits, d);
+      dec[c] = dec_shared(e, d, k);
+    } else if (k.kind == K_FIXED) {
+      const float u = k.dither ? comp_noise(cod, c, slot) : 0.0f;
+      const uint32_t q = enc_fixed(v[c], k, u);
+      row_put(row, k.off, k.nbits, q);
+      dec[c] = dec_fixed(q, k);
+    } else if (k.kind == K_FLOAT) {
+      uint32_t e;
+      const uint32_t d = enc_float(v[c], k, e);
+      row_put(row, k.eoff, k.exp_bits, e);
+      row_put(row, k.off, k.nbits, d);
+      dec[c] = dec_float(e, d, k);
+    } else if (k.kind == K_F32) {
+      row_put(row, k.off, 32u, __float_as_uint(v[c]));
+      dec[c] = v[c];
     } else {
-      const float u = (k.kind == K_FIXED && k.dither) ? noise_u(slot, keys[c]) : 0.0f;
-      encode_member(v[c], k, u, row, false);
+      const uint32_t q = cvt_sat(v[c], k.is_signed) & lowmask(k.nbits);
+      row_put(row, k.off, k.nbits, q);
+      dec[c] = k.is_signed ? (float)sext(q, k.nbits) : (float)q;
     }
   }
 }

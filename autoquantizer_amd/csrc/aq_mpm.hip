@@ -100,7 +100,8 @@ struct KeysK {
   uint32_t k[AQ_MAX_COMPS];
 };
 
-// Dithering keys of one (seed, step): key_c = h(c ^ h(step ^ h(seed_lo ^ h(seed_hi)))).
+// Dithering keys of one (seed, step): key_c = h(c ^ key_base),
+// key_base = h(step ^ h(seed_lo ^ h(seed_hi))).
 static uint32_t host_hash32(uint32_t x) {
   x ^= x >> 16;
   x *= 0x7feb352du;
@@ -109,10 +110,13 @@ static uint32_t host_hash32(uint32_t x) {
   x ^= x >> 16;
   return x;
 }
+static uint32_t key_base(uint64_t seed, uint32_t step) {
+  const uint32_t k = host_hash32((uint32_t)seed ^ host_hash32((uint32_t)(seed >> 32)));
+  return host_hash32(step ^ k);
+}
 static KeysK make_keys(uint64_t seed, uint32_t step) {
   KeysK K;
-  uint32_t k = host_hash32((uint32_t)seed ^ host_hash32((uint32_t)(seed >> 32)));
-  k = host_hash32(step ^ k);
+  const uint32_t k = key_base(seed, step);
   for (int c = 0; c < AQ_MAX_COMPS; ++c) K.k[c] = host_hash32((uint32_t)c ^ k);
   return K;
 }
@@ -391,74 +395,94 @@ extern "C" int aq_layout_plan(const aq_container *cs, int ncont, const aq_place 
 // ---------------------------------------------------------------------------
 // MPM kernels
 // ---------------------------------------------------------------------------
+// Untiled kernels (one particle per thread, grid in global memory).  P2G
+// scatters with float atomics into the accumulation grid grid_f and flags
+// the touched blocks; the gather kernel turns grid_f into grid_v.
 template <int DIM, int MAT, int ST, int NW32, class T>
-__global__ void __launch_bounds__(256) k_p2g(StateK s, LayoutK lay, ParamsT<T> P, int64_t n,
-                                             vec4_t<T> *__restrict__ grid, uint32_t *__restrict__ flags) {
+__global__ void __launch_bounds__(256) k_p2g(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
+                                             vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags) {
   __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
   const int64_t p = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= n) return;
   constexpr int NC = Comps<DIM, MAT>::NC;
   T v[NC];
-  load_state<ST, NW32, NC, T>(s, lay, p, 0xFFFFFFFFu, rows + threadIdx.x * RowsOf<ST, NW32>::RS, v);
-  p2g_particle<DIM, MAT, T>(v, P, grid, flags);
+  load_state<DIM, ST, NW32, NC, T>(s, cod, p, 0xFFFFFFFFu, rows + threadIdx.x * RowsOf<ST, NW32>::RS, v);
+  p2g_particle<DIM, MAT, T>(v, P, grid_f, fflags);
 }
 
 template <int DIM, int MAT, int ST, int NW32, class T>
-__global__ void __launch_bounds__(256) k_g2p(StateK s, LayoutK lay, KeysK keys, ParamsT<T> P, int64_t n,
-                                             const vec4_t<T> *__restrict__ grid) {
+__global__ void __launch_bounds__(256) k_g2p(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
+                                             const vec4_t<T> *__restrict__ grid_v) {
   __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
   const int64_t p = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= n) return;
   constexpr int NC = Comps<DIM, MAT>::NC;
   uint32_t *row = rows + threadIdx.x * RowsOf<ST, NW32>::RS;
   T v[NC], out[NC];
-  load_state<ST, NW32, NC, T>(s, lay, p, s.g2p_mask, row, v);
-  g2p_particle<DIM, MAT, T>(v, P, grid, out);
-  store_state<ST, NW32, NC, T>(s, lay, keys.k, p, out, row);
+  load_state<DIM, ST, NW32, NC, T, true>(s, cod, p, s.g2p_mask, row, v);
+  g2p_particle<DIM, MAT, T>(v, P, grid_v, out);
+  store_state<ST, NW32, NC, T>(s, cod, p, out, row);
 }
 
 // G2P of substep t (reading grid_v) fused with P2G of substep t+1
-// (scattering into grid_s): the state is read once and written once.
+// (scattering into grid_f): the state is read once and written once.
 template <int DIM, int MAT, int ST, int NW32, class T>
-__global__ void __launch_bounds__(256) k_g2p2g(StateK s, LayoutK lay, KeysK keys, ParamsT<T> P, int64_t n,
+__global__ void __launch_bounds__(256) k_g2p2g(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
                                                const vec4_t<T> *__restrict__ grid_v,
-                                               vec4_t<T> *__restrict__ grid_s, uint32_t *__restrict__ flags_s) {
+                                               vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags) {
   __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
   const int64_t p = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= n) return;
   constexpr int NC = Comps<DIM, MAT>::NC;
   uint32_t *row = rows + threadIdx.x * RowsOf<ST, NW32>::RS;
   T v[NC], out[NC];
-  load_state<ST, NW32, NC, T>(s, lay, p, s.g2p_mask, row, v);
+  load_state<DIM, ST, NW32, NC, T, true>(s, cod, p, s.g2p_mask, row, v);
   g2p_particle<DIM, MAT, T>(v, P, grid_v, out);
-  store_state<ST, NW32, NC, T>(s, lay, keys.k, p, out, row);
-  if constexpr (ST == AQ_STORE_QUANT) {
-    // the next P2G sees what storage holds: decode the record just written
-#pragma unroll
-    for (int c = 0; c < NC; ++c) out[c] = (T)decode_row(row, lay.c[c]);
-  }
-  p2g_particle<DIM, MAT, T>(out, P, grid_s, flags_s);
+  store_state<ST, NW32, NC, T>(s, cod, p, out, row);  // out: what storage now holds
+  p2g_particle<DIM, MAT, T>(out, P, grid_f, fflags);
 }
 
 // ---- LDS-tiled kernels over work items ---------------------------------------
 // A work item is up to 256 particles whose stencil base was in one grid block
 // at the last aq_sort: int4 (block, first particle, count, 0).  The item's
 // tile covers the block, the 2-node stencil reach and a 1-cell drift margin
-// on each side: TE = E + 4 nodes per axis starting at block_origin - 1.
-// G2P reads grid velocities staged in LDS.  P2G accumulates into an LDS tile
-// in 64-bit fixed point (integer LDS atomics; float LDS atomics are ~30x
-// slower on this chip): per item and component the contributions are scaled
-// by 2^k with 2^(61-k) above the item's summed contribution bound, so the
-// integer sums cannot overflow and are exact sums of the rounded terms
-// (quantum 2^-61 of the bound); each non-empty node is then flushed once with
-// global float atomics.  A particle whose stencil leaves the tile (it drifted
-// more than a cell since the sort), or an item with a non-finite bound,
-// reads and scatters through global memory.
+// on each side: TE = E + 4 nodes per axis from block_origin - 1, stored as
+// 2^DIM groups of 64 slots (slot = group * 64 + the node's index inside an
+// E^DIM box), so the nodes of any E^DIM window fall in 64 distinct LDS banks.
+//
+// G2P reads grid velocities staged in LDS.  P2G accumulates in LDS in fixed
+// point (integer LDS atomics; float LDS atomics are ~30x slower on this
+// chip): per item and component the contributions are scaled by 2^k, with
+// 2^(BITS-k) above the item's summed contribution bound, so the integer sums
+// cannot overflow and are exact sums of the terms rounded to 2^-BITS of that
+// bound (BITS = 30 in int32 for f32 compute, 62 in int64 for f64).  The
+// finished tile goes to the item's slot of the tile buffer with plain stores;
+// the gather kernel sums the tiles covering each node in a fixed order.  A
+// particle whose stencil leaves the tile (it drifted more than a cell since
+// the sort), or an item with a non-finite bound, reads grid_v and scatters
+// into grid_f.
 template <int DIM> struct Tile {
-  static constexpr int E = BlockShape<DIM>::E;
+  static constexpr int E = BlockShape<DIM>::E, LG = BlockShape<DIM>::LOG;
   static constexpr int TE = E + 4;
-  static constexpr int TN = DIM == 3 ? TE * TE * TE : TE * TE;
-  __device__ static int index(int x, int y, int z) { return DIM == 3 ? (x * TE + y) * TE + z : x * TE + y; }
+  static constexpr int TS = (1 << DIM) * 64;
+  __device__ static int slot(int x, int y, int z) {
+    if constexpr (DIM == 3)
+      return ((((x >> 2) * 2 + (y >> 2)) * 2 + (z >> 2)) << 6) | ((x & 3) << 4) | ((y & 3) << 2) | (z & 3);
+    else
+      return (((x >> 3) * 2 + (y >> 3)) << 6) | ((x & 7) << 3) | (y & 7);
+  }
+  __device__ static void coords(int q, int (&t)[3]) {
+    const int l = q & 63, g = q >> 6;
+    if constexpr (DIM == 3) {
+      t[0] = ((g >> 2) & 1) * 4 + (l >> 4);
+      t[1] = ((g >> 1) & 1) * 4 + ((l >> 2) & 3);
+      t[2] = (g & 1) * 4 + (l & 3);
+    } else {
+      t[0] = ((g >> 1) & 1) * 8 + (l >> 3);
+      t[1] = (g & 1) * 8 + (l & 7);
+      t[2] = 0;
+    }
+  }
 };
 
 template <int DIM>
@@ -486,47 +510,40 @@ __device__ __forceinline__ int64_t node_index(int i, int j, int k, int nb) {
   }
 }
 
-template <int DIM>
-__device__ __forceinline__ void tile_coords(int q, int (&t)[3]) {
-  constexpr int TE = Tile<DIM>::TE;
-  if constexpr (DIM == 3) {
-    t[2] = q % TE;
-    t[1] = (q / TE) % TE;
-    t[0] = q / (TE * TE);
-  } else {
-    t[1] = q % TE;
-    t[0] = q / TE;
-    t[2] = 0;
-  }
-}
-
 template <class T> __device__ __forceinline__ T tldexp(T x, int k);
 template <> __device__ __forceinline__ float tldexp<float>(float x, int k) { return ldexpf(x, k); }
 template <> __device__ __forceinline__ double tldexp<double>(double x, int k) { return ldexp(x, k); }
-template <class T> __device__ __forceinline__ T trint(T x);
-template <> __device__ __forceinline__ float trint<float>(float x) { return rintf(x); }
-template <> __device__ __forceinline__ double trint<double>(double x) { return rint(x); }
 template <class T> __device__ __forceinline__ int texp(T x);  // x = m 2^e, 0.5 <= |m| < 1
 template <> __device__ __forceinline__ int texp<float>(float x) { int e; frexpf(x, &e); return e; }
 template <> __device__ __forceinline__ int texp<double>(double x) { int e; frexp(x, &e); return e; }
 
-// round(c * 2^k) as a two's-complement int64, for |c * 2^k| < 2^62, from
-// exact steps: |f| = hi * 2^32 + lo with hi < 2^30, and lo = |f| - hi * 2^32
-// holds only bits of |f| below 2^32 (representable in T).
-template <class T> __device__ __forceinline__ uint64_t to_fixed(T c, int k) {
-  const T f = trint<T>(tldexp<T>(c, k));
-  const T a = tabs<T>(f);
-  const T hi = tfloor<T>(tldexp<T>(a, -32));
-  const T lo = a - hi * (T)4294967296.0;
-  const uint64_t u = ((uint64_t)(uint32_t)hi << 32) + (uint64_t)(uint32_t)lo;
-  return f < (T)0 ? (uint64_t)0 - u : u;
-}
-
-// q * 2^-k rounded once to double, then to T.
-template <class T> __device__ __forceinline__ T from_fixed(uint64_t q, int k) {
-  const double d = (double)(int32_t)((int64_t)q >> 32) * 4294967296.0 + (double)(uint32_t)q;
-  return (T)ldexp(d, -k);
-}
+// Fixed-point accumulator of the P2G tile.
+template <class T> struct AccOf;
+template <> struct AccOf<float> {
+  using U = uint32_t;
+  static constexpr int BITS = 30;
+  // round(x) as int32 (two's complement), |x| < 2^31
+  __device__ static U fix(float x) { return (U)(int32_t)rintf(x); }
+  __device__ static float unfix(U q, int k) { return ldexpf((float)(int32_t)q, -k); }
+};
+template <> struct AccOf<double> {
+  using U = unsigned long long;
+  static constexpr int BITS = 62;
+  // round(x) as int64 from exact steps, |x| < 2^62: |f| = hi 2^32 + lo with
+  // hi < 2^30 and lo = |f| - hi 2^32 an integer below 2^32
+  __device__ static U fix(double x) {
+    const double f = rint(x);
+    const double a = f < 0.0 ? -f : f;
+    const double hi = floor(ldexp(a, -32));
+    const double lo = a - hi * 4294967296.0;
+    const U u = ((U)(uint32_t)hi << 32) + (U)(uint32_t)lo;
+    return f < 0.0 ? (U)0 - u : u;
+  }
+  __device__ static double unfix(U q, int k) {
+    const double d = (double)(int32_t)((long long)q >> 32) * 4294967296.0 + (double)(uint32_t)q;
+    return ldexp(d, -k);
+  }
+};
 
 template <class T> __device__ __forceinline__ T wave_sum(T x) {
 #pragma unroll
@@ -535,20 +552,22 @@ template <class T> __device__ __forceinline__ T wave_sum(T x) {
 }
 
 template <int DIM, int MAT, int ST, int NW32, class T, bool DO_G2P, bool DO_P2G>
-__global__ void __launch_bounds__(256) k_tiled(StateK s, LayoutK lay, KeysK keys, ParamsT<T> P, int64_t n,
+__global__ void __launch_bounds__(256) k_tiled(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
                                                const int4 *__restrict__ items, const int32_t *__restrict__ nitems,
-                                               const vec4_t<T> *__restrict__ grid_v, vec4_t<T> *__restrict__ grid_s,
-                                               uint32_t *__restrict__ flags_s) {
+                                               const vec4_t<T> *__restrict__ grid_v, vec4_t<T> *__restrict__ tiles,
+                                               vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags) {
   using TL = Tile<DIM>;
-  constexpr int TN = TL::TN, TE = TL::TE, NC = Comps<DIM, MAT>::NC;
-  __shared__ vec4_t<T> tv[DO_G2P ? TN : 1];
-  __shared__ unsigned long long ts[DO_P2G ? 4 : 1][DO_P2G ? TN : 1];
+  using ACC = AccOf<T>;
+  using U = typename ACC::U;
+  constexpr int TS = TL::TS, TE = TL::TE, NC = Comps<DIM, MAT>::NC;
+  __shared__ vec4_t<T> tv[DO_G2P ? TS : 1];
+  __shared__ U ts[DO_P2G ? 4 : 1][DO_P2G ? TS : 1];
   __shared__ T bpart[4][4];
   __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   uint32_t *row = rows + tid * RowsOf<ST, NW32>::RS;
   if constexpr (DO_P2G)
-    for (int q = tid; q < TN; q += 256) ts[0][q] = ts[1][q] = ts[2][q] = ts[3][q] = 0ull;
+    for (int q = tid; q < TS; q += 256) ts[0][q] = ts[1][q] = ts[2][q] = ts[3][q] = (U)0;
   const int ni = *nitems;
   // each XCD (blocks dealt round-robin to 8 XCDs) walks a contiguous item range
   const int xcd = (int)(blockIdx.x & 7u);
@@ -561,25 +580,26 @@ __global__ void __launch_bounds__(256) k_tiled(StateK s, LayoutK lay, KeysK keys
 #pragma unroll
     for (int a = 0; a < 3; ++a) o[a] = a < DIM ? bc[a] * TL::E - 1 : 0;
     if constexpr (DO_G2P) {
-      for (int q = tid; q < TN; q += 256) {
+      for (int q = tid; q < TS; q += 256) {
         int t[3];
-        tile_coords<DIM>(q, t);
+        TL::coords(q, t);
         const int gi = o[0] + t[0], gj = o[1] + t[1], gk = o[2] + t[2];
         vec4_t<T> val;
         val.x = val.y = val.z = val.w = (T)0;
-        const bool in = gi >= 0 && gi < P.n && gj >= 0 && gj < P.n && (DIM == 2 || (gk >= 0 && gk < P.n));
+        const bool in = t[0] < TE && t[1] < TE && (DIM == 2 || t[2] < TE) && gi >= 0 && gi < P.n && gj >= 0 &&
+                        gj < P.n && (DIM == 2 || (gk >= 0 && gk < P.n));
         if (in) val = grid_v[node_index<DIM>(gi, gj, gk, P.nb)];
         tv[q] = val;
       }
     }
-    __syncthreads();  // tile staged; previous item's flush (and zeroing) done
+    __syncthreads();  // tile staged; the previous item's tile written out
     const bool has = tid < item.z && (int64_t)item.y + tid < n;
     const int64_t p = (int64_t)item.y + tid;
     T out[NC];
     if (has) {
       if constexpr (DO_G2P) {
         T v[NC];
-        load_state<ST, NW32, NC, T>(s, lay, p, s.g2p_mask, row, v);
+        load_state<DIM, ST, NW32, NC, T, true>(s, cod, p, s.g2p_mask, row, v);
         const Stencil<DIM, T> S = stencil_of<DIM, MAT, T>(v, P);
         int rel[3] = {0, 0, 0};
         bool inside = true;
@@ -589,36 +609,32 @@ __global__ void __launch_bounds__(256) k_tiled(StateK s, LayoutK lay, KeysK keys
           inside = inside && rel[a] >= 0 && rel[a] + 2 < TE;
         }
         if (inside) {
-          g2p_apply<DIM, MAT, T>(v, P, S, [&](int i, int j, int k) { return tv[TL::index(rel[0] + i, rel[1] + j, rel[2] + k)]; }, out);
+          g2p_apply<DIM, MAT, T>(
+              v, P, S, [&](int i, int j, int k) { return tv[TL::slot(rel[0] + i, rel[1] + j, rel[2] + k)]; }, out);
         } else {
           g2p_particle<DIM, MAT, T>(v, P, grid_v, out);
         }
-        store_state<ST, NW32, NC, T>(s, lay, keys.k, p, out, row);
-        if constexpr (ST == AQ_STORE_QUANT && DO_P2G) {
-          // the next P2G sees what storage holds: decode the record just written
-#pragma unroll
-          for (int c = 0; c < NC; ++c) out[c] = (T)decode_row(row, lay.c[c]);
-        }
+        store_state<ST, NW32, NC, T>(s, cod, p, out, row);  // out: what storage now holds
       } else {
-        load_state<ST, NW32, NC, T>(s, lay, p, 0xFFFFFFFFu, row, out);
+        load_state<DIM, ST, NW32, NC, T>(s, cod, p, 0xFFFFFFFFu, row, out);
       }
     }
     if constexpr (DO_P2G) {
       Stencil<DIM, T> S;
-      P2GPrep<DIM, T> R;
+      P2GTerms<DIM, T> R;
       int rel[3] = {0, 0, 0};
       bool inside = false;
       T B[4] = {(T)0, (T)0, (T)0, (T)0};
       if (has) {
         S = stencil_of<DIM, MAT, T>(out, P);
-        R = p2g_prepare<DIM, MAT, T>(out, P);
+        R = p2g_terms<DIM, MAT, T>(out, P, S);
         inside = true;
 #pragma unroll
         for (int a = 0; a < DIM; ++a) {
           rel[a] = S.base[a] - o[a];
           inside = inside && rel[a] >= 0 && rel[a] + 2 < TE;
         }
-        if (inside) p2g_bounds<DIM, T>(R, P, S, B);
+        if (inside) p2g_bounds<DIM, T>(R, S, B);
       }
 #pragma unroll
       for (int c = 0; c < 4; ++c) B[c] = wave_sum<T>(B[c]);
@@ -632,37 +648,38 @@ __global__ void __launch_bounds__(256) k_tiled(StateK s, LayoutK lay, KeysK keys
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const T sum = ((bpart[0][c] + bpart[1][c]) + bpart[2][c]) + bpart[3][c];
-        finite = finite && sum <= (T)3.0e38;
-        k[c] = 61 - texp<T>(sum * ((T)1 + (T)1 / (T)1024));
+        finite = finite && sum <= (T)1.0e37;
+        k[c] = ACC::BITS - texp<T>(sum * ((T)1 + (T)1 / (T)1024));
       }
       if (has) {
         if (inside && finite) {
-          p2g_emit<DIM, T>(R, P, S, [&](int i, int j, int kk, const T (&mom)[3], T m) {
-            const int q = TL::index(rel[0] + i, rel[1] + j, rel[2] + kk);
 #pragma unroll
-            for (int a = 0; a < DIM; ++a) atomicAdd(&ts[a][q], (unsigned long long)to_fixed<T>(mom[a], k[a]));
-            atomicAdd(&ts[3][q], (unsigned long long)to_fixed<T>(m, k[3]));
+          for (int a = 0; a < DIM; ++a) {
+            R.c[a] = tldexp<T>(R.c[a], k[a]);
+#pragma unroll
+            for (int b = 0; b < DIM; ++b) R.A[a][b] = tldexp<T>(R.A[a][b], k[a]);
+          }
+          R.pm = tldexp<T>(R.pm, k[3]);
+          p2g_emit<DIM, T>(R, S, [&](int i, int j, int kk, const T (&mom)[3], T m) {
+            const int q = TL::slot(rel[0] + i, rel[1] + j, rel[2] + kk);
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) atomicAdd(&ts[a][q], ACC::fix(mom[a]));
+            atomicAdd(&ts[3][q], ACC::fix(m));
           });
         } else {
-          p2g_particle<DIM, MAT, T>(out, P, grid_s, flags_s);
+          p2g_particle<DIM, MAT, T>(out, P, grid_f, fflags);
         }
       }
       __syncthreads();  // tile sums complete
-      for (int q = tid; q < TN; q += 256) {
-        const uint64_t qm = ts[3][q], qx = ts[0][q], qy = ts[1][q], qz = DIM == 3 ? ts[2][q] : 0ull;
-        if ((qm | qx | qy | qz) != 0ull) {
-          int t[3];
-          tile_coords<DIM>(q, t);
-          const int gi = o[0] + t[0], gj = o[1] + t[1], gk = o[2] + t[2];
-          const int64_t node = node_index<DIM>(gi, gj, gk, P.nb);
-          T *nd = (T *)(grid_s + node);
-          atomic_add<T>(nd + 0, from_fixed<T>(qx, k[0]));
-          atomic_add<T>(nd + 1, from_fixed<T>(qy, k[1]));
-          if (DIM == 3) atomic_add<T>(nd + 2, from_fixed<T>(qz, k[2]));
-          atomic_add<T>(nd + 3, from_fixed<T>(qm, k[3]));
-          flags_s[node >> 6] = 1u;
-          ts[0][q] = ts[1][q] = ts[2][q] = ts[3][q] = 0ull;
-        }
+      vec4_t<T> *dst = tiles + (int64_t)it * TS;
+      for (int q = tid; q < TS; q += 256) {
+        vec4_t<T> val;
+        val.x = ACC::unfix(ts[0][q], k[0]);
+        val.y = ACC::unfix(ts[1][q], k[1]);
+        val.z = DIM == 3 ? ACC::unfix(ts[2][q], k[2]) : (T)0;
+        val.w = ACC::unfix(ts[3][q], k[3]);
+        dst[q] = val;
+        ts[0][q] = ts[1][q] = ts[2][q] = ts[3][q] = (U)0;
       }
     } else {
       __syncthreads();  // G2P done with the tile before the next item stages it
@@ -670,46 +687,9 @@ __global__ void __launch_bounds__(256) k_tiled(StateK s, LayoutK lay, KeysK keys
   }
 }
 
-// Grid update (mpm_solver.py:633-665) on one 64-node block per wavefront,
-// for the blocks flagged by P2G.  Optionally zeroes the blocks of another
-// grid that were active (the grid consumed by the previous G2P).
+// Grid update (mpm_solver.py:633-665) of node I with accumulated (m v, m).
 template <int DIM, class T>
-__global__ void __launch_bounds__(256) k_grid_update(ParamsT<T> P, int32_t nblocks, vec4_t<T> *__restrict__ grid,
-                                                     uint32_t *__restrict__ flags, uint32_t *__restrict__ act,
-                                                     vec4_t<T> *__restrict__ other, uint32_t *__restrict__ oflags,
-                                                     uint32_t *__restrict__ oact) {
-  const int32_t b = (int32_t)(blockIdx.x * 4u + (threadIdx.x >> 6));
-  const int lane = threadIdx.x & 63;
-  if (b >= nblocks) return;
-  if (other) {
-    const uint32_t was = oact[b];
-    if (was) {
-      vec4_t<T> z;
-      z.x = z.y = z.z = z.w = (T)0;
-      other[(int64_t)b * 64 + lane] = z;
-    }
-    __builtin_amdgcn_wave_barrier();
-    if (lane == 0 && was) {
-      oflags[b] = 0u;
-      oact[b] = 0u;
-    }
-  }
-  if (!flags[b]) return;
-  if (lane == 0) act[b] = 1u;
-  const int nb = P.nb;
-  int bc[3];
-  block_coords<DIM>(b, nb, bc);
-  int I[3];
-  if constexpr (DIM == 3) {
-    I[0] = bc[0] * 4 + (lane >> 4);
-    I[1] = bc[1] * 4 + ((lane >> 2) & 3);
-    I[2] = bc[2] * 4 + (lane & 3);
-  } else {
-    I[0] = bc[0] * 8 + (lane >> 3);
-    I[1] = bc[1] * 8 + (lane & 7);
-    I[2] = 0;
-  }
-  vec4_t<T> nd = grid[(int64_t)b * 64 + lane];
+__device__ __forceinline__ vec4_t<T> grid_op_node(const ParamsT<T> &P, const int (&I)[3], vec4_t<T> nd) {
   const T inv_m = (T)1 / (nd.w + P.eps_m);
   T v[3] = {inv_m * nd.x, inv_m * nd.y, inv_m * nd.z};
   v[1] = v[1] - P.g_dt;
@@ -729,36 +709,135 @@ __global__ void __launch_bounds__(256) k_grid_update(ParamsT<T> P, int32_t nbloc
   nd.x = v[0];
   nd.y = v[1];
   nd.z = DIM == 3 ? v[2] : (T)0;
-  grid[(int64_t)b * 64 + lane] = nd;
+  return nd;
 }
 
-// Work items from the block counts of a sort: items per block = ceil(count / 256);
-// one workgroup scans them (item_off[b], total in item_off[m]), then each
-// block writes its items.
-__global__ void __launch_bounds__(1024) k_items_scan(const uint32_t *__restrict__ counts,
-                                                     int32_t *__restrict__ item_off, int32_t m) {
-  __shared__ int32_t part[1024];
-  const int t = threadIdx.x;
-  const int per = (m + 1023) / 1024;
-  const int lo = t * per, hi = lo + per < m ? lo + per : m;
-  int32_t sum = 0;
-  for (int i = lo; i < hi; ++i) sum += (int32_t)((counts[i] + 255u) >> 8);
-  part[t] = sum;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int32_t x = t >= off ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += x;
-    __syncthreads();
+// Local coordinates of lane `lane` in a 64-node block.
+template <int DIM> __device__ __forceinline__ void lane_coords(int lane, int (&l)[3]) {
+  if constexpr (DIM == 3) {
+    l[0] = lane >> 4;
+    l[1] = (lane >> 2) & 3;
+    l[2] = lane & 3;
+  } else {
+    l[0] = lane >> 3;
+    l[1] = lane & 7;
+    l[2] = 0;
   }
-  int32_t run = part[t] - sum;
-  for (int i = lo; i < hi; ++i) {
-    item_off[i] = run;
-    run += (int32_t)((counts[i] + 255u) >> 8);
-  }
-  if (t == 1023) item_off[m] = part[1023];
 }
 
+// Gather of P2G results into grid_v, one 64-node block per wavefront: the
+// tiles of the items of the 3^DIM neighbouring blocks that cover each node
+// (in a fixed order: neighbour blocks, then items), plus grid_f for blocks
+// flagged by untiled scatters (grid_f is zeroed after use).  OP applies the
+// grid update; without it grid_v holds (m v, m).  Blocks with no
+// contribution are marked inactive in `act` (and zeroed if they were active).
+template <int DIM, class T, bool OP>
+__global__ void __launch_bounds__(256) k_gather(ParamsT<T> P, int32_t nblocks, const int32_t *__restrict__ item_off,
+                                                const vec4_t<T> *__restrict__ tiles, vec4_t<T> *__restrict__ grid_f,
+                                                uint32_t *__restrict__ fflags, vec4_t<T> *__restrict__ grid_v,
+                                                uint32_t *__restrict__ act) {
+  using TL = Tile<DIM>;
+  constexpr int E = TL::E, TE = TL::TE, TS = TL::TS;
+  const int32_t b = __builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * 4u + (threadIdx.x >> 6)));
+  const int lane = threadIdx.x & 63;
+  if (b >= nblocks) return;
+  const int nb = P.nb;
+  int bc[3], loc[3];
+  block_coords<DIM>(b, nb, bc);
+  lane_coords<DIM>(lane, loc);
+  const bool fall = fflags[b] != 0u;
+  bool any = fall;
+  vec4_t<T> sum;
+  sum.x = sum.y = sum.z = sum.w = (T)0;
+  if (item_off) {
+    constexpr int DZ = DIM == 3 ? 1 : 0;
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dz = -DZ; dz <= DZ; ++dz) {
+          const int d[3] = {dx, dy, dz};
+          int nbc[3];
+          bool ok = true;
+#pragma unroll
+          for (int a = 0; a < DIM; ++a) {
+            nbc[a] = bc[a] + d[a];
+            ok = ok && nbc[a] >= 0 && nbc[a] < nb;
+          }
+          if (!ok) continue;
+          const int32_t nbid = DIM == 3 ? (nbc[0] * nb + nbc[1]) * nb + nbc[2] : nbc[0] * nb + nbc[1];
+          const int32_t r0 = item_off[nbid], r1 = item_off[nbid + 1];
+          if (r1 <= r0) continue;
+          any = true;
+          int t[3] = {0, 0, 0};
+          bool cov = true;
+#pragma unroll
+          for (int a = 0; a < DIM; ++a) {
+            t[a] = loc[a] + 1 - d[a] * E;
+            cov = cov && t[a] >= 0 && t[a] < TE;
+          }
+          if (cov) {
+            const int q = TL::slot(t[0], t[1], t[2]);
+            for (int32_t it = r0; it < r1; ++it) {
+              const vec4_t<T> c = tiles[(int64_t)it * TS + q];
+              sum.x += c.x;
+              sum.y += c.y;
+              sum.z += c.z;
+              sum.w += c.w;
+            }
+          }
+        }
+  }
+  const int64_t node = (int64_t)b * 64 + lane;
+  if (!any) {
+    if (act[b]) {
+      vec4_t<T> z;
+      z.x = z.y = z.z = z.w = (T)0;
+      grid_v[node] = z;
+      __builtin_amdgcn_wave_barrier();
+      if (lane == 0) act[b] = 0u;
+    }
+    return;
+  }
+  if (fall) {
+    const vec4_t<T> c = grid_f[node];
+    sum.x += c.x;
+    sum.y += c.y;
+    sum.z += c.z;
+    sum.w += c.w;
+    vec4_t<T> z;
+    z.x = z.y = z.z = z.w = (T)0;
+    grid_f[node] = z;
+  }
+  if constexpr (OP) {
+    int I[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) I[a] = bc[a] * E + loc[a];
+    sum = grid_op_node<DIM, T>(P, I, sum);
+  }
+  grid_v[node] = sum;
+  __builtin_amdgcn_wave_barrier();
+  if (lane == 0) {
+    act[b] = 1u;
+    if (fall) fflags[b] = 0u;
+  }
+}
+
+// Grid update in place on the active blocks of grid_v (after a gather without OP).
+template <int DIM, class T>
+__global__ void __launch_bounds__(256) k_grid_update(ParamsT<T> P, int32_t nblocks, vec4_t<T> *__restrict__ grid_v,
+                                                     const uint32_t *__restrict__ act) {
+  const int32_t b = (int32_t)(blockIdx.x * 4u + (threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  if (b >= nblocks || !act[b]) return;
+  int bc[3], loc[3], I[3];
+  block_coords<DIM>(b, P.nb, bc);
+  lane_coords<DIM>(lane, loc);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) I[a] = bc[a] * BlockShape<DIM>::E + loc[a];
+  const int64_t node = (int64_t)b * 64 + lane;
+  grid_v[node] = grid_op_node<DIM, T>(P, I, grid_v[node]);
+}
+
+// Work items from the block counts and offsets of a sort (item_off from the scan).
 __global__ void __launch_bounds__(256) k_items_fill(const uint32_t *__restrict__ counts,
                                                     const uint32_t *__restrict__ offsets,
                                                     const int32_t *__restrict__ item_off, int4 *__restrict__ items,
@@ -773,73 +852,160 @@ __global__ void __launch_bounds__(256) k_items_fill(const uint32_t *__restrict__
   }
 }
 
-template <class T>
-__global__ void __launch_bounds__(256) k_grid_clear(int32_t nblocks, vec4_t<T> *__restrict__ grid,
-                                                    uint32_t *__restrict__ flags, uint32_t *__restrict__ act) {
-  const int32_t b = (int32_t)(blockIdx.x * 4u + (threadIdx.x >> 6));
-  const int lane = threadIdx.x & 63;
-  if (b >= nblocks) return;
-  const uint32_t was = act[b];
-  if (was) {
-    vec4_t<T> z;
-    z.x = z.y = z.z = z.w = (T)0;
-    grid[(int64_t)b * 64 + lane] = z;
-  }
-  __builtin_amdgcn_wave_barrier();
-  if (lane == 0) {
-    flags[b] = 0u;
-    act[b] = 0u;
-  }
-}
-
 __global__ void k_fill_u32(uint32_t *a, uint32_t v, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
 }
 
 // ---- sort by grid block of the stencil base --------------------------------
+// Keys and ranks: rank = position among the particles of the same block
+// (counting-sort scatter).  Lanes of a wavefront with equal keys take
+// consecutive ranks with one atomic (particles are mostly sorted already, so
+// a wavefront holds few distinct keys); after four distinct keys the rest
+// take one atomic each.
 template <int DIM, int MAT, int ST, int NW32, class T>
-__global__ void __launch_bounds__(256) k_sort_keys(StateK s, LayoutK lay, ParamsT<T> P, int64_t n,
+__global__ void __launch_bounds__(256) k_sort_keys(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
                                                    uint32_t *__restrict__ key, uint32_t *__restrict__ rank,
                                                    uint32_t *__restrict__ counts) {
   __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  constexpr int NC = Comps<DIM, MAT>::NC;
-  T v[NC];
-  load_state<ST, NW32, NC, T>(s, lay, p, s.g2p_mask, rows + threadIdx.x * RowsOf<ST, NW32>::RS, v);
-  int base[3] = {0, 0, 0};
-  T fx, w[3];
+  const int lane = threadIdx.x & 63;
+  const bool valid = p < n;
+  uint32_t k = 0u;
+  if (valid) {
+    T x[DIM];
+    if constexpr (ST == AQ_STORE_QUANT) {
+      uint32_t *row = rows + threadIdx.x * RowsOf<ST, NW32>::RS;
+      const uint64_t *w = (const uint64_t *)s.ptr;
 #pragma unroll
-  for (int a = 0; a < DIM; ++a) base[a] = stencil_1d<T>(v[a], P, fx, w);
-  const uint32_t k = (uint32_t)block_of_base<DIM>(base, P.nb);
-  key[p] = k;
-  rank[p] = atomicAdd(counts + k, 1u);
+      for (int i = 0; i < NW32 / 2; ++i) {
+        if (i < s.nw && ((s.g2p_mask >> i) & 1u)) {
+          const uint64_t z = w[(int64_t)i * s.cap + p];
+          row[2 * i] = (uint32_t)z;
+          row[2 * i + 1] = (uint32_t)(z >> 32);
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) x[a] = (T)decode_row(row, comp_at(cod, a));
+    } else {
+      const T *vals = (const T *)s.ptr;
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) x[a] = vals[(int64_t)a * s.cap + p];
+    }
+    int base[3] = {0, 0, 0};
+    T fx, w[3];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) base[a] = stencil_1d<T>(x[a], P, fx, w);
+    k = (uint32_t)block_of_base<DIM>(base, P.nb);
+    key[p] = k;
+  }
+  uint64_t pending = __ballot(valid);
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  uint32_t r = 0u;
+  for (int round = 0; round < 4 && pending; ++round) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t k0 = __shfl(k, leader);
+    const uint64_t same = __ballot(k == k0) & pending;
+    uint32_t base = 0u;
+    if (lane == leader) base = atomicAdd(counts + k0, (uint32_t)__popcll(same));
+    base = __shfl(base, leader);
+    if ((same >> lane) & 1ull) r = base + (uint32_t)__popcll(same & below);
+    pending &= ~same;
+  }
+  if ((pending >> lane) & 1ull) r = atomicAdd(counts + k, 1u);
+  if (valid) rank[p] = r;
 }
 
-// Exclusive scan of counts[0..m) into offsets[0..m] by one workgroup of 1024.
-__global__ void __launch_bounds__(1024) k_scan(const uint32_t *__restrict__ counts, uint32_t *__restrict__ offsets,
-                                               int32_t m) {
-  __shared__ uint32_t part[1024];
+// Exclusive scans of counts[0..m) into off[0..m] and of the work items per
+// block, ceil(counts / 256), into ioff[0..m] (totals at [m]): per-chunk scans
+// of 4096 (k_scan_local), a scan of the chunk totals (k_scan_top) and the
+// chunk prefix added back (k_scan_add).
+__global__ void __launch_bounds__(256) k_scan_local(const uint32_t *__restrict__ counts, uint32_t *__restrict__ off,
+                                                    int32_t *__restrict__ ioff, uint32_t *__restrict__ part,
+                                                    int32_t m) {
+  __shared__ uint32_t sa[256], sb[256];
   const int t = threadIdx.x;
-  const int per = (m + 1023) / 1024;
-  const int lo = t * per, hi = lo + per < m ? lo + per : m;
-  uint32_t sum = 0;
-  for (int i = lo; i < hi; ++i) sum += counts[i];
-  part[t] = sum;
+  const int64_t base = (int64_t)blockIdx.x * 4096 + t * 16;
+  uint32_t c[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) c[i] = base + i < m ? counts[base + i] : 0u;
+  uint32_t a = 0u, b = 0u;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    a += c[i];
+    b += (c[i] + 255u) >> 8;
+  }
+  sa[t] = a;
+  sb[t] = b;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const uint32_t x = t >= off ? part[t - off] : 0u;
+  for (int o = 1; o < 256; o <<= 1) {
+    const uint32_t x = t >= o ? sa[t - o] : 0u, y = t >= o ? sb[t - o] : 0u;
     __syncthreads();
-    part[t] += x;
+    sa[t] += x;
+    sb[t] += y;
     __syncthreads();
   }
-  uint32_t run = part[t] - sum;
+  uint32_t ra = sa[t] - a, rb = sb[t] - b;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    if (base + i < m) {
+      off[base + i] = ra;
+      ioff[base + i] = (int32_t)rb;
+    }
+    ra += c[i];
+    rb += (c[i] + 255u) >> 8;
+  }
+  if (t == 255) {
+    part[2 * blockIdx.x] = sa[255];
+    part[2 * blockIdx.x + 1] = sb[255];
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_scan_top(uint32_t *__restrict__ part, int32_t g, uint32_t *__restrict__ off,
+                                                   int32_t *__restrict__ ioff, int32_t m) {
+  __shared__ uint32_t sa[1024], sb[1024];
+  const int t = threadIdx.x;
+  const int per = (g + 1023) / 1024;
+  const int lo = t * per, hi = lo + per < g ? lo + per : g;
+  uint32_t a = 0u, b = 0u;
   for (int i = lo; i < hi; ++i) {
-    offsets[i] = run;
-    run += counts[i];
+    a += part[2 * i];
+    b += part[2 * i + 1];
   }
-  if (t == 1023) offsets[m] = part[1023];
+  sa[t] = a;
+  sb[t] = b;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const uint32_t x = t >= o ? sa[t - o] : 0u, y = t >= o ? sb[t - o] : 0u;
+    __syncthreads();
+    sa[t] += x;
+    sb[t] += y;
+    __syncthreads();
+  }
+  uint32_t ra = sa[t] - a, rb = sb[t] - b;
+  for (int i = lo; i < hi; ++i) {
+    const uint32_t pa = part[2 * i], pb = part[2 * i + 1];
+    part[2 * i] = ra;
+    part[2 * i + 1] = rb;
+    ra += pa;
+    rb += pb;
+  }
+  if (t == 1023) {
+    off[m] = sa[1023];
+    ioff[m] = (int32_t)sb[1023];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_scan_add(uint32_t *__restrict__ off, int32_t *__restrict__ ioff,
+                                                  const uint32_t *__restrict__ part, int32_t m) {
+  const uint32_t pa = part[2 * blockIdx.x], pb = part[2 * blockIdx.x + 1];
+  const int64_t base = (int64_t)blockIdx.x * 4096;
+  for (int i = threadIdx.x; i < 4096; i += 256) {
+    if (base + i < m) {
+      off[base + i] += pa;
+      ioff[base + i] += (int32_t)pb;
+    }
+  }
 }
 
 template <class E>
@@ -866,7 +1032,7 @@ __device__ __forceinline__ void atomic_min_f32(float *a, float v) {
 }
 
 template <int DIM, int MAT, int ST, int NW32, class T>
-__global__ void __launch_bounds__(256) k_limits(StateK s, LayoutK lay, int64_t n, float *__restrict__ mxmn) {
+__global__ void __launch_bounds__(256) k_limits(StateK s, CodecK cod, int64_t n, float *__restrict__ mxmn) {
   constexpr int NC = Comps<DIM, MAT>::NC;
   __shared__ float smx[NC][8], smn[NC][8];
   __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
@@ -874,7 +1040,7 @@ __global__ void __launch_bounds__(256) k_limits(StateK s, LayoutK lay, int64_t n
   T v[NC];
   float mx[NC], mn[NC];
   if (p < n) {
-    load_state<ST, NW32, NC, T>(s, lay, p, 0xFFFFFFFFu, rows + threadIdx.x * RowsOf<ST, NW32>::RS, v);
+    load_state<DIM, ST, NW32, NC, T>(s, cod, p, 0xFFFFFFFFu, rows + threadIdx.x * RowsOf<ST, NW32>::RS, v);
 #pragma unroll
     for (int c = 0; c < NC; ++c) mx[c] = mn[c] = (float)v[c];
   } else {
@@ -913,7 +1079,7 @@ __global__ void __launch_bounds__(256) k_limits(StateK s, LayoutK lay, int64_t n
 }
 
 template <int DIM, int MAT, int ST, int NW32, class T>
-__global__ void __launch_bounds__(256) k_energy(StateK s, LayoutK lay, ParamsT<T> P, int64_t n, double *__restrict__ out) {
+__global__ void __launch_bounds__(256) k_energy(StateK s, CodecK cod, ParamsT<T> P, int64_t n, double *__restrict__ out) {
   constexpr int NC = Comps<DIM, MAT>::NC;
   using CI = Comps<DIM, MAT>;
   __shared__ double part[8];
@@ -922,7 +1088,7 @@ __global__ void __launch_bounds__(256) k_energy(StateK s, LayoutK lay, ParamsT<T
   double e = 0.0;
   if (p < n) {
     T v[NC];
-    load_state<ST, NW32, NC, T>(s, lay, p, 0xFFFFFFFFu, rows + threadIdx.x * RowsOf<ST, NW32>::RS, v);
+    load_state<DIM, ST, NW32, NC, T>(s, cod, p, 0xFFFFFFFFu, rows + threadIdx.x * RowsOf<ST, NW32>::RS, v);
     T v2 = (T)0;
 #pragma unroll
     for (int a = 0; a < DIM; ++a) v2 = v2 + v[CI::CV + a] * v[CI::CV + a];
@@ -946,19 +1112,23 @@ __global__ void __launch_bounds__(256) k_energy(StateK s, LayoutK lay, ParamsT<T
 // ---------------------------------------------------------------------------
 // simulation handle
 // ---------------------------------------------------------------------------
+// Grids: grid_v holds the gathered P2G result ((m v, m) after aq_p2g,
+// velocities after the grid update) on the blocks flagged in `act`, zeros
+// elsewhere; grid_f is the float-atomic accumulation grid of untiled
+// scatters (zero outside the blocks flagged in fflags); `tiles` holds one
+// P2G tile per work item.
 struct aq_sim {
   aq_sim_desc d;
   LayoutK lay;
+  CompK *lay_dev;  // lay.c in the workspace, read by the MPM kernels
   ParamsT<float> pf;
   ParamsT<double> pd;
   int64_t nodes;
   int32_t nblocks;
   int32_t node_bytes;
-  void *grid[2];
-  uint32_t *flags[2], *act[2];
-  bool dirty[2];
-  int cur;  // grid holding the last velocities
-  uint32_t *skey, *srank, *scount, *soff;
+  void *grid_v, *grid_f, *tiles;
+  uint32_t *fflags, *act;
+  uint32_t *skey, *srank, *scount, *soff, *spart;
   void *tmp_state;
   int32_t *tmp_ids;
   int4 *items;
@@ -992,36 +1162,60 @@ static int check_desc(const aq_sim_desc *d) {
   return AQ_OK;
 }
 
-static int32_t max_items_of(const aq_sim_desc *d, int64_t nblocks) {
-  return (int32_t)((d->capacity + 255) / 256 + nblocks);
+static int64_t blocks_of(const aq_sim_desc *d) {
+  const int64_t n = d->n_grid;
+  return (d->dim == 3 ? n * n * n : n * n) / 64;
 }
 
-static void ws_plan(const aq_sim_desc *d, int64_t *offs /*[11]*/, int64_t *total) {
-  const int64_t n = d->n_grid;
-  const int64_t nodes = d->dim == 3 ? n * n * n : n * n;
-  const int64_t nblocks = nodes / 64;
+// Work items of a sort: at most ceil(count_b / 256) per block, so at most
+// capacity / 256 plus one per non-empty block.
+static int32_t max_items_of(const aq_sim_desc *d) {
+  const int64_t nb = blocks_of(d);
+  const int64_t ne = d->capacity < nb ? d->capacity : nb;
+  return (int32_t)(d->capacity / 256 + ne + 1);
+}
+
+static int64_t tile_slots(const aq_sim_desc *d) { return d->dim == 3 ? 512 : 256; }
+
+enum {
+  WS_GRID_V, WS_GRID_F, WS_FLAGS, WS_KEY, WS_RANK, WS_COUNT, WS_OFF, WS_PART, WS_TMP_STATE, WS_TMP_IDS,
+  WS_ITEMS, WS_ITEM_OFF, WS_TILES, WS_LAYOUT, WS_N
+};
+
+static void ws_plan(const aq_sim_desc *d, int64_t *offs /*[WS_N]*/, int64_t *total) {
+  const int64_t nblocks = blocks_of(d);
+  const int64_t nodes = nblocks * 64;
   const int64_t node_bytes = d->storage == AQ_STORE_F64 ? 32 : 16;
   const int64_t cap = d->capacity;
   const int64_t state_bytes = d->storage == AQ_STORE_QUANT ? cap * 8 * d->nwords
                               : cap * d->ncomp * (d->storage == AQ_STORE_F64 ? 8 : 4);
+  const int64_t chunks = (nblocks + 1 + 4095) / 4096;
+  int64_t sz[WS_N];
+  sz[WS_GRID_V] = nodes * node_bytes;
+  sz[WS_GRID_F] = nodes * node_bytes;
+  sz[WS_FLAGS] = nblocks * 4 * 2;  // fflags, act
+  sz[WS_KEY] = cap * 4;
+  sz[WS_RANK] = cap * 4;
+  sz[WS_COUNT] = (nblocks + 1) * 4;
+  sz[WS_OFF] = (nblocks + 1) * 4;
+  sz[WS_PART] = chunks * 8;
+  sz[WS_TMP_STATE] = state_bytes;
+  sz[WS_TMP_IDS] = cap * 4;
+  sz[WS_ITEMS] = (int64_t)max_items_of(d) * 16;
+  sz[WS_ITEM_OFF] = (nblocks + 1) * 4;
+  sz[WS_TILES] = (int64_t)max_items_of(d) * tile_slots(d) * node_bytes;
+  sz[WS_LAYOUT] = AQ_MAX_COMPS * (int64_t)sizeof(CompK);
   int64_t o = 0;
-  offs[0] = o; o = align256(o + nodes * node_bytes);          // grid 0
-  offs[1] = o; o = align256(o + nodes * node_bytes);          // grid 1
-  offs[2] = o; o = align256(o + nblocks * 4 * 4);             // flags0, flags1, act0, act1
-  offs[3] = o; o = align256(o + cap * 4);                     // sort keys
-  offs[4] = o; o = align256(o + cap * 4);                     // sort ranks
-  offs[5] = o; o = align256(o + (nblocks + 1) * 4);           // counts
-  offs[6] = o; o = align256(o + (nblocks + 1) * 4);           // offsets
-  offs[7] = o; o = align256(o + state_bytes);                 // sort temp state
-  offs[8] = o; o = align256(o + cap * 4);                     // sort temp ids
-  offs[9] = o; o = align256(o + (int64_t)max_items_of(d, nblocks) * 16);  // work items
-  offs[10] = o; o = align256(o + (nblocks + 1) * 4);          // item offsets
+  for (int i = 0; i < WS_N; ++i) {
+    offs[i] = o;
+    o = align256(o + sz[i]);
+  }
   *total = o;
 }
 
 extern "C" int64_t aq_sim_workspace_bytes(const aq_sim_desc *d) {
   if (check_desc(d)) return -1;
-  int64_t offs[11], total;
+  int64_t offs[WS_N], total;
   ws_plan(d, offs, &total);
   return total;
 }
@@ -1053,7 +1247,7 @@ extern "C" int aq_sim_create(const aq_sim_desc *d, void *ws, int64_t ws_bytes, a
   int rc = check_desc(d);
   if (rc) return rc;
   if (!out) return fail(AQ_ERR_INVALID, "null output");
-  int64_t offs[11], total;
+  int64_t offs[WS_N], total;
   ws_plan(d, offs, &total);
   if (!ws || ws_bytes < total) return fail(AQ_ERR_CAPACITY, "workspace too small");
   aq_sim *s = new aq_sim();
@@ -1071,39 +1265,37 @@ extern "C" int aq_sim_create(const aq_sim_desc *d, void *ws, int64_t ws_bytes, a
   }
   s->pf = make_params<float>(d);
   s->pd = make_params<double>(d);
-  const int64_t n = d->n_grid;
-  s->nodes = d->dim == 3 ? n * n * n : n * n;
-  s->nblocks = (int32_t)(s->nodes / 64);
+  s->nblocks = (int32_t)blocks_of(d);
+  s->nodes = (int64_t)s->nblocks * 64;
   s->node_bytes = d->storage == AQ_STORE_F64 ? 32 : 16;
   char *base = (char *)ws;
-  s->grid[0] = base + offs[0];
-  s->grid[1] = base + offs[1];
-  uint32_t *fl = (uint32_t *)(base + offs[2]);
-  s->flags[0] = fl;
-  s->flags[1] = fl + s->nblocks;
-  s->act[0] = fl + 2 * s->nblocks;
-  s->act[1] = fl + 3 * s->nblocks;
-  s->skey = (uint32_t *)(base + offs[3]);
-  s->srank = (uint32_t *)(base + offs[4]);
-  s->scount = (uint32_t *)(base + offs[5]);
-  s->soff = (uint32_t *)(base + offs[6]);
-  s->tmp_state = base + offs[7];
-  s->tmp_ids = (int32_t *)(base + offs[8]);
-  s->items = (int4 *)(base + offs[9]);
-  s->item_off = (int32_t *)(base + offs[10]);
-  s->max_items = max_items_of(d, s->nblocks);
+  s->grid_v = base + offs[WS_GRID_V];
+  s->grid_f = base + offs[WS_GRID_F];
+  s->fflags = (uint32_t *)(base + offs[WS_FLAGS]);
+  s->act = s->fflags + s->nblocks;
+  s->skey = (uint32_t *)(base + offs[WS_KEY]);
+  s->srank = (uint32_t *)(base + offs[WS_RANK]);
+  s->scount = (uint32_t *)(base + offs[WS_COUNT]);
+  s->soff = (uint32_t *)(base + offs[WS_OFF]);
+  s->spart = (uint32_t *)(base + offs[WS_PART]);
+  s->tmp_state = base + offs[WS_TMP_STATE];
+  s->tmp_ids = (int32_t *)(base + offs[WS_TMP_IDS]);
+  s->items = (int4 *)(base + offs[WS_ITEMS]);
+  s->item_off = (int32_t *)(base + offs[WS_ITEM_OFF]);
+  s->tiles = base + offs[WS_TILES];
+  s->lay_dev = (CompK *)(base + offs[WS_LAYOUT]);
+  s->max_items = max_items_of(d);
   s->have_items = false;
   s->items_state = nullptr;
   s->items_n = -1;
   s->ws_bytes = ws_bytes;
-  s->dirty[0] = s->dirty[1] = false;
-  s->cur = 0;
-  // grids, flags and activity start at zero
-  hipError_t e = hipMemset(ws, 0, (size_t)offs[3]);
+  // grids and block flags start at zero; the layout table goes to the device
+  hipError_t e = hipMemset(ws, 0, (size_t)offs[WS_KEY]);
+  if (e == hipSuccess) e = hipMemcpy(s->lay_dev, s->lay.c, sizeof(s->lay.c), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     delete s;
-    return fail(AQ_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    return fail(AQ_ERR_HIP, std::string("workspace set-up: ") + hipGetErrorString(e));
   }
   *out = s;
   return AQ_OK;
@@ -1140,6 +1332,26 @@ static StateK make_state(const aq_sim *s, const void *state, const int32_t *ids)
   return k;
 }
 
+static CodecK make_codec(const aq_sim *s, uint32_t step) {
+  CodecK c;
+  c.c = s->lay_dev;
+  c.nc = s->lay.nc;
+  c.nw = s->lay.nw;
+  c.shared_mask = c.gfirst_mask = c.glast_mask = 0u;
+  if (s->d.storage == AQ_STORE_QUANT) {
+    for (int i = 0; i < s->lay.nc; ++i) {
+      const CompK &k = s->lay.c[i];
+      if (k.kind == K_FLOAT && k.shared) {
+        c.shared_mask |= 1u << i;
+        if (k.gfirst) c.gfirst_mask |= 1u << i;
+        if (k.glast) c.glast_mask |= 1u << i;
+      }
+    }
+  }
+  c.key_base = key_base(s->d.seed, step);
+  return c;
+}
+
 // Dispatch over (dim, material, storage, record bucket): calls f(Tag<...>{}).
 template <int D_, int M_, int S_, int W_, class T_> struct Tag {
   static constexpr int DIM = D_, MAT = M_, ST = S_, NW32 = W_;
@@ -1170,6 +1382,18 @@ static void dispatch(const aq_sim *s, F &&f) {
     if (d.dim == 3) f(Tag<3, AQ_FLUID, AQ_STORE_F64, 4, double>{});
     else if (d.material == AQ_FLUID) f(Tag<2, AQ_FLUID, AQ_STORE_F64, 4, double>{});
     else f(Tag<2, AQ_ELASTIC, AQ_STORE_F64, 4, double>{});
+  }
+}
+
+// Dispatch over (dim, compute type) for the grid kernels.
+template <class F>
+static void dispatch_grid(const aq_sim *s, F &&f) {
+  if (s->d.storage == AQ_STORE_F64) {
+    if (s->d.dim == 3) f(Tag<3, 0, 0, 0, double>{});
+    else f(Tag<2, 0, 0, 0, double>{});
+  } else {
+    if (s->d.dim == 3) f(Tag<3, 0, 0, 0, float>{});
+    else f(Tag<2, 0, 0, 0, float>{});
   }
 }
 
@@ -1205,96 +1429,83 @@ static bool items_valid(const aq_sim *s, const void *state, int64_t n) {
   return s->have_items && s->items_state == state && s->items_n == n;
 }
 
-static int clear_grid(aq_sim *s, int g, hipStream_t st) {
-  if (!s->dirty[g]) return AQ_OK;
-  if (s->d.storage == AQ_STORE_F64)
-    hipLaunchKernelGGL(k_grid_clear<double>, dim3(gblocks(s)), dim3(256), 0, st, s->nblocks,
-                       (double4 *)s->grid[g], s->flags[g], s->act[g]);
-  else
-    hipLaunchKernelGGL(k_grid_clear<float>, dim3(gblocks(s)), dim3(256), 0, st, s->nblocks,
-                       (float4 *)s->grid[g], s->flags[g], s->act[g]);
-  LAUNCH_CHECK();
-  s->dirty[g] = false;
-  return AQ_OK;
-}
-
-// P2G into grid g (tiled when the last sort describes this state).
-static int launch_p2g(aq_sim *s, const void *state, int64_t n, int g, hipStream_t st) {
-  if (n == 0) return AQ_OK;
-  StateK sk = make_state(s, state, nullptr);
-  const bool tiled = items_valid(s, state, n);
-  dispatch(s, [&](auto tag_) {
+// Gather of the P2G result into grid_v (OP: with the grid update).
+static int launch_gather(aq_sim *s, bool tiled, bool op, hipStream_t st) {
+  const int32_t *ioff = tiled ? s->item_off : nullptr;
+  dispatch_grid(s, [&](auto tag_) {
     TAG_TYPES(tag_)
-    if (tiled)
-      hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, false, true>), dim3(tblocks(s, n)), dim3(256), 0, st, sk,
-                         s->lay, KeysK{}, params_of<T>(s), n, s->items, s->item_off + s->nblocks,
-                         (const vec4_t<T> *)nullptr, (vec4_t<T> *)s->grid[g], s->flags[g]);
+    (void)MAT;
+    (void)ST;
+    (void)NW32;
+    if (op)
+      hipLaunchKernelGGL((k_gather<DIM, T, true>), dim3(gblocks(s)), dim3(256), 0, st, params_of<T>(s), s->nblocks,
+                         ioff, (const vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags,
+                         (vec4_t<T> *)s->grid_v, s->act);
     else
-      hipLaunchKernelGGL((k_p2g<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, s->lay,
-                         params_of<T>(s), n, (vec4_t<T> *)s->grid[g], s->flags[g]);
+      hipLaunchKernelGGL((k_gather<DIM, T, false>), dim3(gblocks(s)), dim3(256), 0, st, params_of<T>(s), s->nblocks,
+                         ioff, (const vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags,
+                         (vec4_t<T> *)s->grid_v, s->act);
   });
   LAUNCH_CHECK();
   return AQ_OK;
 }
 
-static int launch_grid_update(aq_sim *s, int g, int other, hipStream_t st) {
-  void *og = other >= 0 ? s->grid[other] : nullptr;
-  uint32_t *of = other >= 0 ? s->flags[other] : nullptr;
-  uint32_t *oa = other >= 0 ? s->act[other] : nullptr;
-  if (s->d.storage == AQ_STORE_F64) {
-    if (s->d.dim == 3)
-      hipLaunchKernelGGL((k_grid_update<3, double>), dim3(gblocks(s)), dim3(256), 0, st, s->pd, s->nblocks,
-                         (double4 *)s->grid[g], s->flags[g], s->act[g], (double4 *)og, of, oa);
+// P2G of `state` (tiled when the last sort describes it); *tiled tells which.
+static int launch_p2g(aq_sim *s, const void *state, int64_t n, bool *tiled, hipStream_t st) {
+  *tiled = n > 0 && items_valid(s, state, n);
+  if (n == 0) return AQ_OK;
+  StateK sk = make_state(s, state, nullptr);
+  CodecK cod = make_codec(s, 0u);
+  dispatch(s, [&](auto tag_) {
+    TAG_TYPES(tag_)
+    if (*tiled)
+      hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, false, true>), dim3(tblocks(s, n)), dim3(256), 0, st, sk,
+                         cod, params_of<T>(s), n, s->items, s->item_off + s->nblocks, (const vec4_t<T> *)nullptr,
+                         (vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags);
     else
-      hipLaunchKernelGGL((k_grid_update<2, double>), dim3(gblocks(s)), dim3(256), 0, st, s->pd, s->nblocks,
-                         (double4 *)s->grid[g], s->flags[g], s->act[g], (double4 *)og, of, oa);
-  } else {
-    if (s->d.dim == 3)
-      hipLaunchKernelGGL((k_grid_update<3, float>), dim3(gblocks(s)), dim3(256), 0, st, s->pf, s->nblocks,
-                         (float4 *)s->grid[g], s->flags[g], s->act[g], (float4 *)og, of, oa);
-    else
-      hipLaunchKernelGGL((k_grid_update<2, float>), dim3(gblocks(s)), dim3(256), 0, st, s->pf, s->nblocks,
-                         (float4 *)s->grid[g], s->flags[g], s->act[g], (float4 *)og, of, oa);
-  }
+      hipLaunchKernelGGL((k_p2g<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, cod,
+                         params_of<T>(s), n, (vec4_t<T> *)s->grid_f, s->fflags);
+  });
   LAUNCH_CHECK();
-  if (other >= 0) s->dirty[other] = false;
   return AQ_OK;
 }
 
-static int launch_g2p(aq_sim *s, void *state, int64_t n, const int32_t *ids, int g, uint32_t step,
-                      hipStream_t st) {
+static int launch_g2p(aq_sim *s, void *state, int64_t n, const int32_t *ids, uint32_t step, hipStream_t st) {
   if (n == 0) return AQ_OK;
   StateK sk = make_state(s, state, ids);
-  KeysK keys = make_keys(s->d.seed, step);
+  CodecK cod = make_codec(s, step);
   const bool tiled = items_valid(s, state, n);
   dispatch(s, [&](auto tag_) {
     TAG_TYPES(tag_)
     if (tiled)
       hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, true, false>), dim3(tblocks(s, n)), dim3(256), 0, st, sk,
-                         s->lay, keys, params_of<T>(s), n, s->items, s->item_off + s->nblocks,
-                         (const vec4_t<T> *)s->grid[g], (vec4_t<T> *)nullptr, (uint32_t *)nullptr);
+                         cod, params_of<T>(s), n, s->items, s->item_off + s->nblocks,
+                         (const vec4_t<T> *)s->grid_v, (vec4_t<T> *)nullptr, (vec4_t<T> *)nullptr,
+                         (uint32_t *)nullptr);
     else
-      hipLaunchKernelGGL((k_g2p<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, s->lay, keys,
-                         params_of<T>(s), n, (const vec4_t<T> *)s->grid[g]);
+      hipLaunchKernelGGL((k_g2p<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, cod,
+                         params_of<T>(s), n, (const vec4_t<T> *)s->grid_v);
   });
   LAUNCH_CHECK();
   return AQ_OK;
 }
 
-static int launch_g2p2g(aq_sim *s, void *state, int64_t n, const int32_t *ids, int gv, int gs, uint32_t step,
+// G2P of substep `step` fused with the next P2G; *tiled tells which P2G ran.
+static int launch_g2p2g(aq_sim *s, void *state, int64_t n, const int32_t *ids, uint32_t step, bool *tiled,
                         hipStream_t st) {
+  *tiled = n > 0 && items_valid(s, state, n);
+  if (n == 0) return AQ_OK;
   StateK sk = make_state(s, state, ids);
-  KeysK keys = make_keys(s->d.seed, step);
-  const bool tiled = items_valid(s, state, n);
+  CodecK cod = make_codec(s, step);
   dispatch(s, [&](auto tag_) {
     TAG_TYPES(tag_)
-    if (tiled)
+    if (*tiled)
       hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, true, true>), dim3(tblocks(s, n)), dim3(256), 0, st, sk,
-                         s->lay, keys, params_of<T>(s), n, s->items, s->item_off + s->nblocks,
-                         (const vec4_t<T> *)s->grid[gv], (vec4_t<T> *)s->grid[gs], s->flags[gs]);
+                         cod, params_of<T>(s), n, s->items, s->item_off + s->nblocks,
+                         (const vec4_t<T> *)s->grid_v, (vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags);
     else
-      hipLaunchKernelGGL((k_g2p2g<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, s->lay, keys,
-                         params_of<T>(s), n, (const vec4_t<T> *)s->grid[gv], (vec4_t<T> *)s->grid[gs], s->flags[gs]);
+      hipLaunchKernelGGL((k_g2p2g<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, cod,
+                         params_of<T>(s), n, (const vec4_t<T> *)s->grid_v, (vec4_t<T> *)s->grid_f, s->fflags);
   });
   LAUNCH_CHECK();
   return AQ_OK;
@@ -1304,22 +1515,30 @@ extern "C" int aq_p2g(aq_sim *s, const void *state, int64_t n, void *stream) {
   int rc = check_n(s, state, n);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
-  if ((rc = clear_grid(s, 0, st))) return rc;
-  if ((rc = launch_p2g(s, state, n, 0, st))) return rc;
-  s->dirty[0] = true;
-  s->cur = 0;
-  return AQ_OK;
+  bool tiled;
+  if ((rc = launch_p2g(s, state, n, &tiled, st))) return rc;
+  return launch_gather(s, tiled, false, st);
 }
 
 extern "C" int aq_grid_op(aq_sim *s, void *stream) {
   if (!s) return fail(AQ_ERR_INVALID, "null handle");
-  return launch_grid_update(s, s->cur, -1, (hipStream_t)stream);
+  hipStream_t st = (hipStream_t)stream;
+  dispatch_grid(s, [&](auto tag_) {
+    TAG_TYPES(tag_)
+    (void)MAT;
+    (void)ST;
+    (void)NW32;
+    hipLaunchKernelGGL((k_grid_update<DIM, T>), dim3(gblocks(s)), dim3(256), 0, st, params_of<T>(s), s->nblocks,
+                       (vec4_t<T> *)s->grid_v, (const uint32_t *)s->act);
+  });
+  LAUNCH_CHECK();
+  return AQ_OK;
 }
 
 extern "C" int aq_g2p(aq_sim *s, void *state, int64_t n, const int32_t *ids, uint32_t step, void *stream) {
   int rc = check_n(s, state, n);
   if (rc) return rc;
-  return launch_g2p(s, state, n, ids, s->cur, step, (hipStream_t)stream);
+  return launch_g2p(s, state, n, ids, step, (hipStream_t)stream);
 }
 
 extern "C" int aq_sort(aq_sim *s, void *state, int64_t n, int32_t *ids, void *stream) {
@@ -1328,16 +1547,20 @@ extern "C" int aq_sort(aq_sim *s, void *state, int64_t n, int32_t *ids, void *st
   if (n == 0) return AQ_OK;
   hipStream_t st = (hipStream_t)stream;
   StateK sk = make_state(s, state, nullptr);
-  hipLaunchKernelGGL(k_fill_u32, dim3((s->nblocks + 1 + 255) / 256), dim3(256), 0, st, s->scount, 0u,
-                     (int64_t)s->nblocks + 1);
+  CodecK cod = make_codec(s, 0u);
+  const int32_t m = s->nblocks;
+  hipLaunchKernelGGL(k_fill_u32, dim3((m + 1 + 255) / 256), dim3(256), 0, st, s->scount, 0u, (int64_t)m + 1);
   LAUNCH_CHECK();
   dispatch(s, [&](auto tag_) {
     TAG_TYPES(tag_)
-    hipLaunchKernelGGL((k_sort_keys<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, s->lay,
+    hipLaunchKernelGGL((k_sort_keys<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, cod,
                        params_of<T>(s), n, s->skey, s->srank, s->scount);
   });
   LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, s->scount, s->soff, s->nblocks);
+  const int32_t chunks = (m + 4095) / 4096;
+  hipLaunchKernelGGL(k_scan_local, dim3(chunks), dim3(256), 0, st, s->scount, s->soff, s->item_off, s->spart, m);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, s->spart, chunks, s->soff, s->item_off, m);
+  hipLaunchKernelGGL(k_scan_add, dim3(chunks), dim3(256), 0, st, s->soff, s->item_off, s->spart, m);
   LAUNCH_CHECK();
   const int64_t cap = s->d.capacity;
   size_t esz;
@@ -1363,9 +1586,8 @@ extern "C" int aq_sort(aq_sim *s, void *state, int64_t n, int32_t *ids, void *st
                            (size_t)planes, hipMemcpyDeviceToDevice, st));
   if (ids) HIP_TRY(hipMemcpyAsync(ids, s->tmp_ids, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
   // work items: up to 256 particles of one block each
-  hipLaunchKernelGGL(k_items_scan, dim3(1), dim3(1024), 0, st, s->scount, s->item_off, s->nblocks);
-  hipLaunchKernelGGL(k_items_fill, dim3((s->nblocks + 255) / 256), dim3(256), 0, st, s->scount, s->soff,
-                     s->item_off, s->items, s->nblocks);
+  hipLaunchKernelGGL(k_items_fill, dim3((m + 255) / 256), dim3(256), 0, st, s->scount, s->soff, s->item_off,
+                     s->items, m);
   LAUNCH_CHECK();
   s->have_items = true;
   s->items_state = state;
@@ -1423,40 +1645,36 @@ struct Timer {
     if (TM) (TM)->end();           \
   } while (0)
 
+// nsteps substeps: [sort] P2G, gather+grid update, then per substep the
+// fused G2P->P2G and gather+grid update ([sort] every sort_every substeps),
+// and a last G2P.
 static int advance_impl(aq_sim *s, void *state, int64_t n, int nsteps, uint32_t step0, int sort_every,
                         int32_t *ids, hipStream_t st, Timer *tm) {
   int rc = check_n(s, state, n);
   if (rc) return rc;
   if (nsteps < 0) return fail(AQ_ERR_INVALID, "negative step count");
   if (nsteps == 0 || n == 0) return AQ_OK;
-  int X = 0, Y = 1;
-  if ((rc = clear_grid(s, X, st))) return rc;
-  if ((rc = clear_grid(s, Y, st))) return rc;
   if (sort_every > 0) {
     TIMED(tm, Timer::SORT, rc = aq_sort(s, state, n, ids, st));
     if (rc) return rc;
   }
-  TIMED(tm, Timer::ENDS, rc = launch_p2g(s, state, n, X, st));
+  bool tiled;
+  TIMED(tm, Timer::ENDS, rc = launch_p2g(s, state, n, &tiled, st));
   if (rc) return rc;
-  s->dirty[X] = true;
-  TIMED(tm, Timer::GRID, rc = launch_grid_update(s, X, -1, st));
+  TIMED(tm, Timer::GRID, rc = launch_gather(s, tiled, true, st));
   if (rc) return rc;
   for (int t = 0; t + 1 < nsteps; ++t) {
     if (sort_every > 0 && t > 0 && t % sort_every == 0) {
       TIMED(tm, Timer::SORT, rc = aq_sort(s, state, n, ids, st));
       if (rc) return rc;
     }
-    TIMED(tm, Timer::FUSED, rc = launch_g2p2g(s, state, n, ids, X, Y, step0 + (uint32_t)t, st));
+    TIMED(tm, Timer::FUSED, rc = launch_g2p2g(s, state, n, ids, step0 + (uint32_t)t, &tiled, st));
     if (rc) return rc;
-    s->dirty[Y] = true;
-    TIMED(tm, Timer::GRID, rc = launch_grid_update(s, Y, X, st));
+    TIMED(tm, Timer::GRID, rc = launch_gather(s, tiled, true, st));
     if (rc) return rc;
-    std::swap(X, Y);
   }
-  TIMED(tm, Timer::ENDS, rc = launch_g2p(s, state, n, ids, X, step0 + (uint32_t)(nsteps - 1), st));
-  if (rc) return rc;
-  s->cur = X;
-  return AQ_OK;
+  TIMED(tm, Timer::ENDS, rc = launch_g2p(s, state, n, ids, step0 + (uint32_t)(nsteps - 1), st));
+  return rc;
 }
 
 extern "C" int aq_advance(aq_sim *s, void *state, int64_t n, int nsteps, uint32_t step0, int sort_every,
@@ -1482,11 +1700,10 @@ extern "C" int aq_advance_timed(aq_sim *s, void *state, int64_t n, int nsteps, u
   if (rc) return rc;
   if (!timed) return fail(AQ_ERR_HIP, "event timing failed");
   for (int c = 0; c < Timer::NCAT; ++c) launches[c] = tm.launches[c];
-  // active grid nodes of the last grid (blocks flagged by its P2G)
+  // active grid nodes: nodes of the blocks of the last gather
   unsigned long long *cnt = (unsigned long long *)s->tmp_ids;  // free after the last sort
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (uint32_t *)cnt, 0u, (int64_t)2);
-  hipLaunchKernelGGL(k_count_active, dim3((s->nblocks + 255) / 256), dim3(256), 0, st, s->act[s->cur], s->nblocks,
-                     cnt);
+  hipLaunchKernelGGL(k_count_active, dim3((s->nblocks + 255) / 256), dim3(256), 0, st, s->act, s->nblocks, cnt);
   LAUNCH_CHECK();
   unsigned long long h = 0;
   HIP_TRY(hipMemcpyAsync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -1502,9 +1719,10 @@ extern "C" int aq_update_limits(aq_sim *s, const void *state, int64_t n, float *
   if (n == 0) return AQ_OK;
   hipStream_t st = (hipStream_t)stream;
   StateK sk = make_state(s, state, nullptr);
+  CodecK cod = make_codec(s, 0u);
   dispatch(s, [&](auto tag_) {
     TAG_TYPES(tag_)
-    hipLaunchKernelGGL((k_limits<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, s->lay, n, mxmn);
+    hipLaunchKernelGGL((k_limits<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, cod, n, mxmn);
   });
   LAUNCH_CHECK();
   return AQ_OK;
@@ -1519,9 +1737,10 @@ extern "C" int aq_energy(aq_sim *s, const void *state, int64_t n, double *out, v
   LAUNCH_CHECK();
   if (n == 0) return AQ_OK;
   StateK sk = make_state(s, state, nullptr);
+  CodecK cod = make_codec(s, 0u);
   dispatch(s, [&](auto tag_) {
     TAG_TYPES(tag_)
-    hipLaunchKernelGGL((k_energy<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, s->lay,
+    hipLaunchKernelGGL((k_energy<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, cod,
                        params_of<T>(s), n, out);
   });
   LAUNCH_CHECK();
@@ -1530,7 +1749,7 @@ extern "C" int aq_energy(aq_sim *s, const void *state, int64_t n, double *out, v
 
 extern "C" int aq_sim_grid(aq_sim *s, void **grid, int64_t *nodes) {
   if (!s || !grid || !nodes) return fail(AQ_ERR_INVALID, "null argument");
-  *grid = s->grid[s->cur];
+  *grid = s->grid_v;
   *nodes = s->nodes;
   return AQ_OK;
 }

@@ -7,9 +7,11 @@
 //   P2G scatter           :615-622
 //   grid update           :633-665 (separating walls; sticky walls zero the node)
 //   G2P gather + update   :682-694, :702 (J), :750-758 (F)
-// with the reference's operand order; the file is compiled with
-// -ffp-contract=off, so per-particle results equal the CPU restatement's and
-// only the order of the P2G float atomics differs.
+// The stencil sums are evaluated axis by axis (separable weights, offsets
+// centred on the middle node, fused multiply-adds): the same sums as the
+// reference's node-by-node loops, about a third of the arithmetic, and
+// per-particle results within a few float ulps of the reference's operand
+// order.  The codec (aq_codec.hpp) keeps the reference's unfused rounding.
 //
 // Grid: n^DIM nodes in blocks of 64 (4x4x4 in 3D, 8x8 in 2D), block-major,
 // one 16-byte (float4) or 32-byte (double4) record per node:
@@ -64,11 +66,12 @@ template <int ST, int NW32> struct RowsOf {
   static constexpr int RS = ST == AQ_STORE_QUANT ? NW32 + 1 : 1;
 };
 
-// Load particle p (quantized: the words in `mask` go through the thread's
-// LDS row and every member is decoded from it).
-template <int ST, int NW32, int NC, class T>
-__device__ __forceinline__ void load_state(const StateK &s, const LayoutK &lay, int64_t p,
-                                           uint32_t mask, uint32_t *row, T (&v)[NC]) {
+// Load particle p.  Quantized: the words in `mask` go through the thread's
+// LDS row and the members are decoded from it (with G2P_ONLY, only x and
+// J/F, the members G2P reads).
+template <int DIM, int ST, int NW32, int NC, class T, bool G2P_ONLY = false>
+__device__ __forceinline__ void load_state(const StateK &s, const CodecK &cod, int64_t p, uint32_t mask,
+                                           uint32_t *row, T (&v)[NC]) {
   if constexpr (ST == AQ_STORE_QUANT) {
     const uint64_t *w = (const uint64_t *)s.ptr;
 #pragma unroll
@@ -79,8 +82,15 @@ __device__ __forceinline__ void load_state(const StateK &s, const LayoutK &lay, 
         row[2 * i + 1] = (uint32_t)(x >> 32);
       }
     }
+    constexpr int CJ = 2 * DIM + DIM * DIM;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) v[c] = (T)decode_row(row, lay.c[c]);
+    for (int c = 0; c < NC; ++c) {
+      if (G2P_ONLY && c >= DIM && c < CJ) {
+        v[c] = (T)0;
+        continue;
+      }
+      v[c] = (T)decode_row(row, comp_at(cod, c));
+    }
   } else {
     const T *vals = (const T *)s.ptr;
 #pragma unroll
@@ -88,25 +98,27 @@ __device__ __forceinline__ void load_state(const StateK &s, const LayoutK &lay, 
   }
 }
 
-// Store particle p; quantized records are encoded in the thread's LDS row,
-// which keeps the new record for a decode that follows.
+// Store particle p.  Quantized records are encoded in the thread's LDS row;
+// v[] is replaced by the values the stored record decodes to (what the next
+// P2G must see).  Plain storage stores v[] exactly.
 template <int ST, int NW32, int NC, class T>
-__device__ __forceinline__ void store_state(const StateK &s, const LayoutK &lay,
-                                            const uint32_t *keys, int64_t p, T (&v)[NC],
+__device__ __forceinline__ void store_state(const StateK &s, const CodecK &cod, int64_t p, T (&v)[NC],
                                             uint32_t *row) {
   if constexpr (ST == AQ_STORE_QUANT) {
-    float f[NC];
+    float f[NC], dec[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) f[c] = (float)v[c];
 #pragma unroll
     for (int i = 0; i < NW32; ++i) row[i] = 0u;
-    encode_record<NC>(f, lay, keys, noise_slot(s, p), row);
+    encode_record<NC>(f, cod, noise_slot(s, p), row, dec);
     uint64_t *w = (uint64_t *)s.ptr;
 #pragma unroll
     for (int i = 0; i < NW32 / 2; ++i)
       if (i < s.nw)
         __builtin_nontemporal_store((uint64_t)row[2 * i] | ((uint64_t)row[2 * i + 1] << 32),
                                     w + (int64_t)i * s.cap + p);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = (T)dec[c];
   } else {
     T *vals = (T *)s.ptr;
 #pragma unroll
@@ -212,6 +224,10 @@ __device__ __forceinline__ void affine_of(const T (&v)[NC], const ParamsT<T> &P,
 }
 
 template <class T> __device__ __forceinline__ void atomic_add(T *a, T v) { unsafeAtomicAdd(a, v); }
+template <class T> __device__ __forceinline__ T tfma(T a, T b, T c);
+template <> __device__ __forceinline__ float tfma<float>(float a, float b, float c) { return fmaf(a, b, c); }
+template <> __device__ __forceinline__ double tfma<double>(double a, double b, double c) { return fma(a, b, c); }
+template <class T> __device__ __forceinline__ T tabs(T x) { return x < (T)0 ? -x : x; }
 
 // Per-particle stencil of one particle: base cells, weights, fx.
 template <int DIM, class T> struct Stencil {
@@ -231,124 +247,86 @@ __device__ __forceinline__ Stencil<DIM, T> stencil_of(const T (&v)[NC], const Pa
   return S;
 }
 
-// Per-particle P2G terms (mpm_solver.py:608-613): affine matrix
-// stress * dt-term + p_mass * C, and momentum p_mass * v.
-template <int DIM, class T> struct P2GPrep {
-  T aff[DIM * DIM];
-  T mv[3];
-};
-
-template <int DIM, int MAT, class T, int NC>
-__device__ __forceinline__ P2GPrep<DIM, T> p2g_prepare(const T (&v)[NC], const ParamsT<T> &P) {
-  using CI = Comps<DIM, MAT>;
-  P2GPrep<DIM, T> R;
-  affine_of<DIM, MAT, T>(v, P, R.aff);
-  R.mv[1] = R.mv[2] = (T)0;
-#pragma unroll
-  for (int a = 0; a < DIM; ++a) R.mv[a] = P.p_mass * v[CI::CV + a];
-  return R;
-}
-
-// Scatter of one particle (mpm_solver.py:615-622): calls
-// add(i, j, k, w * (m v + A dpos), w * m) for the 3^DIM stencil nodes
-// (i, j, k offsets from the base), in the reference's operand order.
-template <int DIM, class T, class F>
-__device__ __forceinline__ void p2g_emit(const P2GPrep<DIM, T> &R, const ParamsT<T> &P,
-                                         const Stencil<DIM, T> &S, F &&add) {
-  constexpr int NK = DIM == 3 ? 3 : 1;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j)
-#pragma unroll
-      for (int k = 0; k < NK; ++k) {
-        const int off[3] = {i, j, k};
-        T dpos[3];
-#pragma unroll
-        for (int a = 0; a < DIM; ++a) dpos[a] = ((T)off[a] - S.fx[a]) * P.dx;
-        T weight = (T)1.0;
-#pragma unroll
-        for (int a = 0; a < DIM; ++a) weight *= S.w[a][off[a]];
-        T mom[3] = {(T)0, (T)0, (T)0};
-#pragma unroll
-        for (int a = 0; a < DIM; ++a) {
-          T ad = R.aff[a * DIM + 0] * dpos[0];
-#pragma unroll
-          for (int b = 1; b < DIM; ++b) ad = ad + R.aff[a * DIM + b] * dpos[b];
-          mom[a] = weight * (R.mv[a] + ad);
-        }
-        add(i, j, k, mom, weight * P.p_mass);
-      }
-}
-
-template <int DIM, int MAT, class T, int NC, class F>
-__device__ __forceinline__ void p2g_apply(const T (&v)[NC], const ParamsT<T> &P,
-                                          const Stencil<DIM, T> &S, F &&add) {
-  p2g_emit<DIM, T>(p2g_prepare<DIM, MAT, T>(v, P), P, S, add);
-}
-
-template <class T> __device__ __forceinline__ T tabs(T x) { return x < (T)0 ? -x : x; }
-
-// Upper bound of |contribution| of one particle to any of its stencil nodes,
-// per component (momentum x, y, z; mass): max weight * (|m v_a| +
-// sum_b |A_ab| * max |dpos_b|), with a 2^-10 margin for the rounding of the
-// contributions themselves.  NaN/Inf inputs give a non-finite bound.
-template <int DIM, class T>
-__device__ __forceinline__ void p2g_bounds(const P2GPrep<DIM, T> &R, const ParamsT<T> &P,
-                                           const Stencil<DIM, T> &S, T (&B)[4]) {
-  constexpr T margin = (T)1 + (T)1 / (T)1024;
-  T W = (T)1, D[3];
-#pragma unroll
-  for (int a = 0; a < DIM; ++a) {
-    W *= tmax<T>(tmax<T>(S.w[a][0], S.w[a][1]), S.w[a][2]);
-    D[a] = tmax<T>(tabs<T>(S.fx[a]), tabs<T>((T)2 - S.fx[a])) * P.dx;
-  }
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    B[a] = (T)0;
-    if (a < DIM) {
-      T acc = tabs<T>(R.mv[a]);
-#pragma unroll
-      for (int b = 0; b < DIM; ++b) acc += tabs<T>(R.aff[a * DIM + b]) * D[b];
-      B[a] = W * acc * margin;
-    }
-  }
-  B[3] = W * P.p_mass * margin;
-}
-
-// G2P of one particle (mpm_solver.py:682-702, 750-758): fetch(i, j, k)
+// G2P of one particle (mpm_solver.py:682-702, 750-758); fetch(i, j, k)
 // returns the grid velocity of stencil node (i, j, k) as a vec4.
+//   v_new = sum_n w_n g_n
+//   C_new = 4 inv_dx sum_n w_n g_n (off_n - fx)^T
+//         = 4 inv_dx (M - v_new (fx - 1)^T),  M = sum_n w_n g_n (off_n - 1)^T
+// with the sums over the separable weights taken innermost axis first.
 template <int DIM, int MAT, class T, int NC, class F>
 __device__ __forceinline__ void g2p_apply(const T (&v)[NC], const ParamsT<T> &P,
                                           const Stencil<DIM, T> &S, F &&fetch, T (&out)[NC]) {
   using CI = Comps<DIM, MAT>;
-  T nv[3] = {0, 0, 0}, nC[DIM * DIM];
+  T nv[3] = {(T)0, (T)0, (T)0};
+  T M[3][3];
 #pragma unroll
-  for (int i = 0; i < DIM * DIM; ++i) nC[i] = (T)0;
-  constexpr int NK = DIM == 3 ? 3 : 1;
+  for (int a = 0; a < 3; ++a)
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+    for (int b = 0; b < 3; ++b) M[a][b] = (T)0;
+  if constexpr (DIM == 3) {
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) {
+      T R[3] = {(T)0, (T)0, (T)0}, Ry[3] = {(T)0, (T)0, (T)0}, Rz[3] = {(T)0, (T)0, (T)0};
 #pragma unroll
-      for (int k = 0; k < NK; ++k) {
-        const int off[3] = {i, j, k};
-        T dpos[3];
+      for (int j = 0; j < 3; ++j) {
+        T Pz[3] = {(T)0, (T)0, (T)0}, Qz[3] = {(T)0, (T)0, (T)0};
 #pragma unroll
-        for (int a = 0; a < DIM; ++a) dpos[a] = (T)off[a] - S.fx[a];
-        const vec4_t<T> nd = fetch(i, j, k);
-        const T g[3] = {nd.x, nd.y, nd.z};
-        T weight = (T)1.0;
+        for (int k = 0; k < 3; ++k) {
+          const vec4_t<T> nd = fetch(i, j, k);
+          const T g[3] = {nd.x, nd.y, nd.z};
 #pragma unroll
-        for (int a = 0; a < DIM; ++a) weight *= S.w[a][off[a]];
+          for (int a = 0; a < 3; ++a) {
+            Pz[a] = tfma<T>(S.w[2][k], g[a], Pz[a]);
+            if (k == 0) Qz[a] = tfma<T>(-S.w[2][0], g[a], Qz[a]);
+            if (k == 2) Qz[a] = tfma<T>(S.w[2][2], g[a], Qz[a]);
+          }
+        }
 #pragma unroll
-        for (int a = 0; a < DIM; ++a) nv[a] = nv[a] + weight * g[a];
-        const T cw = P.c4 * weight;
-#pragma unroll
-        for (int a = 0; a < DIM; ++a)
-#pragma unroll
-          for (int b = 0; b < DIM; ++b) nC[a * DIM + b] = nC[a * DIM + b] + cw * (g[a] * dpos[b]);
+        for (int a = 0; a < 3; ++a) {
+          R[a] = tfma<T>(S.w[1][j], Pz[a], R[a]);
+          Rz[a] = tfma<T>(S.w[1][j], Qz[a], Rz[a]);
+          if (j == 0) Ry[a] = tfma<T>(-S.w[1][0], Pz[a], Ry[a]);
+          if (j == 2) Ry[a] = tfma<T>(S.w[1][2], Pz[a], Ry[a]);
+        }
       }
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        nv[a] = tfma<T>(S.w[0][i], R[a], nv[a]);
+        M[a][1] = tfma<T>(S.w[0][i], Ry[a], M[a][1]);
+        M[a][2] = tfma<T>(S.w[0][i], Rz[a], M[a][2]);
+        if (i == 0) M[a][0] = tfma<T>(-S.w[0][0], R[a], M[a][0]);
+        if (i == 2) M[a][0] = tfma<T>(S.w[0][2], R[a], M[a][0]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      T R[2] = {(T)0, (T)0}, Ry[2] = {(T)0, (T)0};
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const vec4_t<T> nd = fetch(i, j, 0);
+        const T g[2] = {nd.x, nd.y};
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          R[a] = tfma<T>(S.w[1][j], g[a], R[a]);
+          if (j == 0) Ry[a] = tfma<T>(-S.w[1][0], g[a], Ry[a]);
+          if (j == 2) Ry[a] = tfma<T>(S.w[1][2], g[a], Ry[a]);
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        nv[a] = tfma<T>(S.w[0][i], R[a], nv[a]);
+        M[a][1] = tfma<T>(S.w[0][i], Ry[a], M[a][1]);
+        if (i == 0) M[a][0] = tfma<T>(-S.w[0][0], R[a], M[a][0]);
+        if (i == 2) M[a][0] = tfma<T>(S.w[0][2], R[a], M[a][0]);
+      }
+    }
+  }
+  T nC[DIM * DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a)
+#pragma unroll
+    for (int b = 0; b < DIM; ++b) nC[a * DIM + b] = P.c4 * tfma<T>(-nv[a], S.fx[b] - (T)1, M[a][b]);
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
     out[CI::CX + a] = v[CI::CX + a] + P.dt * nv[a];
@@ -362,20 +340,117 @@ __device__ __forceinline__ void g2p_apply(const T (&v)[NC], const ParamsT<T> &P,
     for (int a = 1; a < DIM; ++a) tr = tr + nC[a * DIM + a];
     out[CI::CJ] = ((T)1.0 + P.dt * tr) * v[CI::CJ];
   } else {
-    T M[DIM * DIM];
+    T M2[DIM * DIM];
 #pragma unroll
     for (int a = 0; a < DIM; ++a)
 #pragma unroll
-      for (int b = 0; b < DIM; ++b) M[a * DIM + b] = (a == b ? (T)1 : (T)0) + P.dt * nC[a * DIM + b];
+      for (int b = 0; b < DIM; ++b) M2[a * DIM + b] = (a == b ? (T)1 : (T)0) + P.dt * nC[a * DIM + b];
 #pragma unroll
     for (int a = 0; a < DIM; ++a)
 #pragma unroll
       for (int b = 0; b < DIM; ++b) {
-        T acc = M[a * DIM + 0] * v[CI::CJ + 0 * DIM + b];
+        T acc = M2[a * DIM + 0] * v[CI::CJ + 0 * DIM + b];
 #pragma unroll
-        for (int k = 1; k < DIM; ++k) acc = acc + M[a * DIM + k] * v[CI::CJ + k * DIM + b];
+        for (int k = 1; k < DIM; ++k) acc = acc + M2[a * DIM + k] * v[CI::CJ + k * DIM + b];
         out[CI::CJ + a * DIM + b] = acc;
       }
+  }
+}
+
+// P2G terms of one particle (mpm_solver.py:608-622).  The contribution to
+// stencil node n is
+//   w_n (m v + aff dpos_n),  dpos_n = (off_n - fx) dx,   and mass w_n m,
+// written as w_n (c + A (off_n - 1)) with A = aff dx, c = m v - A (fx - 1).
+// Rows of c and A (and pm) may be scaled by powers of two by the caller.
+template <int DIM, class T> struct P2GTerms {
+  T c[3];
+  T A[3][3];
+  T pm;
+};
+
+template <int DIM, int MAT, class T, int NC>
+__device__ __forceinline__ P2GTerms<DIM, T> p2g_terms(const T (&v)[NC], const ParamsT<T> &P,
+                                                      const Stencil<DIM, T> &S) {
+  using CI = Comps<DIM, MAT>;
+  T aff[DIM * DIM];
+  affine_of<DIM, MAT, T>(v, P, aff);
+  P2GTerms<DIM, T> R;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    R.c[a] = (T)0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) R.A[a][b] = (T)0;
+  }
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    T c = P.p_mass * v[CI::CV + a];
+#pragma unroll
+    for (int b = 0; b < DIM; ++b) {
+      R.A[a][b] = aff[a * DIM + b] * P.dx;
+      c = tfma<T>(-R.A[a][b], S.fx[b] - (T)1, c);
+    }
+    R.c[a] = c;
+  }
+  R.pm = P.p_mass;
+  return R;
+}
+
+// Upper bound of |contribution| of one particle to any of its stencil nodes,
+// per component (momentum x, y, z; mass): max weight * (|c_a| + sum_b
+// |A_ab|), with a 2^-10 margin for the rounding of the contributions
+// themselves.  NaN/Inf terms give a non-finite bound.
+template <int DIM, class T>
+__device__ __forceinline__ void p2g_bounds(const P2GTerms<DIM, T> &R, const Stencil<DIM, T> &S, T (&B)[4]) {
+  constexpr T margin = (T)1 + (T)1 / (T)1024;
+  T W = (T)1;
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) W *= tmax<T>(tmax<T>(S.w[a][0], S.w[a][1]), S.w[a][2]);
+  // a NaN weight must not vanish in tmax
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) W = (S.fx[a] == S.fx[a]) ? W : S.fx[a];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    B[a] = (T)0;
+    if (a < DIM) {
+      T acc = tabs<T>(R.c[a]);
+#pragma unroll
+      for (int b = 0; b < DIM; ++b) acc += tabs<T>(R.A[a][b]);
+      B[a] = W * acc * margin;
+    }
+  }
+  B[3] = W * R.pm * margin;
+}
+
+// Scatter of one particle: add(i, j, k, mom[3], m) for the 3^DIM stencil
+// nodes, mom = w_n (c + A (off_n - 1)), m = w_n pm.
+template <int DIM, class T, class F>
+__device__ __forceinline__ void p2g_emit(const P2GTerms<DIM, T> &R, const Stencil<DIM, T> &S, F &&add) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    T ci[3];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) ci[a] = i == 0 ? R.c[a] - R.A[a][0] : (i == 2 ? R.c[a] + R.A[a][0] : R.c[a]);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const T wij = S.w[0][i] * S.w[1][j];
+      T cij[3];
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) cij[a] = j == 0 ? ci[a] - R.A[a][1] : (j == 2 ? ci[a] + R.A[a][1] : ci[a]);
+      if constexpr (DIM == 3) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const T w = wij * S.w[2][k];
+          T mom[3];
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+            mom[a] = w * (k == 0 ? cij[a] - R.A[a][2] : (k == 2 ? cij[a] + R.A[a][2] : cij[a]));
+          add(i, j, k, mom, w * R.pm);
+        }
+      } else {
+        T mom[3] = {wij * cij[0], wij * cij[1], (T)0};
+        add(i, j, 0, mom, wij * R.pm);
+      }
+    }
   }
 }
 
@@ -405,7 +480,7 @@ __device__ __forceinline__ void p2g_particle(const T (&v)[NC], const ParamsT<T> 
   axis_offsets<DIM>(S.base[0], 0, P.nb, o0);
   axis_offsets<DIM>(S.base[1], 1, P.nb, o1);
   if constexpr (DIM == 3) axis_offsets<DIM>(S.base[2], 2, P.nb, o2);
-  p2g_apply<DIM, MAT, T>(v, P, S, [&](int i, int j, int k, const T (&mom)[3], T m) {
+  p2g_emit<DIM, T>(p2g_terms<DIM, MAT, T>(v, P, S), S, [&](int i, int j, int k, const T (&mom)[3], T m) {
     T *node = (T *)(grid + (o0[i] + o1[j] + o2[k]));
 #pragma unroll
     for (int a = 0; a < DIM; ++a) atomic_add<T>(node + a, mom[a]);

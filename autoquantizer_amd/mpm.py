@@ -325,8 +325,7 @@ class MPMSolver:
                                          self.step_count, int(sort_every), _ptr(self.ids), _stream()))
         self.step_count += int(nsteps)
 
-    def grid_velocity(self):
-        """Last grid [nodes, 4] (v_x, v_y, v_z, m) in the blocked node order."""
+    def _grid_view(self):
         g = ctypes.c_void_p()
         nodes = ctypes.c_int64()
         _lib.check(_lib.lib().aq_sim_grid(self._h, ctypes.byref(g), ctypes.byref(nodes)))
@@ -334,8 +333,19 @@ class MPMSolver:
         esz = 8 if dt == torch.float64 else 4
         off = g.value - self.workspace.data_ptr()
         nbytes = nodes.value * 4 * esz
+        return self.workspace[off:off + nbytes].view(dt).reshape(nodes.value, 4)
+
+    def grid_velocity(self):
+        """Last grid [nodes, 4] (v_x, v_y, v_z, m) in the blocked node order."""
         torch.cuda.current_stream().synchronize()
-        return self.workspace[off:off + nbytes].view(dt).reshape(nodes.value, 4).cpu().numpy()
+        return self._grid_view().cpu().numpy()
+
+    def load_grid(self, g):
+        """Overwrite the grid with g ([nodes, 4], blocked node order): the
+        grid the next G2P reads."""
+        v = self._grid_view()
+        v.copy_(torch.as_tensor(np.asarray(g), dtype=v.dtype).reshape(v.shape))
+        torch.cuda.current_stream().synchronize()
 
     def grid_dense(self):
         """Grid [d+1, n^d] in dense (i, j[, k]) order: velocity components then mass."""

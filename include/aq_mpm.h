@@ -155,7 +155,8 @@ typedef struct aq_sim_desc {
 
 typedef struct aq_sim aq_sim;
 
-/* Workspace for the grid (two node buffers), block flags and the sort. */
+/* Workspace: grid node buffers, block flags, sort buffers, work items and
+   one P2G tile per work item. */
 int64_t aq_sim_workspace_bytes(const aq_sim_desc *desc);
 int aq_sim_create(const aq_sim_desc *desc, void *workspace, int64_t workspace_bytes,
                   aq_sim **out);
@@ -169,7 +170,9 @@ int aq_sim_destroy(aq_sim *sim);
    (by its slot when ids is NULL) and by `step`.
    aq_p2g and aq_g2p use the LDS-tiled kernels when the last aq_sort on this
    state and particle count is still current, global-memory kernels
-   otherwise; the results are the same up to the order of float atomics. */
+   otherwise; the results agree up to float rounding (accumulation order).
+   After aq_p2g the grid (aq_sim_grid) holds (m v, m) per node; aq_grid_op
+   turns it into velocities in place. */
 int aq_substep(aq_sim *sim, void *state, int64_t n, int32_t *ids, uint32_t step, void *stream);
 int aq_p2g(aq_sim *sim, const void *state, int64_t n, void *stream);
 int aq_grid_op(aq_sim *sim, void *stream);
@@ -177,8 +180,8 @@ int aq_g2p(aq_sim *sim, void *state, int64_t n, const int32_t *ids, uint32_t ste
 
 /* nsteps substeps with G2P of substep t fused with P2G of substep t+1
    (one launch reads the state once, writes it once and scatters the next
-   grid).  Same results as nsteps aq_substep calls up to the order of the
-   P2G float atomics.  Particles are re-sorted by grid block every
+   grid).  Same results as nsteps aq_substep calls up to float rounding
+   (accumulation order).  Particles are re-sorted by grid block every
    `sort_every` substeps (0 = never: global-memory kernels); ids are
    permuted along. */
 int aq_advance(aq_sim *sim, void *state, int64_t n, int nsteps, uint32_t step0,
@@ -208,9 +211,10 @@ int aq_update_limits(aq_sim *sim, const void *state, int64_t n, float *mx_mn,
 /* compute_ek (mpm_solver.py:1043-1052) reduced in f64 into *out (device). */
 int aq_energy(aq_sim *sim, const void *state, int64_t n, double *out, void *stream);
 
-/* Device pointer of the grid node buffer holding the last grid velocities
-   (float4 or double4 per node, blocked layout) and its node count; for tests
-   and the adjoint. */
+/* Device pointer of the grid node buffer (float4 or double4 per node,
+   blocked layout: (m v, m) after aq_p2g, (v, m) after the grid update;
+   zero outside the blocks the last P2G touched) and its node count.  The
+   next G2P reads this buffer. */
 int aq_sim_grid(aq_sim *sim, void **grid, int64_t *nodes);
 
 #ifdef __cplusplus

@@ -3,8 +3,10 @@
 - P2G grid (mass, momentum) within f32 accumulation-order tolerance, for the
   global-atomic and the LDS-tiled kernels;
 - grid update exact on the GPU's own P2G grid;
-- G2P from the GPU's own grid: bit-exact words with the oracle's G2P + encode
-  on that grid (both unfused, same operand order), global and tiled kernels;
+- G2P from the GPU's own grid against the oracle's G2P + encode on that grid:
+  within one quantization step, nearly all records identical (the kernels
+  sum the stencil axis by axis); the codec fused into G2P bit-exact against
+  the oracle's encode of the fp32-storage kernel's results on the same grid;
 - whole substeps re-seeded from the GPU state each time: one GPU substep vs
   one oracle substep from identical words; fixed-point codes within one step,
   shared-exponent digits within two (only the P2G summation order differs);
@@ -108,12 +110,29 @@ def test_p2g_grid(orc, kind, tiled):
     assert np.abs(g[:d] - gv).max() <= 2e-5 * mom_scale
 
 
+def assert_within_steps(orc, sim, wg, wo, frac_equal=0.95):
+    """Decoded records of two word arrays within one quantization step
+    (fixed point) / two digits (shared exponent) of each other, and nearly
+    all records identical."""
+    oc = oracle_comps(sim)
+    a = orc.unpack(oc, sim.nwords, wg).astype(np.float64)
+    b = orc.unpack(oc, sim.nwords, wo).astype(np.float64)
+    steps = step_sizes(sim, b)
+    rel = np.abs(a - b) / steps
+    for k, c in enumerate(sim.comps):
+        lim = 1.0 if c["kind"] == 0 else 2.0
+        assert rel[k].max() <= lim + 1e-9, (k, rel[k].max())
+    assert np.mean((wg == wo).all(axis=0)) >= frac_equal
+    return float(rel.max())
+
+
 @pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d"])
 @pytest.mark.parametrize("tiled", [False, True])
-def test_g2p_exact_on_gpu_grid(orc, kind, tiled):
-    """G2P is per-particle deterministic: with the GPU grid as input the oracle
-    G2P + encode gives the same words as the GPU's G2P (noise keyed by the
-    particle id, so compared in the original particle order)."""
+def test_g2p_on_gpu_grid(orc, kind, tiled):
+    """G2P from the GPU's own grid against the oracle's G2P (reference
+    operand order) + encode on that grid: the kernels sum the stencil axis by
+    axis, so records agree within one quantization step and nearly all are
+    identical (noise keyed by the particle id: compared in original order)."""
     print(f"[g2p] {kind} tiled={tiled}", flush=True)
     sim, s = scene(orc, kind, 4000, seed=2)
     if tiled:
@@ -127,7 +146,31 @@ def test_g2p_exact_on_gpu_grid(orc, kind, tiled):
     gv = np.ascontiguousarray(g[:s.dim]).astype(np.float32)
     out = orc.g2p(s, st, gv)
     words_orc = orc.pack(oracle_comps(sim), sim.nwords, out, seed=11, step=0)
-    assert np.array_equal(words_gpu, words_orc)
+    assert_within_steps(orc, sim, words_gpu, words_orc, frac_equal=0.97)
+
+
+@pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d"])
+@pytest.mark.parametrize("tiled", [False, True])
+def test_g2p_codec_bit_exact(orc, kind, tiled):
+    """The codec fused into G2P is bit-exact: the quantized kernel's words
+    equal the oracle's encode of what the fp32-storage kernel computes from
+    the same decoded state on the same grid."""
+    print(f"[g2p codec] {kind} tiled={tiled}", flush=True)
+    sim_q, s = scene(orc, kind, 4000, seed=9)
+    sim_f, _ = scene(orc, kind, 4000, seed=9, storage="f32")
+    dec = sim_q.get_state(original_order=True)
+    sim_f.set_state(dec)
+    if tiled:
+        sim_q.build_pid()
+        sim_f.build_pid()
+    sim_q.p2g_bls()
+    sim_q.grid_op()
+    sim_f.load_grid(sim_q.grid_velocity())
+    sim_q.g2p_bls()
+    sim_f.g2p_bls()
+    out = sim_f.get_state(original_order=True).astype(np.float32)
+    words_orc = orc.pack(oracle_comps(sim_q), sim_q.nwords, out, seed=11, step=0)
+    assert np.array_equal(sim_q.get_words(original_order=True), words_orc)
 
 
 @pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d"])
