@@ -62,6 +62,7 @@ EXPORTS = [
     "aq_store_members", "aq_sim_workspace_bytes", "aq_sim_create", "aq_sim_destroy",
     "aq_substep", "aq_p2g", "aq_grid_op", "aq_g2p", "aq_advance", "aq_sort",
     "aq_update_limits", "aq_energy", "aq_sim_grid", "aq_advance_timed",
+    "aq_adj_workspace_bytes", "aq_adj_substep",
 ]
 
 _lib = None
@@ -99,8 +100,11 @@ def lib():
     L.aq_energy.argtypes = [P, P, i64, P, P]
     L.aq_sim_grid.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(i64)]
     L.aq_advance_timed.argtypes = [P, P, i64, i32, u32, i32, P, P, P, P, P]
+    L.aq_adj_workspace_bytes.restype = i64
+    L.aq_adj_workspace_bytes.argtypes = [P]
+    L.aq_adj_substep.argtypes = [P, P, i64, P, P, P, P, i64, P]
     for name in EXPORTS:
-        if name not in ("aq_last_error", "aq_version", "aq_sim_workspace_bytes"):
+        if name not in ("aq_last_error", "aq_version", "aq_sim_workspace_bytes", "aq_adj_workspace_bytes"):
             getattr(L, name).restype = i32
     _lib = L
     return L

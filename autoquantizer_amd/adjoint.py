@@ -1,0 +1,135 @@
+"""Reverse mode through MPM substeps: checkpointed recomputation around the
+hand-written adjoint kernels (aq_adj_substep), a torch.autograd.Function,
+and the reference's error-contribution sums.
+
+The reference runs its AD pass with Taichi's Tape and an O(n log n)
+checkpoint schedule (mpm_exps/engine/mpm_solver.py:918-931, 1121-1188) and
+accumulates, per quantized component, the squared adjoints of the state over
+every intermediate substep (accu_grad :1232-1271, get_gradients :1273-1302).
+Here the forward keeps a checkpoint every `checkpoint_every` substeps; the
+backward re-runs each segment forward from its checkpoint, keeps the
+segment's states, and walks them in reverse with aq_adj_substep.  The
+adjoint runs on plain storage (f32 or f64, as the reference's AD runs on
+unquantized fields); the sums are reduced in f64.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .mpm import MPMSolver, _ptr, _stream
+
+
+class SubstepAdjoint:
+    """The adjoint of one substep of a plain-storage MPMSolver."""
+
+    def __init__(self, solver: MPMSolver):
+        if solver.storage == _lib.AQ_STORE_QUANT:
+            raise ValueError("the adjoint runs on plain (f32/f64) storage")
+        self.solver = solver
+        nbytes = _lib.lib().aq_adj_workspace_bytes(solver._h)
+        if nbytes < 0:
+            _lib.check(-1)
+        self.ws = torch.empty(int(nbytes), dtype=torch.uint8, device=solver.device)
+
+    def step(self, state_t, adj_next, adj_out, gsum=None):
+        """adj_out = dL/ds_t from adj_next = dL/ds_{t+1}; all [ncomp, capacity]
+        tensors of the solver's dtype.  gsum (float64 [ncomp] on the device):
+        sum_p adj_out^2 is added to it."""
+        s = self.solver
+        _lib.check(_lib.lib().aq_adj_substep(s._h, _ptr(state_t), s.current_idx, _ptr(adj_next), _ptr(adj_out),
+                                             _ptr(gsum) if gsum is not None else None, _ptr(self.ws),
+                                             self.ws.numel(), _stream()))
+
+
+def energy_loss(solver, state):
+    """compute_ek (mpm_solver.py:1043-1052) of a [ncomp, >= n] state tensor,
+    differentiable in torch: 3D sum(1/2 m |v|^2 + g m y), 2D sum |v|^2."""
+    n, d = solver.current_idx, solver.dim
+    v = state[d:2 * d, :n].double()
+    if d == 3:
+        return (0.5 * solver.p_mass * (v * v).sum() + solver.param.g * solver.p_mass * state[1, :n].double().sum())
+    return (v * v).sum()
+
+
+def _advance_plain(solver, state, nsteps):
+    """nsteps untiled plain substeps of `state` (written in place)."""
+    if nsteps <= 0:
+        return
+    _lib.check(_lib.lib().aq_advance(solver._h, _ptr(state), solver.current_idx, int(nsteps), 0, 0, None,
+                                     _stream()))
+
+
+class _Schedule:
+    def __init__(self, solver, nsteps, checkpoint_every):
+        if checkpoint_every < 1:
+            raise ValueError("checkpoint_every must be >= 1")
+        self.solver, self.nsteps, self.k = solver, int(nsteps), int(checkpoint_every)
+        self.checkpoints = []
+
+    def forward(self, state0):
+        """Runs the forward from state0 ([ncomp, capacity]); returns s_T."""
+        s = state0.clone()
+        for t0 in range(0, self.nsteps, self.k):
+            self.checkpoints.append(s.clone())
+            _advance_plain(self.solver, s, min(self.k, self.nsteps - t0))
+        return s
+
+    def backward(self, adj_T, gsum=None, adj=None):
+        """dL/ds_0 from dL/ds_T, adding the squared adjoints of states
+        1..T-1 into gsum."""
+        a = adj_T.clone()
+        b = torch.empty_like(a)
+        for j in reversed(range(len(self.checkpoints))):
+            t0 = j * self.k
+            t1 = min(t0 + self.k, self.nsteps)
+            states = [self.checkpoints[j]]
+            for _ in range(t0 + 1, t1):
+                s = states[-1].clone()
+                _advance_plain(self.solver, s, 1)
+                states.append(s)
+            for t in reversed(range(t0, t1)):
+                adj.step(states[t - t0], a, b, gsum if t >= 1 else None)
+                a, b = b, a
+        return a
+
+
+class Rollout(torch.autograd.Function):
+    """state_T = Rollout.apply(state_0, solver, nsteps, checkpoint_every):
+    nsteps plain substeps as a differentiable torch op.  backward runs the
+    hand-written adjoint; the error-contribution sums of the backward pass
+    are left in solver.adjoint_gsum (float64 [ncomp])."""
+
+    @staticmethod
+    def forward(ctx, state0, solver, nsteps, checkpoint_every=16):
+        sched = _Schedule(solver, nsteps, checkpoint_every)
+        out = sched.forward(state0.detach())
+        ctx.sched = sched
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_T):
+        sched = ctx.sched
+        solver = sched.solver
+        adj = SubstepAdjoint(solver)
+        gsum = torch.zeros(solver.ncomp, dtype=torch.float64, device=solver.device)
+        grad0 = sched.backward(grad_T.contiguous().to(solver.state.dtype), gsum, adj)
+        solver.adjoint_gsum = gsum
+        return grad0, None, None, None
+
+
+def error_contributions(solver, nsteps, loss=energy_loss, checkpoint_every=16):
+    """The reference's AD run (run_forward_with_ad_nlogn + get_gradients):
+    nsteps substeps from the solver's current state, loss(solver, s_T), and
+    per component g_c = sum_{t=1}^{T-1} sum_p (dL/ds_{t,p,c})^2 (f64).
+    Returns (loss value, dL/ds_0 as [ncomp, n] numpy, g as numpy [ncomp]).
+    The solver's state is left unchanged."""
+    n = solver.current_idx
+    s0 = solver.state.clone().requires_grad_(True)
+    sT = Rollout.apply(s0, solver, nsteps, checkpoint_every)
+    L = loss(solver, sT)
+    L.backward()
+    torch.cuda.current_stream().synchronize()
+    g = solver.adjoint_gsum.cpu().numpy()
+    return float(L.detach().cpu()), s0.grad[:, :n].double().cpu().numpy(), g

@@ -105,6 +105,11 @@ def lib():
                                         P, P, P, i32]
         L.orc_limits_f32.restype = None
         L.orc_limits_f32.argtypes = [P, P, ctypes.c_int64, P, P]
+        L.orc_adj_work_doubles.restype = ctypes.c_int64
+        L.orc_adj_work_doubles.argtypes = [P]
+        L.orc_adj_substep_f64.argtypes = [P, P, ctypes.c_int64, P, P, P]
+        L.orc_adjoint_run_f64.restype = ctypes.c_double
+        L.orc_adjoint_run_f64.argtypes = [P, P, ctypes.c_int64, i32, i32, P, P, P]
         _lib = L
     return _lib
 
@@ -291,3 +296,27 @@ def limits(s, state):
     lib().orc_limits_f32(ctypes.byref(s), _ptr(np.ascontiguousarray(state, dtype=np.float32)),
                          state.shape[1], _ptr(mx), _ptr(mn))
     return mx, mn
+
+
+def adj_substep(s, state, adj_next):
+    """dL/ds_t from dL/ds_{t+1} for one plain f64 substep from `state` [nc, n]."""
+    st = np.ascontiguousarray(state, dtype=np.float64)
+    an = np.ascontiguousarray(adj_next, dtype=np.float64)
+    out = np.zeros_like(st)
+    work = np.zeros(lib().orc_adj_work_doubles(ctypes.byref(s)), dtype=np.float64)
+    lib().orc_adj_substep_f64(ctypes.byref(s), _ptr(st), st.shape[1], _ptr(an), _ptr(out), _ptr(work))
+    return out
+
+
+def adjoint_run(s, state0, nsteps, loss="energy", direction=None):
+    """(loss, dL/ds_0, error-contribution sums per component) of nsteps plain
+    f64 substeps; loss "energy" (compute_ek) or "linear" (sum(direction * s_T))."""
+    st = np.ascontiguousarray(state0, dtype=np.float64)
+    nc = st.shape[0]
+    adj0 = np.zeros_like(st)
+    gsum = np.zeros(nc, dtype=np.float64)
+    kind = 0 if loss == "energy" else 1
+    d = np.ascontiguousarray(direction if direction is not None else np.zeros_like(st), dtype=np.float64)
+    L = lib().orc_adjoint_run_f64(ctypes.byref(s), _ptr(st), st.shape[1], int(nsteps), kind, _ptr(d),
+                                  _ptr(adj0), _ptr(gsum))
+    return L, adj0, gsum

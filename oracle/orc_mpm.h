@@ -59,6 +59,18 @@ void orc_run_quant_f32(const orc_mpm *s, const orc_comp *comps, int nwords,
                        uint32_t step0, float *grid_m, float *grid_v,
                        float *scratch /* 2 * nc * n floats */, int nthreads);
 
+/* Adjoint (reverse mode) of one plain f64 substep from state st:
+   adj = dL/ds_t given adj_next = dL/ds_{t+1} (adj_ref.c); work holds
+   orc_adj_work_doubles(s) doubles. */
+int64_t orc_adj_work_doubles(const orc_mpm *s);
+void orc_adj_substep_f64(const orc_mpm *s, const double *st, int64_t n, const double *adj_next,
+                         double *adj, double *work);
+/* nsteps plain f64 substeps from state0 and their adjoint: returns the loss
+   (kind 0: compute_ek; kind 1: sum(dir * s_T)), adj0 = dL/ds_0 and
+   gsum[c] = sum over states t = 1..nsteps-1 and particles of (dL/ds_{t,p,c})^2. */
+double orc_adjoint_run_f64(const orc_mpm *s, const double *state0, int64_t n, int nsteps, int loss_kind,
+                           const double *dir, double *adj0, double *gsum);
+
 /* compute_ek (mpm_solver.py:1043-1052), accumulated in double */
 double orc_energy_f32(const orc_mpm *s, const float *state, int64_t n);
 double orc_energy_f64(const orc_mpm *s, const double *state, int64_t n);

@@ -1,0 +1,119 @@
+"""GPU adjoint of the plain substep (aq_adj_substep, autoquantizer_amd.adjoint)
+against the f64 oracle adjoint (itself pinned by finite differences) and
+against central finite differences of the GPU forward."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def make(orc, kind, n, storage, seed=0):
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import autoquantizer_amd as aq
+    rng = np.random.default_rng(seed)
+    cfg = aq.MPMConfig()
+    if kind == "elastic_2d":
+        cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.E, cfg.nu, cfg.sticky = 2, False, 32, 1e-4, 1e4, 0.2, True
+        s = orc.mpm_params(2, orc.ELASTIC, 32, 1e-4, E=1e4, nu=0.2, boundary=orc.STICKY)
+    elif kind == "fluid_2d":
+        cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.la = 2, True, 32, 2e-4, 400.0
+        s = orc.mpm_params(2, orc.FLUID, 32, 2e-4, la=400.0)
+    else:
+        cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.la = 3, True, 16, 2e-4, 400.0
+        s = orc.mpm_params(3, orc.FLUID, 16, 2e-4, la=400.0)
+    d = cfg.dim
+    cfg.p_vol = (1 / cfg.n_grid * 0.5) ** d
+    cfg.n_particles = n
+    cfg.compute_fp = storage
+    sim = aq.MPMSolver(cfg)
+    nc = sim.ncomp
+    st = np.zeros((nc, n))
+    st[:d] = rng.uniform(0.3, 0.6, (d, n))
+    st[d:2 * d] = rng.uniform(-1, 1, (d, n))
+    st[2 * d:2 * d + d * d] = rng.uniform(-2, 2, (d * d, n))
+    if kind == "elastic_2d":
+        st[-4:] = np.eye(2).reshape(-1, 1) + rng.uniform(-0.05, 0.05, (4, n))
+    else:
+        st[-1] = rng.uniform(0.95, 1.05, n)
+    sim.set_state(st)
+    return sim, s, st
+
+
+@pytest.mark.parametrize("kind", ["fluid_2d", "elastic_2d", "fluid_3d"])
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_adjoint_substep_vs_oracle(orc, kind, storage):
+    from autoquantizer_amd.adjoint import SubstepAdjoint
+    print(f"[adj step] {kind} {storage}", flush=True)
+    sim, s, st = make(orc, kind, 300, storage, seed=1)
+    rng = np.random.default_rng(2)
+    adj_next = rng.standard_normal(st.shape)
+    ref = orc.adj_substep(s, sim.get_state(), adj_next)
+    A = SubstepAdjoint(sim)
+    an = torch.zeros_like(sim.state)
+    an[:, :300] = torch.as_tensor(adj_next, dtype=sim.state.dtype, device="cuda")
+    out = torch.zeros_like(sim.state)
+    gsum = torch.zeros(sim.ncomp, dtype=torch.float64, device="cuda")
+    A.step(sim.state, an, out, gsum)
+    got = out[:, :300].double().cpu().numpy()
+    scale = np.abs(ref).max(axis=1, keepdims=True) + 1e-30
+    tol = 1e-9 if storage == "f64" else 2e-4
+    assert (np.abs(got - ref) / scale).max() < tol
+    np.testing.assert_allclose(gsum.cpu().numpy(), (got ** 2).sum(axis=1), rtol=1e-6 if storage == "f32" else 1e-12)
+
+
+@pytest.mark.parametrize("kind", ["fluid_2d", "elastic_2d"])
+def test_error_contributions_vs_oracle(orc, kind):
+    """Checkpointed run (segments of 4) in f64 against the oracle's full run."""
+    from autoquantizer_amd.adjoint import error_contributions
+    sim, s, st = make(orc, kind, 200, "f64", seed=3)
+    L, grad0, g = error_contributions(sim, 10, checkpoint_every=4)
+    L_ref, adj_ref, g_ref = orc.adjoint_run(s, sim.get_state(), 10)
+    assert L == pytest.approx(L_ref, rel=1e-10)
+    np.testing.assert_allclose(g, g_ref, rtol=1e-7)
+    scale = np.abs(adj_ref).max(axis=1, keepdims=True) + 1e-30
+    assert (np.abs(grad0 - adj_ref) / scale).max() < 1e-8
+
+
+@pytest.mark.parametrize("kind", ["fluid_2d", "elastic_2d", "fluid_3d"])
+def test_f32_adjoint_vs_finite_differences(orc, kind):
+    """fp32 adjoint against central differences (h = 1e-3 of each component's
+    scale) of the GPU's own fp32 forward, within 1e-3 relative."""
+    from autoquantizer_amd.adjoint import error_contributions, energy_loss, _advance_plain
+    print(f"[adj fd] {kind}", flush=True)
+    sim, s, st = make(orc, kind, 400, "f32", seed=5)
+    T = 8
+    L, grad0, _ = error_contributions(sim, T, checkpoint_every=3)
+    rng = np.random.default_rng(6)
+    delta = rng.standard_normal(st.shape) * (np.abs(st).max(axis=1, keepdims=True) + 1e-2)
+    base = sim.state.clone()
+
+    def loss_at(eps):
+        x = base.clone()
+        x[:, :400] += torch.as_tensor(eps * delta, dtype=x.dtype, device="cuda")
+        _advance_plain(sim, x, T)
+        return float(energy_loss(sim, x))
+
+    h = 1e-3
+    fd = (loss_at(h) - loss_at(-h)) / (2 * h)
+    ad = float((grad0 * delta).sum())
+    assert ad == pytest.approx(fd, rel=1e-3)
+
+
+def test_rollout_autograd_linear_loss(orc):
+    """Rollout as a torch op: gradient of a linear functional of s_T against
+    the oracle adjoint of the same functional (f64)."""
+    from autoquantizer_amd.adjoint import Rollout
+    sim, s, st = make(orc, "fluid_2d", 150, "f64", seed=7)
+    rng = np.random.default_rng(8)
+    w = rng.standard_normal(st.shape)
+    W = torch.zeros_like(sim.state)
+    W[:, :150] = torch.as_tensor(w, device="cuda")
+    s0 = sim.state.clone().requires_grad_(True)
+    sT = Rollout.apply(s0, sim, 5, 2)
+    (W * sT).sum().backward()
+    _, adj_ref, _ = orc.adjoint_run(s, sim.get_state(), 5, loss="linear", direction=w)
+    got = s0.grad[:, :150].cpu().numpy()
+    scale = np.abs(adj_ref).max(axis=1, keepdims=True) + 1e-30
+    assert (np.abs(got - adj_ref) / scale).max() < 1e-8

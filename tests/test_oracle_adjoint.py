@@ -1,0 +1,87 @@
+"""The f64 oracle adjoint of the plain substep (oracle/adj_ref.c), pinned by
+central finite differences of its own forward (the reference's Taichi
+autodiff cannot run here) and by the accumulation range of the
+error-contribution sums (mpm_solver.py:1167-1188: states 1..T-1)."""
+import numpy as np
+import pytest
+
+
+def small_scene(orc, kind, n, seed=0):
+    rng = np.random.default_rng(seed)
+    if kind == "fluid_2d":
+        s = orc.mpm_params(2, orc.FLUID, 32, 2e-4, la=400.0)
+        d = 2
+    elif kind == "elastic_2d":
+        s = orc.mpm_params(2, orc.ELASTIC, 32, 1e-4, E=1e4, nu=0.2, boundary=orc.STICKY)
+        d = 2
+    else:
+        s = orc.mpm_params(3, orc.FLUID, 16, 2e-4, la=400.0)
+        d = 3
+    nc = orc.num_comps(s)
+    st = np.zeros((nc, n))
+    st[:d] = rng.uniform(0.3, 0.6, (d, n))
+    st[d:2 * d] = rng.uniform(-1, 1, (d, n))
+    st[2 * d:2 * d + d * d] = rng.uniform(-2, 2, (d * d, n))
+    if kind == "elastic_2d":
+        st[-4:] = np.eye(2).reshape(-1, 1) + rng.uniform(-0.05, 0.05, (4, n))
+    else:
+        st[-1] = rng.uniform(0.95, 1.05, n)
+    return s, st
+
+
+def loss_of(orc, s, st, nsteps, loss, direction=None):
+    x = st.copy()
+    orc.run_plain(s, x, nsteps)
+    if loss == "energy":
+        return orc.energy(s, x)
+    return float((direction * x).sum())
+
+
+@pytest.mark.parametrize("kind", ["fluid_2d", "elastic_2d", "fluid_3d"])
+@pytest.mark.parametrize("loss", ["energy", "linear"])
+def test_adjoint_matches_central_differences(orc, kind, loss):
+    s, st = small_scene(orc, kind, 150, seed=1)
+    rng = np.random.default_rng(2)
+    nsteps = 6
+    direction = rng.standard_normal(st.shape) if loss == "linear" else None
+    L, adj0, _ = orc.adjoint_run(s, st, nsteps, loss=loss, direction=direction)
+    assert L == pytest.approx(loss_of(orc, s, st, nsteps, loss, direction), rel=1e-12)
+    # perturbation of every component, scaled to the component's spread
+    delta = rng.standard_normal(st.shape) * (np.abs(st).max(axis=1, keepdims=True) + 1e-3)
+    h = 1e-6
+    fd = (loss_of(orc, s, st + h * delta, nsteps, loss, direction)
+          - loss_of(orc, s, st - h * delta, nsteps, loss, direction)) / (2 * h)
+    ad = float((adj0 * delta).sum())
+    assert ad == pytest.approx(fd, rel=1e-6, abs=1e-9 * abs(L) + 1e-12)
+
+
+@pytest.mark.parametrize("kind", ["fluid_2d", "elastic_2d"])
+def test_adjoint_per_component(orc, kind):
+    """Each component's adjoint alone against finite differences."""
+    s, st = small_scene(orc, kind, 80, seed=3)
+    nsteps = 4
+    _, adj0, _ = orc.adjoint_run(s, st, nsteps)
+    rng = np.random.default_rng(4)
+    for c in range(st.shape[0]):
+        delta = np.zeros_like(st)
+        delta[c] = rng.standard_normal(st.shape[1])
+        h = 1e-5 * (np.abs(st[c]).max() + 1e-2)
+        fd = (loss_of(orc, s, st + h * delta, nsteps, "energy")
+              - loss_of(orc, s, st - h * delta, nsteps, "energy")) / (2 * h)
+        ad = float((adj0 * delta).sum())
+        assert ad == pytest.approx(fd, rel=1e-5, abs=1e-10), c
+
+
+def test_error_sums_cover_intermediate_states(orc):
+    """g_c sums (dL/ds_t)^2 over t = 1..T-1: the adjoint at state t is the
+    adjoint of the run started from s_t."""
+    s, st = small_scene(orc, "fluid_2d", 60, seed=5)
+    T = 5
+    _, _, gsum = orc.adjoint_run(s, st, T)
+    expect = np.zeros(st.shape[0])
+    x = st.copy()
+    for t in range(1, T):
+        orc.run_plain(s, x, 1)
+        _, adj_t, _ = orc.adjoint_run(s, x, T - t)
+        expect += (adj_t ** 2).sum(axis=1)
+    np.testing.assert_allclose(gsum, expect, rtol=1e-10)
