@@ -56,14 +56,21 @@ struct CodecK {
   uint32_t key_base;     // dithering key of component c: hash32(c ^ key_base)
 };
 
-// Descriptor c, loaded at the point of use: the empty asm hides the pointer's
-// value from the optimiser so the scalar load is not hoisted to the kernel
-// entry (holding 16 descriptors there overflows the scalar registers).
-// (Through the constant address space the uniform load is a scalar load.)
+// Descriptors are loaded where they are used, a few at a time: the empty asm
+// hides the table pointer's value from the optimiser so the scalar loads are
+// not hoisted to the kernel entry (holding 16 descriptors there overflows the
+// scalar registers), and a batch of loads shares one wait.  (Through the
+// constant address space the uniform loads are scalar loads.)
 typedef const __attribute__((address_space(4))) uint32_t ConstU32;
-__device__ __forceinline__ CompK comp_at(const CodecK &cod, int c) {
+constexpr int COMP_BATCH = 4;
+
+__device__ __forceinline__ ConstU32 *comp_table(const CodecK &cod) {
   ConstU32 *p = (ConstU32 *)cod.c;
   asm volatile("" : "+s"(p));
+  return p;
+}
+
+__device__ __forceinline__ CompK comp_load(ConstU32 *p, int c) {
   uint32_t w[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) w[i] = p[c * 8 + i];
@@ -71,6 +78,8 @@ __device__ __forceinline__ CompK comp_at(const CodecK &cod, int c) {
   __builtin_memcpy(&k, w, sizeof(k));
   return k;
 }
+
+__device__ __forceinline__ CompK comp_at(const CodecK &cod, int c) { return comp_load(comp_table(cod), c); }
 
 __device__ __forceinline__ uint32_t lowmask(uint32_t nb) {
   return nb >= 32 ? 0xFFFFFFFFu : ((1u << nb) - 1u);
@@ -122,17 +131,18 @@ __device__ __forceinline__ void row_set(uint32_t *row, uint32_t off, uint32_t nb
 }
 
 // fptosi / fptoui with saturation and NaN -> 0 (the reference's values are
-// LLVM poison outside the int range).
+// LLVM poison outside the int range); branch-free: the conversion runs on the
+// value clamped to the largest float inside the range, and the saturated and
+// NaN cases are selected afterwards.
 __device__ __forceinline__ uint32_t cvt_sat(float y, bool is_signed) {
   if (is_signed) {
-    if (!(y == y)) return 0u;
-    if (y >= 2147483648.0f) return 0x7FFFFFFFu;
-    if (y < -2147483648.0f) return 0x80000000u;
-    return (uint32_t)(int32_t)y;
+    int32_t r = (int32_t)fminf(fmaxf(y, -2147483648.0f), 2147483520.0f);
+    r = y >= 2147483648.0f ? 0x7FFFFFFF : r;
+    return y == y ? (uint32_t)r : 0u;
   }
-  if (!(y > 0.0f)) return 0u;
-  if (y >= 4294967296.0f) return 0xFFFFFFFFu;
-  return (uint32_t)y;
+  uint32_t r = (uint32_t)fminf(fmaxf(y, 0.0f), 4294967040.0f);
+  r = y >= 4294967296.0f ? 0xFFFFFFFFu : r;
+  return y > 0.0f ? r : 0u;
 }
 
 // Dithering noise in [0, 1): lowbias32 (C. Wellons, hash-prospector) of the
@@ -303,36 +313,42 @@ __device__ __forceinline__ float comp_noise(const CodecK &cod, int c, uint32_t s
 template <int NC>
 __device__ __forceinline__ void encode_record(const float (&v)[NC], const CodecK &cod, uint32_t slot,
                                               uint32_t *row, float (&dec)[NC]) {
+  // the structure masks and the key base are re-read here (hidden from the
+  // optimiser) rather than kept live across the caller's loops
+  uint32_t sm = cod.shared_mask, gf = cod.gfirst_mask, gl = cod.glast_mask, kb = cod.key_base;
+  asm volatile("" : "+s"(sm), "+s"(gf), "+s"(gl), "+s"(kb));
   // largest f32 exponent of each shared group, spread over its members
   uint32_t gmax[NC];
   uint32_t run = 0u;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     gmax[c] = 0u;
-    if ((cod.shared_mask >> c) & 1u) {
+    if ((sm >> c) & 1u) {
       const uint32_t e = f32_exp(v[c]);
-      run = ((cod.gfirst_mask >> c) & 1u) ? e : (run > e ? run : e);
+      run = ((gf >> c) & 1u) ? e : (run > e ? run : e);
       gmax[c] = run;
     }
   }
 #pragma unroll
   for (int c = NC - 1; c >= 0; --c) {
-    if ((cod.shared_mask >> c) & 1u) {
-      run = ((cod.glast_mask >> c) & 1u) ? gmax[c] : run;
+    if ((sm >> c) & 1u) {
+      run = ((gl >> c) & 1u) ? gmax[c] : run;
       gmax[c] = run;
     }
   }
+  ConstU32 *table = nullptr;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    const CompK k = comp_at(cod, c);
-    if ((cod.shared_mask >> c) & 1u) {
+    if (c % COMP_BATCH == 0) table = comp_table(cod);
+    const CompK k = comp_load(table, c);
+    if ((sm >> c) & 1u) {
       const uint32_t e = shared_exp_field(gmax[c], k);
-      if ((cod.gfirst_mask >> c) & 1u) row_put(row, k.eoff, k.exp_bits, e);
+      if ((gf >> c) & 1u) row_put(row, k.eoff, k.exp_bits, e);
       const uint32_t d = enc_shared(v[c], gmax[c], k);
       row_put(row, k.off, k.nbits, d);
       dec[c] = dec_shared(e, d, k);
     } else if (k.kind == K_FIXED) {
-      const float u = k.dither ? comp_noise(cod, c, slot) : 0.0f;
+      const float u = k.dither ? noise_u(slot, hash32((uint32_t)c ^ kb)) : 0.0f;
       const uint32_t q = enc_fixed(v[c], k, u);
       row_put(row, k.off, k.nbits, q);
       dec[c] = dec_fixed(q, k);

@@ -447,9 +447,8 @@ __global__ void __launch_bounds__(256) k_g2p2g(StateK s, CodecK cod, ParamsT<T> 
 // A work item is up to 256 particles whose stencil base was in one grid block
 // at the last aq_sort: int4 (block, first particle, count, 0).  The item's
 // tile covers the block, the 2-node stencil reach and a 1-cell drift margin
-// on each side: TE = E + 4 nodes per axis from block_origin - 1, stored as
-// 2^DIM groups of 64 slots (slot = group * 64 + the node's index inside an
-// E^DIM box), so the nodes of any E^DIM window fall in 64 distinct LDS banks.
+// on each side: TE = E + 4 nodes per axis from block_origin - 1 (LDS layout
+// in Tile; in the tile buffer, TE^DIM nodes in (x, y, z) order).
 //
 // G2P reads grid velocities staged in LDS.  P2G accumulates in LDS in fixed
 // point (integer LDS atomics; float LDS atomics are ~30x slower on this
@@ -464,26 +463,27 @@ __global__ void __launch_bounds__(256) k_g2p2g(StateK s, CodecK cod, ParamsT<T> 
 // into grid_f.
 template <int DIM> struct Tile {
   static constexpr int E = BlockShape<DIM>::E, LG = BlockShape<DIM>::LOG;
-  static constexpr int TE = E + 4;
-  static constexpr int TS = (1 << DIM) * 64;
-  __device__ static int slot(int x, int y, int z) {
-    if constexpr (DIM == 3)
-      return ((((x >> 2) * 2 + (y >> 2)) * 2 + (z >> 2)) << 6) | ((x & 3) << 4) | ((y & 3) << 2) | (z & 3);
-    else
-      return (((x >> 3) * 2 + (y >> 3)) << 6) | ((x & 7) << 3) | (y & 7);
-  }
-  __device__ static void coords(int q, int (&t)[3]) {
-    const int l = q & 63, g = q >> 6;
+  static constexpr int TE = E + 4;                                // nodes per axis
+  static constexpr int TN = DIM == 3 ? TE * TE * TE : TE * TE;    // nodes (global tile stride)
+  // LDS slot of tile node (x, y, z): x SX + y SY + z.  SX = 112 = 48 mod 64,
+  // SY = 12 (3D) / SX = 72 = 8 mod 64 (2D): the nodes of any 4x4x4 (8x8)
+  // window get 64 distinct banks, and stencil offsets are immediate offsets.
+  static constexpr int SY = DIM == 3 ? 12 : 1;
+  static constexpr int SX = DIM == 3 ? 112 : 72;
+  static constexpr int TS = TE * SX;                              // LDS slots
+  __device__ static int slot(int x, int y, int z) { return DIM == 3 ? x * SX + y * SY + z : x * SX + y; }
+  __device__ static void node_coords(int nd, int (&t)[3]) {
     if constexpr (DIM == 3) {
-      t[0] = ((g >> 2) & 1) * 4 + (l >> 4);
-      t[1] = ((g >> 1) & 1) * 4 + ((l >> 2) & 3);
-      t[2] = (g & 1) * 4 + (l & 3);
+      t[0] = nd / (TE * TE);
+      t[1] = (nd / TE) % TE;
+      t[2] = nd % TE;
     } else {
-      t[0] = ((g >> 1) & 1) * 8 + (l >> 3);
-      t[1] = (g & 1) * 8 + (l & 7);
+      t[0] = nd / TE;
+      t[1] = nd % TE;
       t[2] = 0;
     }
   }
+  __device__ static int node_of(int x, int y, int z) { return DIM == 3 ? (x * TE + y) * TE + z : x * TE + y; }
 };
 
 template <int DIM>
@@ -523,8 +523,12 @@ template <class T> struct AccOf;
 template <> struct AccOf<float> {
   using U = uint32_t;
   static constexpr int BITS = 30;
-  // round(x) as int32 (two's complement), |x| < 2^31
-  __device__ static U fix(float x) { return (U)(int32_t)rintf(x); }
+  // floor(x + 1/2) as int32 (two's complement), |x| < 2^31
+  __device__ static U fix(float x) {
+    int32_t r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return (U)r;
+  }
   __device__ static float unfix(U q, int k) { return ldexpf((float)(int32_t)q, -k); }
 };
 template <> struct AccOf<double> {
@@ -552,15 +556,19 @@ template <class T> __device__ __forceinline__ T wave_sum(T x) {
   return x;
 }
 
+// Quantized records: at least 4 waves per SIMD (<= 128 VGPRs); measured
+// ~11% faster than the 3 the register allocator picks by itself.
+template <int ST> struct TiledWaves { static constexpr int W = ST == AQ_STORE_QUANT ? 4 : 1; };
 template <int DIM, int MAT, int ST, int NW32, class T, bool DO_G2P, bool DO_P2G>
-__global__ void __launch_bounds__(256) k_tiled(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TiledWaves<ST>::W)))
+k_tiled(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
                                                const int4 *__restrict__ items, const int32_t *__restrict__ nitems,
                                                const vec4_t<T> *__restrict__ grid_v, vec4_t<T> *__restrict__ tiles,
                                                vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags) {
   using TL = Tile<DIM>;
   using ACC = AccOf<T>;
   using U = typename ACC::U;
-  constexpr int TS = TL::TS, TE = TL::TE, NC = Comps<DIM, MAT>::NC;
+  constexpr int TS = TL::TS, TN = TL::TN, TE = TL::TE, NC = Comps<DIM, MAT>::NC;
   __shared__ vec4_t<T> tv[DO_G2P ? TS : 1];
   __shared__ U ts[DO_P2G ? 4 : 1][DO_P2G ? TS : 1];
   __shared__ T bpart[4][4];
@@ -581,16 +589,15 @@ __global__ void __launch_bounds__(256) k_tiled(StateK s, CodecK cod, ParamsT<T> 
 #pragma unroll
     for (int a = 0; a < 3; ++a) o[a] = a < DIM ? bc[a] * TL::E - 1 : 0;
     if constexpr (DO_G2P) {
-      for (int q = tid; q < TS; q += 256) {
+      for (int nd = tid; nd < TN; nd += 256) {
         int t[3];
-        TL::coords(q, t);
+        TL::node_coords(nd, t);
         const int gi = o[0] + t[0], gj = o[1] + t[1], gk = o[2] + t[2];
         vec4_t<T> val;
         val.x = val.y = val.z = val.w = (T)0;
-        const bool in = t[0] < TE && t[1] < TE && (DIM == 2 || t[2] < TE) && gi >= 0 && gi < P.n && gj >= 0 &&
-                        gj < P.n && (DIM == 2 || (gk >= 0 && gk < P.n));
+        const bool in = gi >= 0 && gi < P.n && gj >= 0 && gj < P.n && (DIM == 2 || (gk >= 0 && gk < P.n));
         if (in) val = grid_v[node_index<DIM>(gi, gj, gk, P.nb)];
-        tv[q] = val;
+        tv[TL::slot(t[0], t[1], t[2])] = val;
       }
     }
     __syncthreads();  // tile staged; the previous item's tile written out
@@ -672,14 +679,17 @@ __global__ void __launch_bounds__(256) k_tiled(StateK s, CodecK cod, ParamsT<T> 
         }
       }
       __syncthreads();  // tile sums complete
-      vec4_t<T> *dst = tiles + (int64_t)it * TS;
-      for (int q = tid; q < TS; q += 256) {
+      vec4_t<T> *dst = tiles + (int64_t)it * TN;
+      for (int nd = tid; nd < TN; nd += 256) {
+        int t[3];
+        TL::node_coords(nd, t);
+        const int q = TL::slot(t[0], t[1], t[2]);
         vec4_t<T> val;
         val.x = ACC::unfix(ts[0][q], k[0]);
         val.y = ACC::unfix(ts[1][q], k[1]);
         val.z = DIM == 3 ? ACC::unfix(ts[2][q], k[2]) : (T)0;
         val.w = ACC::unfix(ts[3][q], k[3]);
-        dst[q] = val;
+        dst[nd] = val;
         ts[0][q] = ts[1][q] = ts[2][q] = ts[3][q] = (U)0;
       }
     } else {
@@ -726,19 +736,22 @@ template <int DIM> __device__ __forceinline__ void lane_coords(int lane, int (&l
   }
 }
 
-// Gather of P2G results into grid_v, one 64-node block per wavefront: the
-// tiles of the items of the 3^DIM neighbouring blocks that cover each node
-// (in a fixed order: neighbour blocks, then items), plus grid_f for blocks
-// flagged by untiled scatters (grid_f is zeroed after use).  OP applies the
-// grid update; without it grid_v holds (m v, m).  Blocks with no
-// contribution are marked inactive in `act` (and zeroed if they were active).
+// Gather of P2G results into grid_v, one 64-node block per wavefront.  A
+// node is covered by the tiles of 2^DIM neighbouring blocks (per axis its own
+// block and the one on the side of the block edge it is nearest); each lane
+// reads the item ranges of its 2^DIM blocks and the first two items' tile
+// values all at once (independent loads), then any further items, and sums
+// in a fixed order (neighbour, then item); grid_f is added for blocks flagged
+// by untiled scatters (and zeroed after use).  OP applies the grid update;
+// without it grid_v holds (m v, m).  Blocks with no contribution are marked
+// inactive in `act` (and zeroed if they were active).
 template <int DIM, class T, bool OP>
 __global__ void __launch_bounds__(256) k_gather(ParamsT<T> P, int32_t nblocks, const int32_t *__restrict__ item_off,
                                                 const vec4_t<T> *__restrict__ tiles, vec4_t<T> *__restrict__ grid_f,
                                                 uint32_t *__restrict__ fflags, vec4_t<T> *__restrict__ grid_v,
                                                 uint32_t *__restrict__ act) {
   using TL = Tile<DIM>;
-  constexpr int E = TL::E, TE = TL::TE, TS = TL::TS;
+  constexpr int E = TL::E, TN = TL::TN, NN = 1 << DIM;
   const int32_t b = __builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * 4u + (threadIdx.x >> 6)));
   const int lane = threadIdx.x & 63;
   if (b >= nblocks) return;
@@ -747,46 +760,31 @@ __global__ void __launch_bounds__(256) k_gather(ParamsT<T> P, int32_t nblocks, c
   block_coords<DIM>(b, nb, bc);
   lane_coords<DIM>(lane, loc);
   const bool fall = fflags[b] != 0u;
-  bool any = fall;
-  vec4_t<T> sum;
-  sum.x = sum.y = sum.z = sum.w = (T)0;
-  if (item_off) {
-    constexpr int DZ = DIM == 3 ? 1 : 0;
-    for (int dx = -1; dx <= 1; ++dx)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dz = -DZ; dz <= DZ; ++dz) {
-          const int d[3] = {dx, dy, dz};
-          int nbc[3];
-          bool ok = true;
+  int32_t r0[NN], r1[NN], q[NN];
 #pragma unroll
-          for (int a = 0; a < DIM; ++a) {
-            nbc[a] = bc[a] + d[a];
-            ok = ok && nbc[a] >= 0 && nbc[a] < nb;
-          }
-          if (!ok) continue;
-          const int32_t nbid = DIM == 3 ? (nbc[0] * nb + nbc[1]) * nb + nbc[2] : nbc[0] * nb + nbc[1];
-          const int32_t r0 = item_off[nbid], r1 = item_off[nbid + 1];
-          if (r1 <= r0) continue;
-          any = true;
-          int t[3] = {0, 0, 0};
-          bool cov = true;
+  for (int k = 0; k < NN; ++k) {
+    int d[3] = {0, 0, 0}, t[3] = {0, 0, 0};
+    bool ok = item_off != nullptr;
 #pragma unroll
-          for (int a = 0; a < DIM; ++a) {
-            t[a] = loc[a] + 1 - d[a] * E;
-            cov = cov && t[a] >= 0 && t[a] < TE;
-          }
-          if (cov) {
-            const int q = TL::slot(t[0], t[1], t[2]);
-            for (int32_t it = r0; it < r1; ++it) {
-              const vec4_t<T> c = tiles[(int64_t)it * TS + q];
-              sum.x += c.x;
-              sum.y += c.y;
-              sum.z += c.z;
-              sum.w += c.w;
-            }
-          }
-        }
+    for (int a = 0; a < DIM; ++a) {
+      d[a] = ((k >> a) & 1) ? (loc[a] == E - 1 ? 1 : -1) : 0;
+      const int c = bc[a] + d[a];
+      t[a] = loc[a] + 1 - d[a] * E;
+      ok = ok && c >= 0 && c < nb && t[a] < TL::TE;  // 2D: only 3 nodes reach past the edge
+    }
+    q[k] = TL::node_of(t[0], t[1], t[2]);
+    r0[k] = r1[k] = 0;
+    if (ok) {
+      const int32_t nbid = DIM == 3 ? ((bc[0] + d[0]) * nb + (bc[1] + d[1])) * nb + (bc[2] + d[2])
+                                    : (bc[0] + d[0]) * nb + (bc[1] + d[1]);
+      r0[k] = item_off[nbid];
+      r1[k] = item_off[nbid + 1];
+    }
   }
+  bool mine = false;
+#pragma unroll
+  for (int k = 0; k < NN; ++k) mine = mine || r1[k] > r0[k];
+  const bool any = fall || __ballot(mine) != 0ull;
   const int64_t node = (int64_t)b * 64 + lane;
   if (!any) {
     if (act[b]) {
@@ -797,6 +795,34 @@ __global__ void __launch_bounds__(256) k_gather(ParamsT<T> P, int32_t nblocks, c
       if (lane == 0) act[b] = 0u;
     }
     return;
+  }
+  vec4_t<T> c0[NN], c1[NN];
+#pragma unroll
+  for (int k = 0; k < NN; ++k) {
+    c0[k].x = c0[k].y = c0[k].z = c0[k].w = (T)0;
+    c1[k] = c0[k];
+    if (r1[k] > r0[k]) c0[k] = tiles[(int64_t)r0[k] * TN + q[k]];
+    if (r1[k] > r0[k] + 1) c1[k] = tiles[(int64_t)(r0[k] + 1) * TN + q[k]];
+  }
+  vec4_t<T> sum;
+  sum.x = sum.y = sum.z = sum.w = (T)0;
+#pragma unroll
+  for (int k = 0; k < NN; ++k) {
+    sum.x += c0[k].x;
+    sum.y += c0[k].y;
+    sum.z += c0[k].z;
+    sum.w += c0[k].w;
+    sum.x += c1[k].x;
+    sum.y += c1[k].y;
+    sum.z += c1[k].z;
+    sum.w += c1[k].w;
+    for (int32_t it = r0[k] + 2; it < r1[k]; ++it) {
+      const vec4_t<T> c = tiles[(int64_t)it * TN + q[k]];
+      sum.x += c.x;
+      sum.y += c.y;
+      sum.z += c.z;
+      sum.w += c.w;
+    }
   }
   if (fall) {
     const vec4_t<T> c = grid_f[node];
@@ -1176,7 +1202,7 @@ static int32_t max_items_of(const aq_sim_desc *d) {
   return (int32_t)(d->capacity / 256 + ne + 1);
 }
 
-static int64_t tile_slots(const aq_sim_desc *d) { return d->dim == 3 ? 512 : 256; }
+static int64_t tile_slots(const aq_sim_desc *d) { return d->dim == 3 ? 512 : 144; }  // Tile<DIM>::TN
 
 enum {
   WS_GRID_V, WS_GRID_F, WS_FLAGS, WS_KEY, WS_RANK, WS_COUNT, WS_OFF, WS_PART, WS_TMP_STATE, WS_TMP_IDS,

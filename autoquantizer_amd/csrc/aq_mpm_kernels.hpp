@@ -83,13 +83,15 @@ __device__ __forceinline__ void load_state(const StateK &s, const CodecK &cod, i
       }
     }
     constexpr int CJ = 2 * DIM + DIM * DIM;
+    ConstU32 *table = comp_table(cod);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       if (G2P_ONLY && c >= DIM && c < CJ) {
         v[c] = (T)0;
         continue;
       }
-      v[c] = (T)decode_row(row, comp_at(cod, c));
+      if (!G2P_ONLY && c % COMP_BATCH == 0) table = comp_table(cod);
+      v[c] = (T)decode_row(row, comp_load(table, c));
     }
   } else {
     const T *vals = (const T *)s.ptr;

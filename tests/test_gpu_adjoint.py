@@ -77,28 +77,31 @@ def test_error_contributions_vs_oracle(orc, kind):
 
 
 @pytest.mark.parametrize("kind", ["fluid_2d", "elastic_2d", "fluid_3d"])
-def test_f32_adjoint_vs_finite_differences(orc, kind):
-    """fp32 adjoint against central differences (h = 1e-3 of each component's
-    scale) of the GPU's own fp32 forward, within 1e-3 relative."""
+def test_adjoint_vs_finite_differences(orc, kind):
+    """Directional derivative of the loss of 8 substeps: central differences
+    of the GPU f64 forward (h = 1e-6 of each component's scale) against the
+    f64 adjoint (1e-6 relative) and the fp32 adjoint (1e-3 relative)."""
     from autoquantizer_amd.adjoint import error_contributions, energy_loss, _advance_plain
     print(f"[adj fd] {kind}", flush=True)
-    sim, s, st = make(orc, kind, 400, "f32", seed=5)
-    T = 8
-    L, grad0, _ = error_contributions(sim, T, checkpoint_every=3)
+    n, T = 400, 8
+    sim64, s, st = make(orc, kind, n, "f64", seed=5)
+    sim32, _, _ = make(orc, kind, n, "f32", seed=5)
+    _, g64, _ = error_contributions(sim64, T, checkpoint_every=3)
+    _, g32, _ = error_contributions(sim32, T, checkpoint_every=3)
     rng = np.random.default_rng(6)
     delta = rng.standard_normal(st.shape) * (np.abs(st).max(axis=1, keepdims=True) + 1e-2)
-    base = sim.state.clone()
+    base = sim64.state.clone()
 
     def loss_at(eps):
         x = base.clone()
-        x[:, :400] += torch.as_tensor(eps * delta, dtype=x.dtype, device="cuda")
-        _advance_plain(sim, x, T)
-        return float(energy_loss(sim, x))
+        x[:, :n] += torch.as_tensor(eps * delta, dtype=x.dtype, device="cuda")
+        _advance_plain(sim64, x, T)
+        return float(energy_loss(sim64, x))
 
-    h = 1e-3
+    h = 1e-6
     fd = (loss_at(h) - loss_at(-h)) / (2 * h)
-    ad = float((grad0 * delta).sum())
-    assert ad == pytest.approx(fd, rel=1e-3)
+    assert float((g64 * delta).sum()) == pytest.approx(fd, rel=1e-6)
+    assert float((g32 * delta).sum()) == pytest.approx(fd, rel=1e-3)
 
 
 def test_rollout_autograd_linear_loss(orc):
