@@ -54,6 +54,8 @@ struct CodecK {
   uint32_t gfirst_mask;  // bit c: first member of its group
   uint32_t glast_mask;   // bit c: last member of its group
   uint32_t key_base;     // dithering key of component c: hash32(c ^ key_base)
+  uint32_t stream;       // fields lie in encode order at increasing offsets (StreamSink applies)
+  uint32_t pad;
 };
 
 // Descriptors are loaded where they are used, a few at a time: the empty asm
@@ -307,12 +309,59 @@ __device__ __forceinline__ float comp_noise(const CodecK &cod, int c, uint32_t s
   return noise_u(slot, hash32((uint32_t)c ^ cod.key_base));
 }
 
-// Encode NC values (compile-time count) into a zeroed LDS row and return in
-// dec[] the values the stored fields decode to (the decode of each field's
-// digits, computed from the digits in registers).
-template <int NC>
+// Field writers of the record encoder: put(off, nbits, value) with value
+// masked to nbits.
+// RowSink ORs fields into a zeroed per-thread LDS row (any layout).
+struct RowSink {
+  uint32_t *row;
+  __device__ __forceinline__ void put(uint32_t off, uint32_t nb, uint32_t v) { row_put(row, off, nb, v); }
+};
+
+// StreamSink builds the record in a 64-bit register and stores each word of
+// the plane-major record as it completes (words[i * cap + p]); fields must
+// come at non-decreasing offsets (CodecK::stream).  Every offset is uniform,
+// so the word bookkeeping is scalar and no LDS is touched.
+struct StreamSink {
+  uint64_t *words;
+  int64_t cap, p;
+  uint32_t nw;
+  uint32_t pos;  // bit position of the stream (uniform)
+  uint64_t acc;  // bits of the current word
+  __device__ __forceinline__ void store() {
+    __builtin_nontemporal_store(acc, words + (int64_t)(pos >> 6) * cap + p);
+  }
+  __device__ __forceinline__ void put(uint32_t off, uint32_t nb, uint32_t v) {
+    while ((pos >> 6) < (off >> 6)) {  // words before the field are complete
+      store();
+      acc = 0ull;
+      pos = ((pos >> 6) + 1u) << 6;
+    }
+    pos = off;
+    const uint32_t b = pos & 63u;
+    acc |= (uint64_t)v << b;
+    if (b + nb >= 64u) {
+      store();
+      acc = b + nb > 64u ? (uint64_t)v >> (64u - b) : 0ull;
+    }
+    pos += nb;
+  }
+  __device__ __forceinline__ void finish() {
+    while ((pos >> 6) < nw) {
+      store();
+      acc = 0ull;
+      pos = ((pos >> 6) + 1u) << 6;
+    }
+  }
+};
+
+// Encode NC values (compile-time count) with `sink` and return in dec[] the
+// values the stored fields decode to (the decode of each field's digits,
+// computed from the digits in registers).  Fields are written in component
+// order, a shared group's exponent before its first member and an own
+// exponent before its digits.
+template <int NC, class Sink>
 __device__ __forceinline__ void encode_record(const float (&v)[NC], const CodecK &cod, uint32_t slot,
-                                              uint32_t *row, float (&dec)[NC]) {
+                                              Sink &sink, float (&dec)[NC]) {
   // the structure masks and the key base are re-read here (hidden from the
   // optimiser) rather than kept live across the caller's loops
   uint32_t sm = cod.shared_mask, gf = cod.gfirst_mask, gl = cod.glast_mask, kb = cod.key_base;
@@ -343,27 +392,27 @@ __device__ __forceinline__ void encode_record(const float (&v)[NC], const CodecK
     const CompK k = comp_load(table, c);
     if ((sm >> c) & 1u) {
       const uint32_t e = shared_exp_field(gmax[c], k);
-      if ((gf >> c) & 1u) row_put(row, k.eoff, k.exp_bits, e);
+      if ((gf >> c) & 1u) sink.put(k.eoff, k.exp_bits, e);
       const uint32_t d = enc_shared(v[c], gmax[c], k);
-      row_put(row, k.off, k.nbits, d);
+      sink.put(k.off, k.nbits, d);
       dec[c] = dec_shared(e, d, k);
     } else if (k.kind == K_FIXED) {
       const float u = k.dither ? noise_u(slot, hash32((uint32_t)c ^ kb)) : 0.0f;
       const uint32_t q = enc_fixed(v[c], k, u);
-      row_put(row, k.off, k.nbits, q);
+      sink.put(k.off, k.nbits, q);
       dec[c] = dec_fixed(q, k);
     } else if (k.kind == K_FLOAT) {
       uint32_t e;
       const uint32_t d = enc_float(v[c], k, e);
-      row_put(row, k.eoff, k.exp_bits, e);
-      row_put(row, k.off, k.nbits, d);
+      sink.put(k.eoff, k.exp_bits, e);
+      sink.put(k.off, k.nbits, d);
       dec[c] = dec_float(e, d, k);
     } else if (k.kind == K_F32) {
-      row_put(row, k.off, 32u, __float_as_uint(v[c]));
+      sink.put(k.off, 32u, __float_as_uint(v[c]));
       dec[c] = v[c];
     } else {
       const uint32_t q = cvt_sat(v[c], k.is_signed) & lowmask(k.nbits);
-      row_put(row, k.off, k.nbits, q);
+      sink.put(k.off, k.nbits, q);
       dec[c] = k.is_signed ? (float)sext(q, k.nbits) : (float)q;
     }
   }

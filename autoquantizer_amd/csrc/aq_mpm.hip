@@ -588,6 +588,17 @@ k_tiled(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
     block_coords<DIM>(item.x, P.nb, bc);
 #pragma unroll
     for (int a = 0; a < 3; ++a) o[a] = a < DIM ? bc[a] * TL::E - 1 : 0;
+    const bool has = tid < item.z && (int64_t)item.y + tid < n;
+    const int64_t p = (int64_t)item.y + tid;
+    // the particle's loads go out before the tile's, so their latencies overlap
+    Fetched<ST, NW32> fw;
+    T v[NC];
+    if (has) {
+      if constexpr (DO_G2P)
+        fetch_state<DIM, ST, NW32, NC, T, true>(s, p, s.g2p_mask, fw, v);
+      else
+        fetch_state<DIM, ST, NW32, NC, T>(s, p, 0xFFFFFFFFu, fw, v);
+    }
     if constexpr (DO_G2P) {
       for (int nd = tid; nd < TN; nd += 256) {
         int t[3];
@@ -601,13 +612,10 @@ k_tiled(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
       }
     }
     __syncthreads();  // tile staged; the previous item's tile written out
-    const bool has = tid < item.z && (int64_t)item.y + tid < n;
-    const int64_t p = (int64_t)item.y + tid;
     T out[NC];
     if (has) {
       if constexpr (DO_G2P) {
-        T v[NC];
-        load_state<DIM, ST, NW32, NC, T, true>(s, cod, p, s.g2p_mask, row, v);
+        finish_state<DIM, ST, NW32, NC, T, true>(s, cod, s.g2p_mask, fw, row, v);
         const Stencil<DIM, T> S = stencil_of<DIM, MAT, T>(v, P);
         int rel[3] = {0, 0, 0};
         bool inside = true;
@@ -624,7 +632,9 @@ k_tiled(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
         }
         store_state<ST, NW32, NC, T>(s, cod, p, out, row);  // out: what storage now holds
       } else {
-        load_state<DIM, ST, NW32, NC, T>(s, cod, p, 0xFFFFFFFFu, row, out);
+        finish_state<DIM, ST, NW32, NC, T>(s, cod, 0xFFFFFFFFu, fw, row, v);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) out[c] = v[c];
       }
     }
     if constexpr (DO_P2G) {
@@ -1165,8 +1175,29 @@ struct aq_sim {
   const void *items_state;
   int64_t items_n;
   bool dither_any;
+  bool stream_order;  // the encoder's field order has non-decreasing offsets
   int64_t ws_bytes;
 };
+
+// Whether the record's fields, in the encoder's order (components in order,
+// a shared group's exponent before its first member, an own exponent before
+// its digits), lie at non-decreasing bit offsets: then records are written
+// as a bit stream from registers (StreamSink).
+static bool stream_ordered(const LayoutK &lay) {
+  uint32_t pos = 0;
+  auto put = [&](uint32_t off, uint32_t nb) {
+    if (off < pos) return false;
+    pos = off + nb;
+    return pos <= (uint32_t)lay.nw * 64u;
+  };
+  for (int c = 0; c < lay.nc; ++c) {
+    const CompK &k = lay.c[c];
+    if (k.kind == K_FLOAT && (!k.shared || k.gfirst))
+      if (!put(k.eoff, k.exp_bits)) return false;
+    if (!put(k.off, k.kind == K_F32 ? 32u : k.nbits)) return false;
+  }
+  return true;
+}
 
 static int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
@@ -1280,6 +1311,7 @@ extern "C" int aq_sim_create(const aq_sim_desc *d, void *ws, int64_t ws_bytes, a
   aq_sim *s = new aq_sim();
   s->d = *d;
   s->dither_any = false;
+  s->stream_order = false;
   if (d->storage == AQ_STORE_QUANT) {
     rc = build_layout(d->comps, d->ncomp, d->nwords, &s->lay);
     if (rc) {
@@ -1287,6 +1319,7 @@ extern "C" int aq_sim_create(const aq_sim_desc *d, void *ws, int64_t ws_bytes, a
       return rc;
     }
     for (int c = 0; c < d->ncomp; ++c) s->dither_any = s->dither_any || s->lay.c[c].dither;
+    s->stream_order = stream_ordered(s->lay);
   } else {
     std::memset(&s->lay, 0, sizeof(s->lay));
   }
@@ -1376,6 +1409,8 @@ static CodecK make_codec(const aq_sim *s, uint32_t step) {
     }
   }
   c.key_base = key_base(s->d.seed, step);
+  c.stream = s->stream_order ? 1u : 0u;
+  c.pad = 0u;
   return c;
 }
 

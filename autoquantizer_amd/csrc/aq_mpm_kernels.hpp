@@ -66,23 +66,45 @@ template <int ST, int NW32> struct RowsOf {
   static constexpr int RS = ST == AQ_STORE_QUANT ? NW32 + 1 : 1;
 };
 
-// Load particle p.  Quantized: the words in `mask` go through the thread's
-// LDS row and the members are decoded from it (with G2P_ONLY, only x and
-// J/F, the members G2P reads).
+// Raw words of a quantized record fetched ahead of their decode.
+template <int ST, int NW32> struct Fetched {
+  uint64_t w[ST == AQ_STORE_QUANT ? NW32 / 2 : 1];
+};
+
+// Global loads of particle p: the record words in `mask` (quantized) or the
+// value planes (plain; with G2P_ONLY only x and J/F, the members G2P reads).
 template <int DIM, int ST, int NW32, int NC, class T, bool G2P_ONLY = false>
-__device__ __forceinline__ void load_state(const StateK &s, const CodecK &cod, int64_t p, uint32_t mask,
-                                           uint32_t *row, T (&v)[NC]) {
+__device__ __forceinline__ void fetch_state(const StateK &s, int64_t p, uint32_t mask, Fetched<ST, NW32> &f,
+                                            T (&v)[NC]) {
+  constexpr int CJ = 2 * DIM + DIM * DIM;
   if constexpr (ST == AQ_STORE_QUANT) {
     const uint64_t *w = (const uint64_t *)s.ptr;
 #pragma unroll
     for (int i = 0; i < NW32 / 2; ++i) {
+      f.w[i] = 0ull;
+      if (i < s.nw && ((mask >> i) & 1u)) f.w[i] = __builtin_nontemporal_load(w + (int64_t)i * s.cap + p);
+    }
+  } else {
+    const T *vals = (const T *)s.ptr;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = (G2P_ONLY && c >= DIM && c < CJ) ? (T)0 : vals[(int64_t)c * s.cap + p];
+  }
+}
+
+// Decode of fetched words through the thread's LDS row (quantized; plain
+// values are already in v).
+template <int DIM, int ST, int NW32, int NC, class T, bool G2P_ONLY = false>
+__device__ __forceinline__ void finish_state(const StateK &s, const CodecK &cod, uint32_t mask,
+                                             const Fetched<ST, NW32> &f, uint32_t *row, T (&v)[NC]) {
+  if constexpr (ST == AQ_STORE_QUANT) {
+    constexpr int CJ = 2 * DIM + DIM * DIM;
+#pragma unroll
+    for (int i = 0; i < NW32 / 2; ++i) {
       if (i < s.nw && ((mask >> i) & 1u)) {
-        const uint64_t x = __builtin_nontemporal_load(w + (int64_t)i * s.cap + p);
-        row[2 * i] = (uint32_t)x;
-        row[2 * i + 1] = (uint32_t)(x >> 32);
+        row[2 * i] = (uint32_t)f.w[i];
+        row[2 * i + 1] = (uint32_t)(f.w[i] >> 32);
       }
     }
-    constexpr int CJ = 2 * DIM + DIM * DIM;
     ConstU32 *table = comp_table(cod);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -93,11 +115,16 @@ __device__ __forceinline__ void load_state(const StateK &s, const CodecK &cod, i
       if (!G2P_ONLY && c % COMP_BATCH == 0) table = comp_table(cod);
       v[c] = (T)decode_row(row, comp_load(table, c));
     }
-  } else {
-    const T *vals = (const T *)s.ptr;
-#pragma unroll
-    for (int c = 0; c < NC; ++c) v[c] = vals[(int64_t)c * s.cap + p];
   }
+}
+
+// Load particle p (fetch + decode).
+template <int DIM, int ST, int NW32, int NC, class T, bool G2P_ONLY = false>
+__device__ __forceinline__ void load_state(const StateK &s, const CodecK &cod, int64_t p, uint32_t mask,
+                                           uint32_t *row, T (&v)[NC]) {
+  Fetched<ST, NW32> f;
+  fetch_state<DIM, ST, NW32, NC, T, G2P_ONLY>(s, p, mask, f, v);
+  finish_state<DIM, ST, NW32, NC, T, G2P_ONLY>(s, cod, mask, f, row, v);
 }
 
 // Store particle p.  Quantized records are encoded in the thread's LDS row;
@@ -110,15 +137,22 @@ __device__ __forceinline__ void store_state(const StateK &s, const CodecK &cod, 
     float f[NC], dec[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) f[c] = (float)v[c];
-#pragma unroll
-    for (int i = 0; i < NW32; ++i) row[i] = 0u;
-    encode_record<NC>(f, cod, noise_slot(s, p), row, dec);
     uint64_t *w = (uint64_t *)s.ptr;
+    if (cod.stream) {
+      StreamSink sink{w, s.cap, p, (uint32_t)s.nw, 0u, 0ull};
+      encode_record<NC>(f, cod, noise_slot(s, p), sink, dec);
+      sink.finish();
+    } else {
 #pragma unroll
-    for (int i = 0; i < NW32 / 2; ++i)
-      if (i < s.nw)
-        __builtin_nontemporal_store((uint64_t)row[2 * i] | ((uint64_t)row[2 * i + 1] << 32),
-                                    w + (int64_t)i * s.cap + p);
+      for (int i = 0; i < NW32; ++i) row[i] = 0u;
+      RowSink sink{row};
+      encode_record<NC>(f, cod, noise_slot(s, p), sink, dec);
+#pragma unroll
+      for (int i = 0; i < NW32 / 2; ++i)
+        if (i < s.nw)
+          __builtin_nontemporal_store((uint64_t)row[2 * i] | ((uint64_t)row[2 * i + 1] << 32),
+                                      w + (int64_t)i * s.cap + p);
+    }
 #pragma unroll
     for (int c = 0; c < NC; ++c) v[c] = (T)dec[c];
   } else {

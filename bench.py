@@ -198,7 +198,8 @@ def main():
     value = n * K / q["elapsed"]
     fp32_value = n * K / f["elapsed"]
 
-    # roofline of the dominant kernel: the fused G2P->P2G launch
+    # roofline of the dominant kernel: the fused G2P->P2G launch (HBM traffic
+    # per launch is measured separately with rocprofv3 FETCH_SIZE/WRITE_SIZE)
     avg_s = q["ms"][0] / 1e3 / max(q["launches"][0], 1)
     per_particle = q["g2p_words"] * 8 + q["nwords"] * 8  # words G2P reads + whole record written
     per_node = 32  # grid velocity read once (16 B) + accumulated momentum/mass written once (16 B)
@@ -231,7 +232,7 @@ def main():
                       "first_p2g_last_g2p": q["ms"][3], "launches": q["launches"]},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "k_g2p2g<3,fluid,quant>", "kernel_us": avg_s * 1e6,
+                     "kernel": "k_tiled<3,fluid,quant,G2P+P2G> (fused substep kernel)", "kernel_us": avg_s * 1e6,
                      "algorithmic_bytes": alg_bytes, "active_nodes": q["act"],
                      "bytes_per_particle": per_particle, "bytes_per_node": per_node},
         "scheme_search_s": None,

@@ -173,6 +173,49 @@ def test_g2p_codec_bit_exact(orc, kind, tiled):
     assert np.array_equal(sim_q.get_words(original_order=True), words_orc)
 
 
+def test_g2p_codec_bit_exact_any_field_order(orc):
+    """A record whose fields are not in component order (J first, then C's
+    shared group, v, x) goes through the general encoder: still bit-exact."""
+    _torch()
+    import autoquantizer_amd as aq
+    from autoquantizer_amd import layouts
+    names, t, g, sh = layouts.fluid_3d()
+    order = ["J"] + [f"C{a}{b}" for a in range(3) for b in range(3)] + [f"v{a}" for a in range(3)] + \
+            [f"x{a}" for a in range(3)]
+    sims = []
+    for storage in ("quant", "f32"):
+        cfg = aq.MPMConfig()
+        cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.la = 3, True, 64, 2e-4, 400.0
+        cfg.p_vol = (1 / 64 * 0.5) ** 3
+        cfg.n_particles = 3000
+        cfg.seed = 11
+        if storage == "quant":
+            sims.append(aq.MPMSolver(cfg, types=t, groups=[("pack", 0, order)], shared=sh))
+        else:
+            cfg.compute_fp = "f32"
+            sims.append(aq.MPMSolver(cfg))
+    sim_q, sim_f = sims
+    assert sim_q.comps[15]["bit_offset"] < sim_q.comps[0]["bit_offset"]
+    rng = np.random.default_rng(12)
+    vals = np.zeros((16, 3000))
+    vals[:3] = rng.uniform(0.1, 0.5, (3, 3000))
+    vals[3:6] = rng.uniform(-1, 1, (3, 3000))
+    vals[6:15] = rng.uniform(-0.5, 0.5, (9, 3000))
+    vals[15] = rng.uniform(0.99, 1.01, 3000)
+    sim_q.set_state(vals)
+    sim_f.set_state(sim_q.get_state(original_order=True))
+    sim_q.build_pid()
+    sim_f.build_pid()
+    sim_q.p2g_bls()
+    sim_q.grid_op()
+    sim_f.load_grid(sim_q.grid_velocity())
+    sim_q.g2p_bls()
+    sim_f.g2p_bls()
+    out = sim_f.get_state(original_order=True).astype(np.float32)
+    words_orc = orc.pack(oracle_comps(sim_q), sim_q.nwords, out, seed=11, step=0)
+    assert np.array_equal(sim_q.get_words(original_order=True), words_orc)
+
+
 @pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d"])
 def test_grid_op_exact(orc, kind):
     """Grid update on the GPU's P2G grid equals the oracle's grid_op exactly on
