@@ -57,12 +57,14 @@ class AQError(RuntimeError):
     pass
 
 
+AQ_MLP_IN, AQ_MLP_HIDDEN, AQ_MLP_OUT, AQ_MLP_PARAMS = 24, 64, 4, 10180  # include/aq_mpm.h
+
 EXPORTS = [
     "aq_last_error", "aq_version", "aq_layout_plan", "aq_pack", "aq_unpack",
     "aq_store_members", "aq_sim_workspace_bytes", "aq_sim_create", "aq_sim_destroy",
     "aq_substep", "aq_p2g", "aq_grid_op", "aq_g2p", "aq_advance", "aq_sort",
     "aq_update_limits", "aq_energy", "aq_sim_grid", "aq_advance_timed",
-    "aq_adj_workspace_bytes", "aq_adj_substep",
+    "aq_adj_workspace_bytes", "aq_adj_substep", "aq_mlp_forward", "aq_mlp_backward", "aq_adam_step",
 ]
 
 _lib = None
@@ -103,6 +105,10 @@ def lib():
     L.aq_adj_workspace_bytes.restype = i64
     L.aq_adj_workspace_bytes.argtypes = [P]
     L.aq_adj_substep.argtypes = [P, P, i64, P, P, P, P, i64, P]
+    L.aq_mlp_forward.argtypes = [P, P, i32, P, P, P]
+    L.aq_mlp_backward.argtypes = [P, P, P, P, P, i32, P, P, P, P]
+    f32 = ctypes.c_float
+    L.aq_adam_step.argtypes = [P, P, P, P, i32, i32, f32, f32, f32, f32, P]
     for name in EXPORTS:
         if name not in ("aq_last_error", "aq_version", "aq_sim_workspace_bytes", "aq_adj_workspace_bytes"):
             getattr(L, name).restype = i32

@@ -9,7 +9,7 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "libaqmpm.so")
-SOURCES = [os.path.join(PKG, "csrc", "aq_mpm.hip")]
+SOURCES = [os.path.join(PKG, "csrc", "aq_mpm.hip"), os.path.join(PKG, "csrc", "aq_mlp.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -35,6 +35,8 @@ static int fail(int code, const std::string &msg) {
 #define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
 
 extern "C" const char *aq_last_error(void) { return g_err.c_str(); }
+// for the other translation units of the library (aq_mlp.hip)
+__attribute__((visibility("hidden"))) int aq_fail(int code, const char *msg) { return fail(code, msg); }
 extern "C" const char *aq_version(void) { return "aqmpm 0.1 gfx950"; }
 
 // ---------------------------------------------------------------------------

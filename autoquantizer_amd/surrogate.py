@@ -1,0 +1,239 @@
+"""Learned surrogate of the scheme search's error-contribution sums.
+
+The reference finds a bit allocation by running the adjoint of the scene
+(run_forward_with_ad_nlogn + get_gradients) before the analytical solve.
+Here a small MLP (24 -> 64 -> 64 -> 64 -> 4, tanh, fp32) predicts the
+per-field sensitivity coefficients log10 of the mean error-contribution sum of
+the field's components from cheap statistics of the scene's initial state;
+it is trained by the library's own HIP kernels (aq_mlp_forward,
+aq_mlp_backward, aq_adam_step) on adjoint runs of the method itself.  The
+adjoint remains the judge and the fallback: search_scheme_surrogate can
+check a predicted allocation against the adjoint's sums and fall back to the
+adjoint allocation when the predicted one misses the error bound.
+
+Scenes are the 2D elastic jelly of the adjoint configuration: fields x, v, C,
+F (12 components).
+"""
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .mpm import _ptr, _stream
+from .solvers import LagrangianEB, modelled_error
+
+N_IN, N_HID, N_OUT = _lib.AQ_MLP_IN, _lib.AQ_MLP_HIDDEN, _lib.AQ_MLP_OUT
+N_PARAM = _lib.AQ_MLP_PARAMS
+FIELDS_2D = (("x", slice(0, 2)), ("v", slice(2, 4)), ("C", slice(4, 8)), ("F", slice(8, 12)))
+
+
+def scene_features(state, n_grid, dt, E):
+    """24 statistics of a 2D elastic state [12, n]: mean/std/min/max of |x|,
+    |v|, |C|, |F - I| (16), log10 particle count, log2 grid resolution,
+    log10 dt, log10 E, mean and std of each position axis (4)."""
+    st = np.asarray(state, dtype=np.float64)
+    n = st.shape[1]
+    f = []
+    for name, sl in FIELDS_2D:
+        a = st[sl]
+        if name == "F":
+            a = a - np.array([1.0, 0.0, 0.0, 1.0])[:, None]
+        a = np.abs(a)
+        f += [a.mean(), a.std(), a.min(), a.max()]
+    f += [np.log10(n), np.log2(n_grid), np.log10(dt), np.log10(E)]
+    f += [st[0].mean(), st[0].std(), st[1].mean(), st[1].std()]
+    return np.array(f, dtype=np.float64)
+
+
+def field_targets(g):
+    """log10 of the mean error-contribution sum of each field's components."""
+    g = np.asarray(g, dtype=np.float64)
+    return np.array([np.log10(max(g[sl].mean(), 1e-300)) for _, sl in FIELDS_2D])
+
+
+def component_sums(pred):
+    """Per-component sums [12] from per-field predictions [4] (log10)."""
+    out = np.zeros(12)
+    for (_, sl), v in zip(FIELDS_2D, pred):
+        out[sl] = 10.0 ** v
+    return out
+
+
+class Surrogate:
+    """The MLP with input/output standardisation; parameters on the GPU."""
+
+    def __init__(self, seed=0, device="cuda"):
+        self.device = device
+        rng = np.random.default_rng(seed)
+        shapes = [(N_HID, N_IN), (N_HID,), (N_HID, N_HID), (N_HID,), (N_HID, N_HID), (N_HID,), (N_OUT, N_HID),
+                  (N_OUT,)]
+        parts = []
+        for s in shapes:
+            if len(s) == 2:  # Glorot uniform
+                lim = np.sqrt(6.0 / (s[0] + s[1]))
+                parts.append(rng.uniform(-lim, lim, s).ravel())
+            else:
+                parts.append(np.zeros(s))
+        self.params = torch.as_tensor(np.concatenate(parts), dtype=torch.float32, device=device)
+        assert self.params.numel() == N_PARAM
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.t = 0
+        self.x_mean = np.zeros(N_IN)
+        self.x_std = np.ones(N_IN)
+        self.y_mean = np.zeros(N_OUT)
+        self.y_std = np.ones(N_OUT)
+        # added to the prediction (log10) before the allocation: a quantile of
+        # held-out residuals, so that under-predicted sums rarely cost the bound
+        self.margin = np.zeros(N_OUT)
+
+    # ---- raw kernels ------------------------------------------------------
+    def _forward(self, xn):
+        n = xn.shape[0]
+        act = torch.empty((n, 3 * N_HID), dtype=torch.float32, device=self.device)
+        y = torch.empty((n, N_OUT), dtype=torch.float32, device=self.device)
+        _lib.check(_lib.lib().aq_mlp_forward(_ptr(self.params), _ptr(xn), n, _ptr(act), _ptr(y), _stream()))
+        return y, act
+
+    def loss_and_grad(self, xn, yn):
+        """Mean squared error on standardised data and its gradient (HIP)."""
+        y, act = self._forward(xn)
+        n = xn.shape[0]
+        delta = torch.empty((n, 3 * N_HID + N_OUT), dtype=torch.float32, device=self.device)
+        grad = torch.empty(N_PARAM, dtype=torch.float32, device=self.device)
+        loss = torch.zeros(1, dtype=torch.float64, device=self.device)
+        _lib.check(_lib.lib().aq_mlp_backward(_ptr(self.params), _ptr(xn), _ptr(act), _ptr(y), _ptr(yn), n,
+                                              _ptr(delta), _ptr(grad), _ptr(loss), _stream()))
+        return float(loss.item()) / (n * N_OUT), grad
+
+    def adam_step(self, grad, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.t += 1
+        _lib.check(_lib.lib().aq_adam_step(_ptr(self.params), _ptr(grad), _ptr(self.m), _ptr(self.v), N_PARAM,
+                                           self.t, lr, beta1, beta2, eps, _stream()))
+
+    # ---- training and inference -------------------------------------------
+    def fit(self, X, Y, steps=2000, lr=3e-3, lr_final=3e-4):
+        """Full-batch Adam on standardised features/targets; returns the loss
+        history (every step)."""
+        X = np.asarray(X, dtype=np.float64)
+        Y = np.asarray(Y, dtype=np.float64)
+        self.x_mean, self.x_std = X.mean(0), X.std(0) + 1e-12
+        self.y_mean, self.y_std = Y.mean(0), Y.std(0) + 1e-12
+        xn = torch.as_tensor((X - self.x_mean) / self.x_std, dtype=torch.float32, device=self.device)
+        yn = torch.as_tensor((Y - self.y_mean) / self.y_std, dtype=torch.float32, device=self.device)
+        hist = []
+        for k in range(steps):
+            lr_k = lr * (lr_final / lr) ** (k / max(steps - 1, 1))
+            L, g = self.loss_and_grad(xn, yn)
+            self.adam_step(g, lr_k)
+            hist.append(L)
+        return np.array(hist)
+
+    def predict(self, X):
+        X = np.atleast_2d(np.asarray(X, dtype=np.float64))
+        xn = torch.as_tensor((X - self.x_mean) / self.x_std, dtype=torch.float32, device=self.device)
+        y, _ = self._forward(xn)
+        return y.double().cpu().numpy() * self.y_std + self.y_mean
+
+    def save(self, path):
+        np.savez(path, params=self.params.cpu().numpy(), x_mean=self.x_mean, x_std=self.x_std,
+                 y_mean=self.y_mean, y_std=self.y_std, margin=self.margin)
+
+    @classmethod
+    def load(cls, path, device="cuda"):
+        d = np.load(path, allow_pickle=False)
+        s = cls(device=device)
+        s.params = torch.as_tensor(d["params"], dtype=torch.float32, device=device)
+        s.x_mean, s.x_std, s.y_mean, s.y_std = d["x_mean"], d["x_std"], d["y_mean"], d["y_std"]
+        if "margin" in d:
+            s.margin = d["margin"]
+        return s
+
+
+# ---- scenes of the adjoint configuration -----------------------------------
+
+def random_scene(rng, n, n_grid):
+    """A 2D elastic block: center, size in [0.1, 0.3], |v0| <= 3, E log-uniform
+    in [5e3, 5e4]; dt at a CFL number of 0.3 of the elastic wave speed."""
+    size = rng.uniform(0.1, 0.3)
+    lo = 0.05 + size / 2
+    cx, cy = rng.uniform(lo, 1 - lo, 2)
+    speed = 3.0 * np.sqrt(rng.uniform())
+    ang = rng.uniform(0, 2 * np.pi)
+    E = float(10 ** rng.uniform(np.log10(5e3), np.log10(5e4)))
+    dt = float(0.3 / n_grid / np.sqrt(E))
+    st = np.zeros((12, n))
+    st[0] = rng.uniform(cx - size / 2, cx + size / 2, n)
+    st[1] = rng.uniform(cy - size / 2, cy + size / 2, n)
+    st[2] = speed * np.cos(ang)
+    st[3] = speed * np.sin(ang)
+    st[8] = st[11] = 1.0
+    return st, E, dt
+
+
+def make_solver(n, n_grid, E, dt, device="cuda"):
+    from . import MPMConfig, MPMSolver
+    cfg = MPMConfig()
+    cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.E, cfg.nu, cfg.sticky = 2, False, n_grid, dt, E, 0.2, True
+    cfg.p_vol = (1 / n_grid * 0.5) ** 2
+    cfg.n_particles = n
+    cfg.compute_fp = "f32"
+    return MPMSolver(cfg, device=device)
+
+
+def adjoint_sample(st, E, dt, n_grid, nsteps, checkpoint_every=16):
+    """(features, per-component error-contribution sums, ranges) of one scene."""
+    from .adjoint import error_contributions
+    from .search import detect_ranges, elastic_2d_ranges
+    n = st.shape[1]
+    sim = make_solver(n, n_grid, E, dt)
+    sim.set_state(st)
+    ranges = elastic_2d_ranges(detect_ranges(sim, nsteps), 2)
+    sim.set_state(st)
+    _, _, g = error_contributions(sim, nsteps, checkpoint_every=checkpoint_every)
+    return scene_features(st, n_grid, dt, E), g, ranges
+
+
+def make_dataset(nscenes, seed, n, n_grid, nsteps, checkpoint_every=16, log=None):
+    """Features [N, 24], per-component sums [N, 12], ranges [N, 12] and the
+    scene parameters of nscenes seeded random scenes."""
+    rng = np.random.default_rng(seed)
+    X, G, R, P = [], [], [], []
+    for k in range(nscenes):
+        st, E, dt = random_scene(rng, n, n_grid)
+        x, g, r = adjoint_sample(st, E, dt, n_grid, nsteps, checkpoint_every)
+        X.append(x)
+        G.append(g)
+        R.append(r)
+        P.append((E, dt))
+        if log and (k + 1) % 16 == 0:
+            log(f"dataset: {k + 1}/{nscenes} scenes")
+    return np.array(X), np.array(G), np.array(R), np.array(P)
+
+
+def search_scheme_surrogate(model, st, E, dt, n_grid, nsteps, ref, tol_rel, judge_g=None):
+    """Bit allocation from the surrogate: features + MLP + range run +
+    LagrangianEB.  With judge_g (the adjoint's per-component sums) the
+    allocation is checked against the bound under the adjoint's sums and
+    replaced by the adjoint allocation when it misses.  Returns dict(bits,
+    ranges, seconds, met, fell_back)."""
+    from .search import detect_ranges, elastic_2d_ranges
+    t0 = time.perf_counter()
+    n = st.shape[1]
+    pred = model.predict(scene_features(st, n_grid, dt, E))[0] + model.margin
+    sim = make_solver(n, n_grid, E, dt)
+    sim.set_state(st)
+    ranges = elastic_2d_ranges(detect_ranges(sim, nsteps), 2)
+    bits, _ = LagrangianEB(component_sums(pred) * ranges ** 2, ref=ref, tol_rel=tol_rel)
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - t0
+    met, fell_back = None, False
+    if judge_g is not None:
+        G = np.asarray(judge_g) * ranges ** 2
+        met = modelled_error(G, bits) <= (ref * tol_rel) ** 2
+        if not met:
+            bits, _ = LagrangianEB(G, ref=ref, tol_rel=tol_rel)
+            fell_back = True
+    return dict(bits=bits + 1, ranges=ranges, seconds=seconds, met=met, fell_back=fell_back, pred=pred)

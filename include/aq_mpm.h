@@ -231,6 +231,27 @@ int64_t aq_adj_workspace_bytes(const aq_sim *sim);
 int aq_adj_substep(aq_sim *sim, const void *state, int64_t n, const void *adj_next, void *adj_out,
                    double *gsum, void *workspace, int64_t workspace_bytes, void *stream);
 
+/* ---- surrogate MLP of the scheme search: training and inference kernels
+   (no counterpart in the reference, whose search always runs the adjoint) ---
+   in -> 64 (tanh) -> 64 (tanh) -> 64 (tanh) -> out; parameters one flat fp32
+   array W1[64][in], b1, W2[64][64], b2, W3[64][64], b3, W4[out][64], b4. */
+#define AQ_MLP_IN 24
+#define AQ_MLP_HIDDEN 64
+#define AQ_MLP_OUT 4
+#define AQ_MLP_PARAMS 10180
+/* y[s][out] for x[s][in]; act [s][3 * 64] receives the hidden activations
+   (h1, h2, h3) that aq_mlp_backward uses. */
+int aq_mlp_forward(const float *params, const float *x, int32_t nsamples, float *act, float *y, void *stream);
+/* Gradient of mean((y - target)^2) over samples and outputs with respect to
+   the parameters, into grad[AQ_MLP_PARAMS]; delta: scratch [s][3 * 64 + out];
+   loss (NULL or device double): the summed squared error is added to it. */
+int aq_mlp_backward(const float *params, const float *x, const float *act, const float *y,
+                    const float *target, int32_t nsamples, float *delta, float *grad, double *loss,
+                    void *stream);
+/* One Adam step (bias-corrected, step t >= 1) on n parameters. */
+int aq_adam_step(float *params, const float *grad, float *m, float *v, int32_t n, int32_t t, float lr,
+                 float beta1, float beta2, float eps, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

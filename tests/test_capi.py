@@ -117,3 +117,13 @@ def test_sim_desc_validation():
     assert L.aq_sim_workspace_bytes(ctypes.byref(d)) < 0
     d.n_grid, d.capacity = 64, 1000
     assert L.aq_sim_workspace_bytes(ctypes.byref(d)) > 64 ** 3 * 16 * 2
+
+
+def test_header_constants_match_binding():
+    from autoquantizer_amd import _lib
+    txt = open(os.path.join(ROOT, "include", "aq_mpm.h")).read()
+    defs = dict(re.findall(r"#define (AQ_[A-Z_0-9]+) \(?(-?\d+)\)?", txt))
+    for name in ("AQ_MAX_COMPS", "AQ_MAX_WORDS", "AQ_MLP_IN", "AQ_MLP_HIDDEN", "AQ_MLP_OUT", "AQ_MLP_PARAMS"):
+        assert int(defs[name]) == getattr(_lib, name), name
+    h, i, o = _lib.AQ_MLP_HIDDEN, _lib.AQ_MLP_IN, _lib.AQ_MLP_OUT
+    assert _lib.AQ_MLP_PARAMS == h * i + h + 2 * (h * h + h) + o * h + o
