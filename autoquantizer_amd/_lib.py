@@ -64,7 +64,8 @@ EXPORTS = [
     "aq_store_members", "aq_sim_workspace_bytes", "aq_sim_create", "aq_sim_destroy",
     "aq_substep", "aq_p2g", "aq_grid_op", "aq_g2p", "aq_advance", "aq_sort",
     "aq_update_limits", "aq_energy", "aq_sim_grid", "aq_advance_timed",
-    "aq_adj_workspace_bytes", "aq_adj_substep", "aq_mlp_forward", "aq_mlp_backward", "aq_adam_step",
+    "aq_adj_workspace_bytes", "aq_adj_substep", "aq_noise_add", "aq_noise_dot", "aq_mlp_forward",
+    "aq_mlp_backward", "aq_adam_step",
 ]
 
 _lib = None
@@ -105,9 +106,11 @@ def lib():
     L.aq_adj_workspace_bytes.restype = i64
     L.aq_adj_workspace_bytes.argtypes = [P]
     L.aq_adj_substep.argtypes = [P, P, i64, P, P, P, P, i64, P]
+    L.aq_noise_add.argtypes = [P, P, i64, P, u64, u32, P]
+    L.aq_noise_dot.argtypes = [P, P, i64, u64, u32, P, P]
     L.aq_mlp_forward.argtypes = [P, P, i32, P, P, P]
-    L.aq_mlp_backward.argtypes = [P, P, P, P, P, i32, P, P, P, P]
     f32 = ctypes.c_float
+    L.aq_mlp_backward.argtypes = [P, P, P, P, P, i32, f32, P, P, P, P]
     L.aq_adam_step.argtypes = [P, P, P, P, i32, i32, f32, f32, f32, f32, P]
     for name in EXPORTS:
         if name not in ("aq_last_error", "aq_version", "aq_sim_workspace_bytes", "aq_adj_workspace_bytes"):

@@ -61,49 +61,84 @@ def _advance_plain(solver, state, nsteps):
                                      _stream()))
 
 
+class Noise:
+    """Modelled rounding noise: after substep t (1-based) the state gets
+    sigma[c] * xi(seed, t, c, p), xi uniform in [-1/2, 1/2) (aq_noise_add)."""
+
+    def __init__(self, sigma, seed=0):
+        self.sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+        self.seed = int(seed)
+
+    def add(self, solver, state, step):
+        _lib.check(_lib.lib().aq_noise_add(solver._h, _ptr(state), solver.current_idx,
+                                           self.sigma.ctypes.data_as(ctypes.c_void_p), self.seed, int(step),
+                                           _stream()))
+
+    def dot(self, solver, adj, step, out):
+        _lib.check(_lib.lib().aq_noise_dot(solver._h, _ptr(adj), solver.current_idx, self.seed, int(step), _ptr(out),
+                                           _stream()))
+
+
 class _Schedule:
-    def __init__(self, solver, nsteps, checkpoint_every):
+    def __init__(self, solver, nsteps, checkpoint_every, noise=None):
         if checkpoint_every < 1:
             raise ValueError("checkpoint_every must be >= 1")
+        if noise is not None and len(noise.sigma) != solver.ncomp:
+            raise ValueError("sigma needs one entry per component")
         self.solver, self.nsteps, self.k = solver, int(nsteps), int(checkpoint_every)
+        self.noise = noise
         self.checkpoints = []
+
+    def _advance(self, s, t0, nsteps):
+        """nsteps substeps of s, the first producing state t0 + 1."""
+        if self.noise is None:
+            _advance_plain(self.solver, s, nsteps)
+            return
+        for t in range(t0 + 1, t0 + nsteps + 1):
+            _advance_plain(self.solver, s, 1)
+            self.noise.add(self.solver, s, t)
 
     def forward(self, state0):
         """Runs the forward from state0 ([ncomp, capacity]); returns s_T."""
         s = state0.clone()
         for t0 in range(0, self.nsteps, self.k):
             self.checkpoints.append(s.clone())
-            _advance_plain(self.solver, s, min(self.k, self.nsteps - t0))
+            self._advance(s, t0, min(self.k, self.nsteps - t0))
         return s
 
-    def backward(self, adj_T, gsum=None, adj=None):
+    def backward(self, adj_T, gsum=None, adj=None, noise_dot=None):
         """dL/ds_0 from dL/ds_T, adding the squared adjoints of states
-        1..T-1 into gsum."""
+        1..T-1 into gsum and, with noise, sum_p dL/ds_t xi_t (t = 1..T) into
+        noise_dot (dL/dsigma)."""
         a = adj_T.clone()
         b = torch.empty_like(a)
         for j in reversed(range(len(self.checkpoints))):
             t0 = j * self.k
             t1 = min(t0 + self.k, self.nsteps)
             states = [self.checkpoints[j]]
-            for _ in range(t0 + 1, t1):
+            for t in range(t0 + 1, t1):
                 s = states[-1].clone()
-                _advance_plain(self.solver, s, 1)
+                self._advance(s, t - 1, 1)
                 states.append(s)
             for t in reversed(range(t0, t1)):
+                if noise_dot is not None:
+                    self.noise.dot(self.solver, a, t + 1, noise_dot)
                 adj.step(states[t - t0], a, b, gsum if t >= 1 else None)
                 a, b = b, a
         return a
 
 
 class Rollout(torch.autograd.Function):
-    """state_T = Rollout.apply(state_0, solver, nsteps, checkpoint_every):
-    nsteps plain substeps as a differentiable torch op.  backward runs the
-    hand-written adjoint; the error-contribution sums of the backward pass
-    are left in solver.adjoint_gsum (float64 [ncomp])."""
+    """state_T = Rollout.apply(state_0, solver, nsteps, checkpoint_every, noise):
+    nsteps plain substeps (with a Noise after each when noise is given) as a
+    differentiable torch op.  backward runs the hand-written adjoint; the
+    error-contribution sums of the backward pass are left in
+    solver.adjoint_gsum and, with noise, dL/dsigma in solver.adjoint_dsigma
+    (float64 [ncomp] on the device)."""
 
     @staticmethod
-    def forward(ctx, state0, solver, nsteps, checkpoint_every=16):
-        sched = _Schedule(solver, nsteps, checkpoint_every)
+    def forward(ctx, state0, solver, nsteps, checkpoint_every=16, noise=None):
+        sched = _Schedule(solver, nsteps, checkpoint_every, noise)
         out = sched.forward(state0.detach())
         ctx.sched = sched
         return out
@@ -114,9 +149,11 @@ class Rollout(torch.autograd.Function):
         solver = sched.solver
         adj = SubstepAdjoint(solver)
         gsum = torch.zeros(solver.ncomp, dtype=torch.float64, device=solver.device)
-        grad0 = sched.backward(grad_T.contiguous().to(solver.state.dtype), gsum, adj)
+        dsig = None if sched.noise is None else torch.zeros_like(gsum)
+        grad0 = sched.backward(grad_T.contiguous().to(solver.state.dtype), gsum, adj, dsig)
         solver.adjoint_gsum = gsum
-        return grad0, None, None, None
+        solver.adjoint_dsigma = dsig
+        return grad0, None, None, None, None
 
 
 def error_contributions(solver, nsteps, loss=energy_loss, checkpoint_every=16):
@@ -133,3 +170,41 @@ def error_contributions(solver, nsteps, loss=energy_loss, checkpoint_every=16):
     torch.cuda.current_stream().synchronize()
     g = solver.adjoint_gsum.cpu().numpy()
     return float(L.detach().cpu()), s0.grad[:, :n].double().cpu().numpy(), g
+
+
+def final_position_error(solver, state, xref):
+    """sum_p |x_T,p - xref_p|^2 (f64) of a [ncomp, >= n] state against the
+    [dim, n] positions of an unquantized FP64 run."""
+    n, d = solver.current_idx, solver.dim
+    e = state[:d, :n].double() - xref
+    return (e * e).sum()
+
+
+def noise_scale_gradients(solver, nsteps, sigma, xref, seed=0, checkpoint_every=16):
+    """The adjoint error-gradient run: nsteps substeps from the solver's
+    current state with modelled rounding noise sigma[c] * xi after every
+    substep, L = |x_T - xref|^2, and dL/dsigma[c] = sum_t sum_p
+    dL/ds_t[c,p] xi_t[c,p] from the hand-written adjoint (products and sums
+    in f64).  Returns (L, dL/dsigma as numpy [ncomp]).  The solver's state
+    is left unchanged."""
+    xr = torch.as_tensor(np.asarray(xref), dtype=torch.float64, device=solver.device)
+    s0 = solver.state.clone().requires_grad_(True)
+    sT = Rollout.apply(s0, solver, nsteps, checkpoint_every, Noise(sigma, seed))
+    L = final_position_error(solver, sT, xr)
+    L.backward()
+    torch.cuda.current_stream().synchronize()
+    return float(L.detach().cpu()), solver.adjoint_dsigma.cpu().numpy()
+
+
+def field_gradients(dsigma, sigma, fields):
+    """Per-field derivatives dL/ds_f at s_f = 1 when every component c of
+    field f has noise width s_f * sigma[c]: sum_{c in f} sigma[c] dL/dsigma[c].
+    fields: sequence of (name, slice)."""
+    ds = np.asarray(dsigma, dtype=np.float64) * np.asarray(sigma, dtype=np.float64)
+    return {name: float(ds[sl].sum()) for name, sl in fields}
+
+
+def noise_widths(bits, ranges):
+    """sigma[c] = the quantization step of a signed fixed-point component:
+    range / 2^(bits - 1) (sign bit included in bits)."""
+    return np.asarray(ranges, dtype=np.float64) * np.exp2(1.0 - np.asarray(bits, dtype=np.float64))

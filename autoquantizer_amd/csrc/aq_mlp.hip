@@ -11,6 +11,7 @@
 // outputs, mean((y - t)^2).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <string>
 
 #include "aq_mpm.h"
@@ -76,10 +77,11 @@ __global__ void __launch_bounds__(256) k_dense_bwd(const float *__restrict__ W, 
 }
 
 // Parameter gradients: one thread per parameter, summing over samples in a
-// fixed order: dW_l[o][i] = sum_s delta_l[s][o] in_l[s][i], db_l[o] = sum_s delta_l[s][o].
-__global__ void __launch_bounds__(256) k_mlp_grads(const float *__restrict__ X, const float *__restrict__ act,
-                                                   const float *__restrict__ delta, int32_t nsamples,
-                                                   float *__restrict__ grad) {
+// fixed order: dW_l[o][i] = sum_s delta_l[s][o] in_l[s][i] + 2 wd W_l[o][i],
+// db_l[o] = sum_s delta_l[s][o].
+__global__ void __launch_bounds__(256) k_mlp_grads(const float *__restrict__ prm, const float *__restrict__ X,
+                                                   const float *__restrict__ act, const float *__restrict__ delta,
+                                                   int32_t nsamples, float wd, float *__restrict__ grad) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= NPARAM) return;
   // layer: delta column offset, input array, its row stride and column offset
@@ -100,7 +102,7 @@ __global__ void __launch_bounds__(256) k_mlp_grads(const float *__restrict__ X, 
     const float d = delta[(int64_t)s * (3 * H + OUT) + dofs + o];
     acc = i < 0 ? acc + d : fmaf(d, in[(int64_t)s * stride + col + i], acc);
   }
-  grad[q] = acc;
+  grad[q] = i < 0 ? acc : fmaf(2.0f * wd, prm[q], acc);
 }
 
 // Adam (Kingma & Ba 2015) with bias correction; t is the 1-based step.
@@ -147,9 +149,11 @@ extern "C" int aq_mlp_forward(const float *params, const float *x, int32_t nsamp
 }
 
 extern "C" int aq_mlp_backward(const float *params, const float *x, const float *act, const float *y,
-                               const float *target, int32_t nsamples, float *delta, float *grad, double *loss,
-                               void *stream) {
+                               const float *target, int32_t nsamples, float weight_decay, float *delta,
+                               float *grad, double *loss, void *stream) {
   if (nsamples < 1) return aq_fail(AQ_ERR_INVALID, "no samples");
+  if (!(weight_decay >= 0.0f) || !std::isfinite(weight_decay))
+    return aq_fail(AQ_ERR_INVALID, "weight decay must be finite and >= 0");
   if (!params || !x || !act || !y || !target || !delta || !grad) return aq_fail(AQ_ERR_INVALID, "null buffer");
   hipStream_t st = (hipStream_t)stream;
   const int64_t nh = (int64_t)nsamples * H;
@@ -161,7 +165,8 @@ extern "C" int aq_mlp_backward(const float *params, const float *x, const float 
                      delta + H, H, nsamples);
   hipLaunchKernelGGL(k_dense_bwd, dim3(nblk(nh)), dim3(256), 0, st, params + OW2, delta + H, H, act, delta, H,
                      nsamples);
-  hipLaunchKernelGGL(k_mlp_grads, dim3(nblk(NPARAM)), dim3(256), 0, st, x, act, delta, nsamples, grad);
+  hipLaunchKernelGGL(k_mlp_grads, dim3(nblk(NPARAM)), dim3(256), 0, st, params, x, act, delta, nsamples,
+                     weight_decay, grad);
   MLP_TRY(hipGetLastError());
   return AQ_OK;
 }

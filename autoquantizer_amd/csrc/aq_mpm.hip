@@ -1811,6 +1811,88 @@ extern "C" int aq_energy(aq_sim *s, const void *state, int64_t n, double *out, v
   return AQ_OK;
 }
 
+// ---- modelled quantization noise of the adjoint configuration ------------
+// xi(step, c, p) = u - 1/2 with u the dithering noise of (seed, step,
+// component c, particle slot p): uniform in [-1/2, 1/2), 24-bit grid.
+struct SigmaK {
+  double s[AQ_MAX_COMPS];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_noise_add(T *__restrict__ state, int64_t cap, int64_t n, int32_t nc,
+                                                   SigmaK sig, KeysK keys) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  for (int c = 0; c < nc; ++c) {
+    if (sig.s[c] == 0.0) continue;
+    const T xi = (T)(noise_u((uint32_t)p, keys.k[c]) - 0.5f);
+    state[(int64_t)c * cap + p] += (T)sig.s[c] * xi;
+  }
+}
+
+// out[c] += sum_p adj[c][p] xi(step, c, p), products and sums in double.
+template <typename T>
+__global__ void __launch_bounds__(256) k_noise_dot(const T *__restrict__ adj, int64_t cap, int64_t n, int32_t nc,
+                                                   KeysK keys, double *__restrict__ out) {
+  __shared__ double part[4][AQ_MAX_COMPS];
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int c = 0; c < nc; ++c) {
+    double q = 0.0;
+    if (p < n) q = (double)adj[(int64_t)c * cap + p] * (double)(noise_u((uint32_t)p, keys.k[c]) - 0.5f);
+    for (int o = 32; o > 0; o >>= 1) q += __shfl_xor(q, o);
+    if (lane == 0) part[wave][c] = q;
+  }
+  __syncthreads();
+  if (threadIdx.x < nc) {
+    const int c = threadIdx.x;
+    atomicAdd(out + c, ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c]);
+  }
+}
+
+extern "C" int aq_noise_add(aq_sim *s, void *state, int64_t n, const double *sigma, uint64_t seed, uint32_t step,
+                            void *stream) {
+  int rc = check_n(s, state, n);
+  if (rc) return rc;
+  if (s->d.storage == AQ_STORE_QUANT) return fail(AQ_ERR_INVALID, "noise is added to plain (f32/f64) storage");
+  if (!sigma) return fail(AQ_ERR_INVALID, "null sigma");
+  if (n == 0) return AQ_OK;
+  SigmaK sig{};
+  for (int c = 0; c < s->d.ncomp; ++c) {
+    if (!(sigma[c] >= 0.0) || !std::isfinite(sigma[c])) return fail(AQ_ERR_INVALID, "sigma must be finite and >= 0");
+    sig.s[c] = sigma[c];
+  }
+  const KeysK keys = make_keys(seed, step);
+  hipStream_t st = (hipStream_t)stream;
+  if (s->d.storage == AQ_STORE_F64)
+    hipLaunchKernelGGL(k_noise_add<double>, dim3(pblocks(n)), dim3(256), 0, st, (double *)state, s->d.capacity, n,
+                       s->d.ncomp, sig, keys);
+  else
+    hipLaunchKernelGGL(k_noise_add<float>, dim3(pblocks(n)), dim3(256), 0, st, (float *)state, s->d.capacity, n,
+                       s->d.ncomp, sig, keys);
+  LAUNCH_CHECK();
+  return AQ_OK;
+}
+
+extern "C" int aq_noise_dot(aq_sim *s, const void *adj, int64_t n, uint64_t seed, uint32_t step, double *out,
+                            void *stream) {
+  int rc = check_n(s, adj, n);
+  if (rc) return rc;
+  if (s->d.storage == AQ_STORE_QUANT) return fail(AQ_ERR_INVALID, "the adjoint runs on plain (f32/f64) storage");
+  if (!out) return fail(AQ_ERR_INVALID, "null output");
+  if (n == 0) return AQ_OK;
+  const KeysK keys = make_keys(seed, step);
+  hipStream_t st = (hipStream_t)stream;
+  if (s->d.storage == AQ_STORE_F64)
+    hipLaunchKernelGGL(k_noise_dot<double>, dim3(pblocks(n)), dim3(256), 0, st, (const double *)adj, s->d.capacity,
+                       n, s->d.ncomp, keys, out);
+  else
+    hipLaunchKernelGGL(k_noise_dot<float>, dim3(pblocks(n)), dim3(256), 0, st, (const float *)adj, s->d.capacity, n,
+                       s->d.ncomp, keys, out);
+  LAUNCH_CHECK();
+  return AQ_OK;
+}
+
 extern "C" int aq_sim_grid(aq_sim *s, void **grid, int64_t *nodes) {
   if (!s || !grid || !nodes) return fail(AQ_ERR_INVALID, "null argument");
   *grid = s->grid_v;

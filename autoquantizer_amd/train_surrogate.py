@@ -1,18 +1,29 @@
 """Trains the scheme-search surrogate on adjoint runs and evaluates it.
 
     python -m autoquantizer_amd.train_surrogate [--train 512] [--test 64]
-        [--particles 65536] [--grid 256] [--steps 256] [--out autoquantizer_amd/data]
+        [--particles 65536] [--grid 256] [--steps 256] [--out DIR]
 
-Scenes: 2D elastic blocks of the adjoint configuration (random position,
-size 0.1-0.3, |v0| <= 3, E log-uniform in [5e3, 5e4], seeded).  Each scene
-gets a range run and a checkpointed adjoint run (the method's own kernels);
-the MLP is trained by the library's HIP kernels on the features -> per-field
-log10 sensitivity coefficients.  The held-out scenes measure the prediction
-error, how often the predicted allocation meets the error bound under the
-adjoint's sums (with a safety margin calibrated on a validation split of the
-training scenes), and the search time of both paths.  Writes
-surrogate_elastic2d.npz (weights, standardisation, margin) and
-surrogate_elastic2d_eval.json.
+Scenes (surrogate.random_scene): 2D elastic blocks of the adjoint
+configuration, random position, size 0.1-0.3, |v0| <= 3, E log-uniform in
+[5e3, 5e4], seeded.  Each scene gets a range run and a checkpointed adjoint
+run (the method's own kernels); the MLP is trained by the library's HIP
+kernels on features -> per-field log10 error-contribution sums.
+
+Margin: a model trained on 7/8 of the training scenes predicts the remaining
+1/8; the margin is the smallest uniform log10 offset (step 0.05) at which at
+most --miss-target of those validation scenes get an allocation that misses
+the bound under their adjoint sums.  The final model (all training scenes)
+carries that margin.
+
+Held-out evaluation (--test scenes never seen in training): prediction MSE,
+the fraction of allocations meeting the bound under the adjoint's sums (the
+judge), the extra bits per particle over the adjoint allocation, and the
+search time of both paths measured on --timed held-out scenes.  The expected
+search time with fallback is t_surrogate + miss_rate * t_adjoint.
+
+Writes DIR/surrogate_elastic2d.npz (weights, standardisation, margin),
+DIR/surrogate_elastic2d_dataset.npz (features, sums, ranges, E and dt of
+every scene) and DIR/surrogate_elastic2d_eval.json.
 """
 import argparse
 import json
@@ -21,6 +32,19 @@ import sys
 import time
 
 import numpy as np
+
+
+def allocations(pred, G, R, tol_rel, margin=0.0):
+    """(bits from prediction + margin, bits from the adjoint sums, met?) per scene."""
+    from .solvers import LagrangianEB, modelled_error
+    from .surrogate import component_sums
+    out = []
+    for p, g, r in zip(pred, G, R):
+        Gt = g * r ** 2
+        bs, _ = LagrangianEB(component_sums(p + margin) * r ** 2, ref=1.0, tol_rel=tol_rel)
+        ba, _ = LagrangianEB(Gt, ref=1.0, tol_rel=tol_rel)
+        out.append((bs, ba, modelled_error(Gt, bs) <= tol_rel ** 2 * (1 + 1e-12)))
+    return out
 
 
 def main(argv=None):
@@ -33,76 +57,96 @@ def main(argv=None):
     ap.add_argument("--checkpoint-every", type=int, default=16)
     ap.add_argument("--tol-rel", type=float, default=1e-3)
     ap.add_argument("--fit-steps", type=int, default=3000)
+    ap.add_argument("--weight-decay", type=float, default=3e-3)
+    ap.add_argument("--miss-target", type=float, default=0.05)
+    ap.add_argument("--timed", type=int, default=8)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--dataset", default=None, help="reuse a saved dataset instead of running the adjoint")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "data"))
     a = ap.parse_args(argv)
 
     import torch
     from .search import search_scheme
-    from .solvers import LagrangianEB, modelled_error
-    from .surrogate import (Surrogate, component_sums, field_targets, make_dataset, make_solver, random_scene,
-                            search_scheme_surrogate)
+    from .surrogate import Surrogate, field_targets, make_dataset, make_solver, random_scene, search_scheme_surrogate
 
     log = lambda m: print(m, flush=True)  # noqa: E731
+    os.makedirs(a.out, exist_ok=True)
     t0 = time.perf_counter()
-    X, G, R, P = make_dataset(a.train + a.test, seed=2024, n=a.particles, n_grid=a.grid, nsteps=a.steps,
-                              checkpoint_every=a.checkpoint_every, log=log)
-    t_data = time.perf_counter() - t0
+    if a.dataset:
+        d = np.load(a.dataset, allow_pickle=False)
+        X, G, R, P = d["X"], d["G"], d["R"], d["P"]
+    else:
+        X, G, R, P = make_dataset(a.train + a.test, seed=a.seed, n=a.particles, n_grid=a.grid, nsteps=a.steps,
+                                  checkpoint_every=a.checkpoint_every, log=log)
+        np.savez(os.path.join(a.out, "surrogate_elastic2d_dataset.npz"), X=X, G=G, R=R, P=P)
+    t_data = None if a.dataset else time.perf_counter() - t0
+    log(f"dataset: {len(X)} scenes" + ("" if a.dataset else f" in {t_data:.1f} s"))
     Y = np.array([field_targets(g) for g in G])
-    ntr = a.train
+    ntr, nte = a.train, a.test
     nval = max(ntr // 8, 1)
-    # margin: a high quantile of the validation residuals (under-prediction of a sum costs the bound)
+
     m0 = Surrogate(seed=0)
-    m0.fit(X[:ntr - nval], Y[:ntr - nval], steps=a.fit_steps)
-    resid = Y[ntr - nval:ntr] - m0.predict(X[ntr - nval:ntr])
-    margin = np.quantile(resid, 0.9, axis=0)
-    margin = np.maximum(margin, 0.0)
+    m0.fit(X[:ntr - nval], Y[:ntr - nval], steps=a.fit_steps, weight_decay=a.weight_decay)
+    pv = m0.predict(X[ntr - nval:ntr])
+    val_mse = float(np.mean((pv - Y[ntr - nval:ntr]) ** 2))
+    margin = 0.0
+    while margin < 4.0:
+        miss = 1.0 - np.mean([m for _, _, m in allocations(pv, G[ntr - nval:ntr], R[ntr - nval:ntr], a.tol_rel,
+                                                            margin)])
+        if miss <= a.miss_target:
+            break
+        margin = round(margin + 0.05, 2)
+    log(f"margin {margin} (validation miss {miss:.3f})")
+
     model = Surrogate(seed=0)
+    torch.cuda.synchronize()
     t1 = time.perf_counter()
-    hist = model.fit(X[:ntr], Y[:ntr], steps=a.fit_steps)
+    hist = model.fit(X[:ntr], Y[:ntr], steps=a.fit_steps, weight_decay=a.weight_decay)
     torch.cuda.synchronize()
     t_fit = time.perf_counter() - t1
-    pred = model.predict(X[ntr:])
-    mse = float(np.mean((pred - Y[ntr:]) ** 2))
-    base = float(np.mean((Y[:ntr].mean(0) - Y[ntr:]) ** 2))
+    model.margin = np.full(len(model.y_mean), margin)
+    model.save(os.path.join(a.out, "surrogate_elastic2d.npz"))
 
-    # held-out: allocation from the prediction (+ margin) against the adjoint's sums
-    met = met_margin = 0
-    for k in range(a.test):
-        g = G[ntr + k]
-        ranges = R[ntr + k]
-        Gt = g * ranges ** 2
-        ref = 1.0
-        bound = (ref * a.tol_rel) ** 2
-        b0, _ = LagrangianEB(component_sums(pred[k]) * ranges ** 2, ref=ref, tol_rel=a.tol_rel)
-        b1, _ = LagrangianEB(component_sums(pred[k] + margin) * ranges ** 2, ref=ref, tol_rel=a.tol_rel)
-        met += modelled_error(Gt, b0) <= bound
-        met_margin += modelled_error(Gt, b1) <= bound
-    # timing of both paths on a few held-out scenes
-    rng = np.random.default_rng(2024)
+    te = slice(ntr, ntr + nte)
+    pred = model.predict(X[te])
+    mse = float(np.mean((pred - Y[te]) ** 2))
+    base = float(np.mean((Y[:ntr].mean(0) - Y[te]) ** 2))
+    raw = allocations(pred, G[te], R[te], a.tol_rel, 0.0)
+    mar = allocations(pred, G[te], R[te], a.tol_rel, margin)
+    met_raw = float(np.mean([m for _, _, m in raw]))
+    met = float(np.mean([m for _, _, m in mar]))
+    # storage cost: summed fraction bits per particle over the adjoint allocation (a component
+    # allocated <= 0 fraction bits stores none)
+    extra_bits = float(np.mean([int(np.maximum(bs, 0).sum()) - int(np.maximum(ba, 0).sum()) for bs, ba, _ in mar]))
+
+    rng = np.random.default_rng(a.seed)
     for _ in range(ntr):
         random_scene(rng, a.particles, a.grid)
     t_adj, t_sur = [], []
-    model.margin = margin
-    for k in range(min(4, a.test)):
+    for k in range(min(a.timed, nte)):
         st, E, dt = random_scene(rng, a.particles, a.grid)
         torch.cuda.synchronize()
         s0 = time.perf_counter()
         search_scheme(lambda: make_solver(a.particles, a.grid, E, dt), st, a.steps, tol_rel=a.tol_rel, ref=1.0,
                       checkpoint_every=a.checkpoint_every)
+        torch.cuda.synchronize()
         t_adj.append(time.perf_counter() - s0)
-        r = search_scheme_surrogate(model, st, E, dt, a.grid, a.steps, ref=1.0, tol_rel=a.tol_rel)
-        t_sur.append(r["seconds"])
-    os.makedirs(a.out, exist_ok=True)
-    model.save(os.path.join(a.out, "surrogate_elastic2d.npz"))
-    ev = {"scenes": {"train": ntr, "test": a.test, "particles": a.particles, "grid": a.grid, "substeps": a.steps,
-                     "checkpoint_every": a.checkpoint_every},
+        t_sur.append(search_scheme_surrogate(model, st, E, dt, a.grid, a.steps, ref=1.0, tol_rel=a.tol_rel)["seconds"])
+        log(f"timed scene {k}: adjoint {t_adj[-1]:.4f} s, surrogate {t_sur[-1]:.4f} s")
+    ta, ts = float(np.median(t_adj)), float(np.median(t_sur))
+    expected = ts + (1.0 - met) * ta
+    ev = {"scenes": {"train": ntr, "validation_within_train": nval, "test": nte, "particles": a.particles,
+                     "grid": a.grid, "substeps": a.steps, "checkpoint_every": a.checkpoint_every, "seed": a.seed},
           "dataset_seconds": t_data, "fit_seconds": t_fit, "fit_steps": a.fit_steps,
+          "weight_decay": a.weight_decay, "validation_mse_log10": val_mse,
           "loss_first_last": [float(hist[0]), float(hist[-1])],
           "heldout_mse_log10": mse, "heldout_mse_log10_training_mean": base,
-          "bound_met_fraction": met / a.test, "bound_met_fraction_with_margin": met_margin / a.test,
-          "margin_log10": margin.tolist(), "tol_rel": a.tol_rel,
-          "search_seconds_adjoint": float(np.mean(t_adj)), "search_seconds_surrogate": float(np.mean(t_sur)),
-          "search_speedup": float(np.mean(t_adj) / np.mean(t_sur))}
+          "tol_rel": a.tol_rel, "margin_log10": margin,
+          "bound_met_fraction_no_margin": met_raw, "bound_met_fraction": met,
+          "extra_fraction_bits_per_particle": extra_bits,
+          "search_seconds_adjoint": ta, "search_seconds_surrogate": ts,
+          "search_seconds_surrogate_with_fallback": expected,
+          "search_speedup": ta / ts, "search_speedup_with_fallback": ta / expected}
     json.dump(ev, open(os.path.join(a.out, "surrogate_elastic2d_eval.json"), "w"), indent=1)
     print(json.dumps(ev), flush=True)
     return 0

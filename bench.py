@@ -111,23 +111,18 @@ def cpu_baseline(n, n_grid, target_s=10.0):
                       f"with OpenMP, {tk:.1f} s"}
 
 
-def scheme_search(n=65536, n_grid=256, nsteps=256, checkpoint_every=16, seed=0, dt=2.5e-5):
-    """Wall time of one scheme search on the adjoint scene: 2D elastic jelly
-    (fixed-corotated, E=1e4, nu=0.2), n particles uniform in [0.3,0.5]^2 with
-    v=(0,-1), n_grid^2 grid, sticky walls: range run + checkpointed adjoint
-    run (fp32, f64 sums) + FP64 analytical solve at rate 0.5.  dt = 2.5e-5:
-    with E = 1e4 the explicit update diverges at dt = 1e-4 and 5e-5 on this
-    grid (wave speed 100 per unit time, CFL number dt * 100 * n_grid)."""
-    import torch
+def _adjoint_scene(n, n_grid, dt, seed=0):
+    """The adjoint scene: 2D elastic jelly (fixed-corotated, E=1e4, nu=0.2),
+    n particles uniform in [0.3,0.5]^2 with v=(0,-1), n_grid^2 grid, sticky
+    walls.  Returns (make_solver(storage), initial state [12, n])."""
     import autoquantizer_amd as aq
-    from autoquantizer_amd.search import search_scheme
 
-    def make():
+    def make(storage="f32"):
         cfg = aq.MPMConfig()
         cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.E, cfg.nu, cfg.sticky = 2, False, n_grid, dt, 1e4, 0.2, True
         cfg.p_vol = (1 / n_grid * 0.5) ** 2
         cfg.n_particles = n
-        cfg.compute_fp = "f32"
+        cfg.compute_fp = storage
         return aq.MPMSolver(cfg)
 
     rng = np.random.default_rng(seed)
@@ -135,16 +130,103 @@ def scheme_search(n=65536, n_grid=256, nsteps=256, checkpoint_every=16, seed=0, 
     init[:2] = rng.uniform(0.3, 0.5, (2, n))
     init[3] = -1.0
     init[8] = init[11] = 1.0
-    search_scheme(make, init, 8, rate=0.5, checkpoint_every=4)  # warm-up (kernel loading)
+    return make, init
+
+
+def scheme_search(n=65536, n_grid=256, nsteps=256, checkpoint_every=16, dt=2.5e-5, tol_rel=1e-3):
+    """Wall time of one scheme search on the adjoint scene (range run +
+    checkpointed adjoint run in fp32 with f64 sums + FP64 error-bound solve
+    at tol_rel of |x| = 1), and of the surrogate search on the same scene
+    when trained weights are present (features + MLP + range run + solve),
+    judged by the adjoint's sums.  dt = 2.5e-5: with E = 1e4 the explicit
+    update diverges at dt = 1e-4 and 5e-5 on this grid (wave speed 100 per
+    unit time, CFL number dt * 100 * n_grid)."""
+    import torch
+    from autoquantizer_amd.search import search_scheme
+
+    make, init = _adjoint_scene(n, n_grid, dt)
+    search_scheme(make, init, 8, tol_rel=tol_rel, ref=1.0, checkpoint_every=4)  # warm-up (kernel loading)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = search_scheme(make, init, nsteps, rate=0.5, checkpoint_every=checkpoint_every)
+    res = search_scheme(make, init, nsteps, tol_rel=tol_rel, ref=1.0, checkpoint_every=checkpoint_every)
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
-    return {"seconds": total, "phases": res["seconds"], "bits": [int(b) for b in res["bits"]],
-            "grads": [float(g) for g in res["grads"]],
+    out = {"seconds": total, "phases": res["seconds"], "bits": [int(b) for b in res["bits"]],
+           "grads": [float(g) for g in res["grads"]],
+           "scene": f"2D elastic, {n} particles, {n_grid}^2, dt {dt}, {nsteps} substeps, checkpoints every "
+                    f"{checkpoint_every}, error bound {tol_rel} of |x|"}
+    wpath = os.path.join(ROOT, "autoquantizer_amd", "data", "surrogate_elastic2d.npz")
+    if os.path.exists(wpath):
+        from autoquantizer_amd.surrogate import Surrogate, search_scheme_surrogate
+        model = Surrogate.load(wpath)
+        search_scheme_surrogate(model, init, 1e4, dt, n_grid, 8, ref=1.0, tol_rel=tol_rel)  # warm-up
+        r = search_scheme_surrogate(model, init, 1e4, dt, n_grid, nsteps, ref=1.0, tol_rel=tol_rel,
+                                    judge_g=res["grads"])
+        sur_bits = r["bits"] if not r["fell_back"] else None
+        out["surrogate"] = {"seconds": r["seconds"], "speedup": total / r["seconds"],
+                            "meets_bound_under_adjoint_sums": bool(r["met"]),
+                            "bits": [int(b) for b in (sur_bits if sur_bits is not None else r["bits"])],
+                            "extra_bits_per_particle": int(np.maximum(np.asarray(r["bits"]) - 1, 0).sum()
+                                                           - np.maximum(np.asarray(res["bits"]) - 1, 0).sum())}
+    return out
+
+
+def adjoint_check(n=65536, n_grid=256, nsteps=256, checkpoint_every=16, dt=2.5e-5):
+    """The adjoint error-gradient run on the adjoint scene: noise per field
+    uniform of width 2^-bits times the field's range (x, v 16 bits; C, F 13
+    bits), d |x_T - x_T^FP64|^2 / d(noise scale of the field) from the fp32
+    adjoint (f64 sums), against central differences (h = 1e-3) of the f64
+    forward.  Each field is checked in a run with that field's noise alone:
+    with all fields noisy, x_T is moved ~1e7x more by the position noise than
+    by the C noise, and a difference quotient of the C scale would measure
+    rounding, not the derivative."""
+    import torch
+    from autoquantizer_amd.adjoint import (Noise, _Schedule, _advance_plain, field_gradients, final_position_error,
+                                           noise_scale_gradients)
+    from autoquantizer_amd.search import detect_ranges, elastic_2d_ranges
+
+    make, init = _adjoint_scene(n, n_grid, dt)
+    s64 = make("f64")
+    s64.set_state(init)
+    ranges = elastic_2d_ranges(detect_ranges(s64, nsteps), 2)
+    s64.set_state(init)
+    base = s64.state.clone()
+    x = base.clone()
+    _advance_plain(s64, x, nsteps)
+    xref = x[:2, :n].clone()
+    xr = xref.cpu().numpy()
+    bits = np.array([16] * 4 + [13] * 8, dtype=np.float64)
+    sigma = ranges * np.exp2(-bits)
+    fields = [("x", slice(0, 2)), ("v", slice(2, 4)), ("C", slice(4, 8)), ("F", slice(8, 12))]
+    s32 = make("f32")
+    s32.set_state(init)
+    noise_scale_gradients(s32, 8, sigma, xr, seed=1, checkpoint_every=4)  # warm-up
+    s32.set_state(init)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    L, dsig = noise_scale_gradients(s32, nsteps, sigma, xr, seed=1, checkpoint_every=checkpoint_every)
+    t_adj = time.perf_counter() - t0
+    all_fields = field_gradients(dsig, sigma, fields)
+
+    def loss(sig):
+        sched = _Schedule(s64, nsteps, checkpoint_every, Noise(sig, seed=1))
+        return float(final_position_error(s64, sched.forward(base), xref))
+
+    h = 1e-3
+    per_field = {}
+    for name, sl in fields:
+        sf = np.zeros(12)
+        sf[sl] = sigma[sl]
+        s32.set_state(init)
+        Lf, df = noise_scale_gradients(s32, nsteps, sf, xr, seed=1, checkpoint_every=checkpoint_every)
+        ad = float(sf @ df)
+        fd = (loss(sf * (1 + h)) - loss(sf * (1 - h))) / (2 * h)
+        per_field[name] = {"loss": Lf, "adjoint": ad, "finite_difference": fd,
+                           "relative_difference": abs(ad - fd) / max(abs(fd), 1e-300)}
+    return {"seconds": t_adj, "loss": L, "field_gradients_all_noise": all_fields, "per_field": per_field,
+            "max_relative_difference": max(v["relative_difference"] for v in per_field.values()),
             "scene": f"2D elastic, {n} particles, {n_grid}^2, dt {dt}, {nsteps} substeps, checkpoints every "
-                     f"{checkpoint_every}, rate 0.5"}
+                     f"{checkpoint_every}, fp32 adjoint, f64 forward for the differences"}
 
 
 def main():
@@ -244,6 +326,10 @@ def main():
             out["scheme_search"] = ss
         except Exception as e:  # reported, not hidden: the line still carries the throughput
             out["scheme_search"] = {"error": f"{type(e).__name__}: {e}"}
+        try:
+            out["adjoint_check"] = adjoint_check()
+        except Exception as e:
+            out["adjoint_check"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_fp64:
         sim = make_solver("f64", n, args.grid)
         sim.set_state(init)

@@ -231,6 +231,21 @@ int64_t aq_adj_workspace_bytes(const aq_sim *sim);
 int aq_adj_substep(aq_sim *sim, const void *state, int64_t n, const void *adj_next, void *adj_out,
                    double *gsum, void *workspace, int64_t workspace_bytes, void *stream);
 
+/* Modelled quantization noise of the adjoint error-gradient run: each
+   quantized store after a substep adds a uniform rounding error of width
+   sigma[c] (the model behind the reference's error sums,
+   mpm_solver.py:1232-1271 and solvers/analytical_solver.py:3-84, applied
+   here as explicit noise so that d loss / d sigma can be taken and checked
+   by finite differences).  xi(step, c, p) is uniform in [-1/2, 1/2), the dithering noise of
+   (seed, step, component c, particle slot p) minus 1/2.
+   aq_noise_add: state[c][p] += sigma[c] xi (sigma: host double[ncomp],
+   finite, >= 0).  aq_noise_dot: out[c] += sum_p adj[c][p] xi (out: device
+   double[ncomp]), the derivative of a loss w.r.t. sigma[c] for the noise
+   added at `step` when adj = dL/d(state after the noise).  Plain storage. */
+int aq_noise_add(aq_sim *sim, void *state, int64_t n, const double *sigma, uint64_t seed, uint32_t step,
+                 void *stream);
+int aq_noise_dot(aq_sim *sim, const void *adj, int64_t n, uint64_t seed, uint32_t step, double *out, void *stream);
+
 /* ---- surrogate MLP of the scheme search: training and inference kernels
    (no counterpart in the reference, whose search always runs the adjoint) ---
    in -> 64 (tanh) -> 64 (tanh) -> 64 (tanh) -> out; parameters one flat fp32
@@ -242,12 +257,14 @@ int aq_adj_substep(aq_sim *sim, const void *state, int64_t n, const void *adj_ne
 /* y[s][out] for x[s][in]; act [s][3 * 64] receives the hidden activations
    (h1, h2, h3) that aq_mlp_backward uses. */
 int aq_mlp_forward(const float *params, const float *x, int32_t nsamples, float *act, float *y, void *stream);
-/* Gradient of mean((y - target)^2) over samples and outputs with respect to
-   the parameters, into grad[AQ_MLP_PARAMS]; delta: scratch [s][3 * 64 + out];
-   loss (NULL or device double): the summed squared error is added to it. */
+/* Gradient of mean((y - target)^2) over samples and outputs plus
+   weight_decay * (sum of the squared weight-matrix entries; biases excluded)
+   with respect to the parameters, into grad[AQ_MLP_PARAMS]; delta: scratch
+   [s][3 * 64 + out]; loss (NULL or device double): the summed squared error
+   (without the decay term) is added to it. */
 int aq_mlp_backward(const float *params, const float *x, const float *act, const float *y,
-                    const float *target, int32_t nsamples, float *delta, float *grad, double *loss,
-                    void *stream);
+                    const float *target, int32_t nsamples, float weight_decay, float *delta, float *grad,
+                    double *loss, void *stream);
 /* One Adam step (bias-corrected, step t >= 1) on n parameters. */
 int aq_adam_step(float *params, const float *grad, float *m, float *v, int32_t n, int32_t t, float lr,
                  float beta1, float beta2, float eps, void *stream);

@@ -254,6 +254,12 @@ float orc_noise_u(uint64_t seed, uint32_t step, uint32_t slot, uint32_t comp) {
   return (float)(h >> 8) * (1.0f / 16777216.0f);
 }
 
+void orc_noise_fill(uint64_t seed, uint32_t step, int nc, int64_t n, double *xi) {
+  for (int c = 0; c < nc; c++)
+    for (int64_t p = 0; p < n; p++)
+      xi[(int64_t)c * n + p] = (double)orc_noise_u(seed, step, (uint32_t)p, (uint32_t)c) - 0.5;
+}
+
 /* ---- record level ----------------------------------------------------- */
 
 static int group_members(const orc_comp *c, int nc, int k, int *idx) {

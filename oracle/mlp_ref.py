@@ -31,8 +31,10 @@ def forward(p, x):
     return h3 @ W4.T + b4, (h1, h2, h3)
 
 
-def loss_and_grad(p, x, t):
-    """mean((y - t)^2) and its gradient with respect to the flat parameters."""
+def loss_and_grad(p, x, t, weight_decay=0.0):
+    """mean((y - t)^2) and the gradient of mean((y - t)^2) + weight_decay *
+    (sum of squared weight-matrix entries) with respect to the flat
+    parameters (the returned loss excludes the decay term)."""
     W1, b1, W2, b2, W3, b3, W4, b4 = unflatten(p)
     x = np.asarray(x, dtype=np.float64)
     y, (h1, h2, h3) = forward(p, x)
@@ -42,7 +44,9 @@ def loss_and_grad(p, x, t):
     d3 = (d4 @ W4) * (1 - h3 * h3)
     d2 = (d3 @ W3) * (1 - h2 * h2)
     d1 = (d2 @ W2) * (1 - h1 * h1)
-    g = [d1.T @ x, d1.sum(0), d2.T @ h1, d2.sum(0), d3.T @ h2, d3.sum(0), d4.T @ h3, d4.sum(0)]
+    wd2 = 2.0 * weight_decay
+    g = [d1.T @ x + wd2 * W1, d1.sum(0), d2.T @ h1 + wd2 * W2, d2.sum(0), d3.T @ h2 + wd2 * W3, d3.sum(0),
+         d4.T @ h3 + wd2 * W4, d4.sum(0)]
     return L, np.concatenate([a.ravel() for a in g])
 
 

@@ -84,6 +84,9 @@ int32_t orc_sext(uint32_t code, int nbits);
 uint32_t orc_hash32(uint32_t x);
 uint32_t orc_noise_key(uint64_t seed, uint32_t step, uint32_t comp);
 float orc_noise_u(uint64_t seed, uint32_t step, uint32_t slot, uint32_t comp);
+/* xi[c][p] = orc_noise_u(seed, step, p, c) - 1/2, the modelled rounding
+   noise of the adjoint error-gradient run (uniform in [-1/2, 1/2)). */
+void orc_noise_fill(uint64_t seed, uint32_t step, int nc, int64_t n, double *xi);
 
 /* ---- record level: words are word planes, words[w * n + p] ----------- */
 /* noise: [nc][n] uniform [0,1) values, or NULL to use orc_noise_u(seed, step, p, c). */

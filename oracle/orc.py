@@ -81,6 +81,8 @@ def lib():
         L.orc_noise_key.argtypes = [ctypes.c_uint64, u32, u32]
         L.orc_noise_u.restype = f32
         L.orc_noise_u.argtypes = [ctypes.c_uint64, u32, u32, u32]
+        L.orc_noise_fill.restype = None
+        L.orc_noise_fill.argtypes = [ctypes.c_uint64, u32, ctypes.c_int, ctypes.c_int64, P]
         for name in ("orc_pack_f32", "orc_pack_f64"):
             getattr(L, name).restype = None
             getattr(L, name).argtypes = [P, i32, i32, P, P, ctypes.c_uint64, u32, P, ctypes.c_int64]
@@ -176,6 +178,13 @@ def shared_decode(e, d, nbits, signed, exp_bits):
 
 def noise_u(seed, step, slot, comp):
     return lib().orc_noise_u(seed, step, slot, comp)
+
+
+def noise_xi(seed, step, nc, n):
+    """[nc, n] float64 modelled rounding noise xi = u - 1/2 of (seed, step)."""
+    out = np.zeros((nc, n), dtype=np.float64)
+    lib().orc_noise_fill(seed, step, nc, n, _ptr(out))
+    return out
 
 
 # ---- record level ----------------------------------------------------------
@@ -320,3 +329,42 @@ def adjoint_run(s, state0, nsteps, loss="energy", direction=None):
     L = lib().orc_adjoint_run_f64(ctypes.byref(s), _ptr(st), st.shape[1], int(nsteps), kind, _ptr(d),
                                   _ptr(adj0), _ptr(gsum))
     return L, adj0, gsum
+
+
+def noise_scale_run(s, state0, nsteps, sigma, seed, xref):
+    """The adjoint error-gradient functional on plain f64 substeps:
+    s_t = substep(s_{t-1}) + sigma * xi_t (t = 1..nsteps, xi_t = noise_xi(seed, t)),
+    L = sum_p |x_T - xref|^2.  Returns (L, dL/dsigma [nc]) with
+    dL/dsigma_c = sum_t sum_p lambda_t[c, p] xi_t[c, p], lambda_t = dL/ds_t,
+    from the adjoint of adj_substep (reverse mode over stored states)."""
+    st = np.array(state0, dtype=np.float64)
+    nc, n = st.shape
+    dim = 3 if nc in (16, 24) else 2
+    sig = np.asarray(sigma, dtype=np.float64)[:, None]
+    states = [st.copy()]
+    for t in range(1, nsteps + 1):
+        run_plain(s, st, 1)
+        st += sig * noise_xi(seed, t, nc, n)
+        states.append(st.copy())
+    d = st[:dim] - np.asarray(xref, dtype=np.float64)
+    L = float((d * d).sum())
+    lam = np.zeros_like(st)
+    lam[:dim] = 2.0 * d
+    dsig = np.zeros(nc)
+    for t in range(nsteps, 0, -1):
+        dsig += (lam * noise_xi(seed, t, nc, n)).sum(axis=1)
+        lam = adj_substep(s, states[t - 1], lam)
+    return L, dsig
+
+
+def noise_scale_loss(s, state0, nsteps, sigma, seed, xref):
+    """L of noise_scale_run without the adjoint (for finite differences)."""
+    st = np.array(state0, dtype=np.float64)
+    nc, n = st.shape
+    dim = 3 if nc in (16, 24) else 2
+    sig = np.asarray(sigma, dtype=np.float64)[:, None]
+    for t in range(1, nsteps + 1):
+        run_plain(s, st, 1)
+        st += sig * noise_xi(seed, t, nc, n)
+    d = st[:dim] - np.asarray(xref, dtype=np.float64)
+    return float((d * d).sum())

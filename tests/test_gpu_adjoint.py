@@ -120,3 +120,98 @@ def test_rollout_autograd_linear_loss(orc):
     got = s0.grad[:, :150].cpu().numpy()
     scale = np.abs(adj_ref).max(axis=1, keepdims=True) + 1e-30
     assert (np.abs(got - adj_ref) / scale).max() < 1e-8
+
+
+# ---- the adjoint error-gradient run: d |x_T - x_ref|^2 / d sigma -----------
+
+def _sigma(st):
+    return np.abs(st).max(axis=1) * 2.0 ** -10 + 1e-6
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_noise_add_and_dot_vs_oracle(orc, storage):
+    from autoquantizer_amd.adjoint import Noise
+    print(f"[noise add/dot] {storage}", flush=True)
+    n = 1000
+    sim, s, st = make(orc, "elastic_2d", n, storage, seed=21)
+    sigma = _sigma(st)
+    nz = Noise(sigma, seed=77)
+    x = sim.state.clone()
+    nz.add(sim, x, 5)
+    xi = orc.noise_xi(77, 5, sim.ncomp, n)
+    expect = sim.get_state() + sigma[:, None] * xi
+    got = x[:, :n].double().cpu().numpy()
+    if storage == "f64":
+        assert np.array_equal(got, expect)
+    else:
+        np.testing.assert_allclose(got, expect, rtol=2e-7, atol=1e-7 * sigma.max())
+    rng = np.random.default_rng(22)
+    a = rng.standard_normal((sim.ncomp, n))
+    A = torch.zeros_like(sim.state)
+    A[:, :n] = torch.as_tensor(a, dtype=A.dtype, device="cuda")
+    out = torch.zeros(sim.ncomp, dtype=torch.float64, device="cuda")
+    nz.dot(sim, A, 5, out)
+    a_used = A[:, :n].double().cpu().numpy()
+    np.testing.assert_allclose(out.cpu().numpy(), (a_used * xi).sum(axis=1), rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("kind", ["elastic_2d", "fluid_2d", "fluid_3d"])
+def test_noise_scale_gradients_vs_oracle(orc, kind):
+    """Checkpointed f64 run (segments of 3) against the oracle's stored-state run."""
+    from autoquantizer_amd.adjoint import noise_scale_gradients, _advance_plain
+    print(f"[noise grad] {kind}", flush=True)
+    n, T = 200, 10
+    sim, s, st = make(orc, kind, n, "f64", seed=23)
+    x = sim.state.clone()
+    _advance_plain(sim, x, T)
+    xref = x[:sim.dim, :n].cpu().numpy()
+    sigma = _sigma(st)
+    L, dsig = noise_scale_gradients(sim, T, sigma, xref, seed=3, checkpoint_every=3)
+    L_ref, dsig_ref = orc.noise_scale_run(s, sim.get_state(), T, sigma, 3, xref)
+    assert L == pytest.approx(L_ref, rel=1e-8)
+    np.testing.assert_allclose(dsig, dsig_ref, rtol=1e-7, atol=1e-9 * np.abs(dsig_ref).max())
+
+
+@pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d"])
+def test_noise_scale_fp32_adjoint_vs_finite_differences(orc, kind):
+    """Per-field derivative of |x_T - x_T^FP64|^2 w.r.t. the field's noise
+    scale: fp32 adjoint (f64 reduction) against central differences
+    (h = 1e-3) of the GPU f64 forward, within 1e-3 relative per field, or
+    within 1e-6 of the largest field's derivative for a field that
+    contributes less than that (in the 3D fluid the velocity noise moves x_T
+    six orders less than the position noise, and the fp32 rounding of its
+    adjoint, summed over particles against zero-mean noise, is of that
+    order)."""
+    from autoquantizer_amd.adjoint import (Noise, _Schedule, _advance_plain, field_gradients,
+                                           final_position_error, noise_scale_gradients)
+    print(f"[noise fd] {kind}", flush=True)
+    n, T = 500, 16
+    sim64, s, st = make(orc, kind, n, "f64", seed=25)
+    sim32, _, _ = make(orc, kind, n, "f32", seed=25)
+    d = sim64.dim
+    x = sim64.state.clone()
+    _advance_plain(sim64, x, T)
+    xref = x[:d, :n].clone()
+    sigma = _sigma(st)
+    _, dsig = noise_scale_gradients(sim32, T, sigma, xref.cpu().numpy(), seed=4, checkpoint_every=4)
+    fields = [("x", slice(0, d)), ("v", slice(d, 2 * d)), ("C", slice(2 * d, 2 * d + d * d)),
+              ("J|F", slice(2 * d + d * d, sim64.ncomp))]
+    ad = field_gradients(dsig, sigma, fields)
+    base = sim64.state.clone()
+
+    def loss(scale_f):
+        sched = _Schedule(sim64, T, 4, Noise(sigma * scale_f, seed=4))
+        return float(final_position_error(sim64, sched.forward(base), xref))
+
+    h = 1e-3
+    fd = {}
+    for name, sl in fields:
+        up = np.ones(sim64.ncomp)
+        dn = np.ones(sim64.ncomp)
+        up[sl] += h
+        dn[sl] -= h
+        fd[name] = (loss(up) - loss(dn)) / (2 * h)
+        print(name, "adjoint", ad[name], "fd", fd[name])
+    big = max(abs(v) for v in fd.values())
+    for name, _ in fields:
+        assert ad[name] == pytest.approx(fd[name], rel=1e-3, abs=1e-6 * big), name

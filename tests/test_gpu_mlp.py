@@ -39,13 +39,15 @@ def test_forward_matches_oracle_and_torch():
     np.testing.assert_allclose(y.cpu().numpy(), torch_forward(p, x), rtol=1e-4, atol=1e-5)
 
 
-def test_backward_matches_oracle():
+@pytest.mark.parametrize("wd", [0.0, 3e-3])
+def test_backward_matches_oracle(wd):
     m = model(2)
     rng = np.random.default_rng(3)
     x = rng.standard_normal((257, M.IN)).astype(np.float32)
     t = rng.standard_normal((257, M.OUT)).astype(np.float32)
-    L, g = m.loss_and_grad(torch.as_tensor(x, device="cuda"), torch.as_tensor(t, device="cuda"))
-    Lo, go = M.loss_and_grad(m.params.cpu().numpy().astype(np.float64), x.astype(np.float64), t.astype(np.float64))
+    L, g = m.loss_and_grad(torch.as_tensor(x, device="cuda"), torch.as_tensor(t, device="cuda"), wd)
+    Lo, go = M.loss_and_grad(m.params.cpu().numpy().astype(np.float64), x.astype(np.float64), t.astype(np.float64),
+                             weight_decay=wd)
     assert L == pytest.approx(Lo, rel=1e-5)
     g = g.cpu().numpy()
     assert np.abs(g - go).max() <= 1e-4 * np.abs(go).max() + 1e-7

@@ -85,3 +85,53 @@ def test_error_sums_cover_intermediate_states(orc):
         _, adj_t, _ = orc.adjoint_run(s, x, T - t)
         expect += (adj_t ** 2).sum(axis=1)
     np.testing.assert_allclose(gsum, expect, rtol=1e-10)
+
+
+# ---- the adjoint error-gradient run: d |x_T - x_ref|^2 / d sigma -----------
+
+def test_noise_xi_is_centred_uniform(orc):
+    xi = orc.noise_xi(7, 3, 4, 20000)
+    assert xi.min() >= -0.5 and xi.max() < 0.5
+    assert abs(xi.mean()) < 0.01 and abs(xi.var() - 1 / 12) < 0.002
+    assert xi[2, 11] == orc.noise_u(7, 3, 11, 2) - 0.5
+    # components and steps draw different noise
+    assert abs(np.corrcoef(xi[0], xi[1])[0, 1]) < 0.03
+    assert abs(np.corrcoef(xi[0], orc.noise_xi(7, 4, 1, 20000)[0])[0, 1]) < 0.03
+
+
+def _noise_scene(orc, kind, n, T, seed):
+    s, st = small_scene(orc, kind, n, seed=seed)
+    ref = st.copy()
+    orc.run_plain(s, ref, T)
+    d = 3 if kind == "fluid_3d" else 2
+    sigma = np.abs(st).max(axis=1) * 2.0 ** -10 + 1e-6
+    return s, st, ref[:d], sigma
+
+
+@pytest.mark.parametrize("kind", ["elastic_2d", "fluid_2d", "fluid_3d"])
+def test_noise_scale_gradient_matches_central_differences(orc, kind):
+    """Every component's dL/dsigma against central differences with
+    h = 1e-3 of sigma."""
+    T = 6
+    s, st, xref, sigma = _noise_scene(orc, kind, 120, T, seed=11)
+    L, dsig = orc.noise_scale_run(s, st, T, sigma, 5, xref)
+    assert L == pytest.approx(orc.noise_scale_loss(s, st, T, sigma, 5, xref), rel=1e-12)
+    for c in range(len(sigma)):
+        h = 1e-3 * sigma[c]
+        e = np.zeros_like(sigma)
+        e[c] = h
+        fd = (orc.noise_scale_loss(s, st, T, sigma + e, 5, xref)
+              - orc.noise_scale_loss(s, st, T, sigma - e, 5, xref)) / (2 * h)
+        assert dsig[c] == pytest.approx(fd, rel=1e-6, abs=1e-9 * L / sigma[c]), c
+
+
+def test_noise_scale_gradient_is_homogeneous(orc):
+    """Against the noise-free run, L is quadratic in sigma to leading order,
+    so sum_c sigma_c dL/dsigma_c = 2 L (Euler) up to the curvature of the
+    dynamics; with sigma = 0, L and its gradient vanish."""
+    T = 8
+    s, st, xref, sigma = _noise_scene(orc, "elastic_2d", 150, T, seed=12)
+    L, dsig = orc.noise_scale_run(s, st, T, sigma * 1e-3, 9, xref)
+    assert (sigma * 1e-3) @ dsig == pytest.approx(2 * L, rel=1e-4)
+    L0, d0 = orc.noise_scale_run(s, st, T, np.zeros_like(sigma), 9, xref)
+    assert L0 == 0.0 and np.all(d0 == 0.0)
