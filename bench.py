@@ -179,7 +179,10 @@ def adjoint_check(n=65536, n_grid=256, nsteps=256, checkpoint_every=16, dt=2.5e-
     forward.  Each field is checked in a run with that field's noise alone:
     with all fields noisy, x_T is moved ~1e7x more by the position noise than
     by the C noise, and a difference quotient of the C scale would measure
-    rounding, not the derivative."""
+    rounding, not the derivative.  The f64 adjoint is reported beside the
+    fp32 one: the fp32 run's own rounding moves x_T by about as much as the
+    v and C noise do, so its loss (and the fp32 adjoint, exact for that
+    loss) differs from the f64 loss the differences are taken of there."""
     import torch
     from autoquantizer_amd.adjoint import (Noise, _Schedule, _advance_plain, field_gradients, final_position_error,
                                            noise_scale_gradients)
@@ -220,11 +223,16 @@ def adjoint_check(n=65536, n_grid=256, nsteps=256, checkpoint_every=16, dt=2.5e-
         s32.set_state(init)
         Lf, df = noise_scale_gradients(s32, nsteps, sf, xr, seed=1, checkpoint_every=checkpoint_every)
         ad = float(sf @ df)
+        s64.set_state(init)
+        L64, d64 = noise_scale_gradients(s64, nsteps, sf, xr, seed=1, checkpoint_every=checkpoint_every)
+        ad64 = float(sf @ d64)
         fd = (loss(sf * (1 + h)) - loss(sf * (1 - h))) / (2 * h)
-        per_field[name] = {"loss": Lf, "adjoint": ad, "finite_difference": fd,
-                           "relative_difference": abs(ad - fd) / max(abs(fd), 1e-300)}
+        per_field[name] = {"loss_fp32": Lf, "loss_f64": L64, "adjoint_fp32": ad, "adjoint_f64": ad64,
+                           "finite_difference": fd,
+                           "relative_difference_fp32": abs(ad - fd) / max(abs(fd), 1e-300),
+                           "relative_difference_f64": abs(ad64 - fd) / max(abs(fd), 1e-300)}
     return {"seconds": t_adj, "loss": L, "field_gradients_all_noise": all_fields, "per_field": per_field,
-            "max_relative_difference": max(v["relative_difference"] for v in per_field.values()),
+            "max_relative_difference_f64": max(v["relative_difference_f64"] for v in per_field.values()),
             "scene": f"2D elastic, {n} particles, {n_grid}^2, dt {dt}, {nsteps} substeps, checkpoints every "
                      f"{checkpoint_every}, fp32 adjoint, f64 forward for the differences"}
 
