@@ -525,6 +525,8 @@ template <class T> struct AccOf;
 template <> struct AccOf<float> {
   using U = uint32_t;
   static constexpr int BITS = 30;
+  // LDS planes: two int32 sums per 64-bit word, (m v_x, m) and (m v_y, m v_z)
+  static constexpr int PLANES = 2;
   // floor(x + 1/2) as int32 (two's complement), |x| < 2^31
   __device__ static U fix(float x) {
     int32_t r;
@@ -532,10 +534,22 @@ template <> struct AccOf<float> {
     return (U)r;
   }
   __device__ static float unfix(U q, int k) { return ldexpf((float)(int32_t)q, -k); }
+  // hi 2^32 + lo as a 64-bit integer (lo sign-extended).  A 64-bit sum of
+  // such words is (sum hi) 2^32 + (sum lo) exactly, so as long as both sums
+  // fit in int32 (the per-item scales keep them below 2^31) unpack recovers
+  // them: one 64-bit LDS atomic does the work of two 32-bit ones.
+  __device__ static unsigned long long pack(float hi, float lo) {
+    return ((unsigned long long)fix(hi) << 32) + (unsigned long long)(long long)(int32_t)fix(lo);
+  }
+  __device__ static void unpack(unsigned long long w, int32_t &hi, int32_t &lo) {
+    lo = (int32_t)(uint32_t)w;
+    hi = (int32_t)(uint32_t)((w - (unsigned long long)(long long)lo) >> 32);
+  }
 };
 template <> struct AccOf<double> {
   using U = unsigned long long;
   static constexpr int BITS = 62;
+  static constexpr int PLANES = 4;  // one int64 sum per component
   // round(x) as int64 from exact steps, |x| < 2^62: |f| = hi 2^32 + lo with
   // hi < 2^30 and lo = |f| - hi 2^32 an integer below 2^32
   __device__ static U fix(double x) {
@@ -572,13 +586,16 @@ k_tiled(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
   using U = typename ACC::U;
   constexpr int TS = TL::TS, TN = TL::TN, TE = TL::TE, NC = Comps<DIM, MAT>::NC;
   __shared__ vec4_t<T> tv[DO_G2P ? TS : 1];
-  __shared__ U ts[DO_P2G ? 4 : 1][DO_P2G ? TS : 1];
+  constexpr int NP = ACC::PLANES;
+  __shared__ unsigned long long ts[DO_P2G ? NP : 1][DO_P2G ? TS : 1];
   __shared__ T bpart[4][4];
   __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   uint32_t *row = rows + tid * RowsOf<ST, NW32>::RS;
   if constexpr (DO_P2G)
-    for (int q = tid; q < TS; q += 256) ts[0][q] = ts[1][q] = ts[2][q] = ts[3][q] = (U)0;
+    for (int q = tid; q < TS; q += 256)
+#pragma unroll
+      for (int a = 0; a < NP; ++a) ts[a][q] = 0ull;
   const int ni = *nitems;
   // each XCD (blocks dealt round-robin to 8 XCDs) walks a contiguous item range
   const int xcd = (int)(blockIdx.x & 7u);
@@ -682,9 +699,14 @@ k_tiled(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
           R.pm = tldexp<T>(R.pm, k[3]);
           p2g_emit<DIM, T>(R, S, [&](int i, int j, int kk, const T (&mom)[3], T m) {
             const int q = TL::slot(rel[0] + i, rel[1] + j, rel[2] + kk);
+            if constexpr (NP == 2) {
+              atomicAdd(&ts[0][q], ACC::pack(mom[0], m));
+              atomicAdd(&ts[1][q], ACC::pack(mom[1], DIM == 3 ? mom[2] : (T)0));
+            } else {
 #pragma unroll
-            for (int a = 0; a < DIM; ++a) atomicAdd(&ts[a][q], ACC::fix(mom[a]));
-            atomicAdd(&ts[3][q], ACC::fix(m));
+              for (int a = 0; a < DIM; ++a) atomicAdd(&ts[a][q], (unsigned long long)ACC::fix(mom[a]));
+              atomicAdd(&ts[3][q], (unsigned long long)ACC::fix(m));
+            }
           });
         } else {
           p2g_particle<DIM, MAT, T>(out, P, grid_f, fflags);
@@ -697,12 +719,23 @@ k_tiled(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
         TL::node_coords(nd, t);
         const int q = TL::slot(t[0], t[1], t[2]);
         vec4_t<T> val;
-        val.x = ACC::unfix(ts[0][q], k[0]);
-        val.y = ACC::unfix(ts[1][q], k[1]);
-        val.z = DIM == 3 ? ACC::unfix(ts[2][q], k[2]) : (T)0;
-        val.w = ACC::unfix(ts[3][q], k[3]);
+        if constexpr (NP == 2) {
+          int32_t mx, m, my, mz;
+          ACC::unpack(ts[0][q], mx, m);
+          ACC::unpack(ts[1][q], my, mz);
+          val.x = ACC::unfix((U)mx, k[0]);
+          val.y = ACC::unfix((U)my, k[1]);
+          val.z = DIM == 3 ? ACC::unfix((U)mz, k[2]) : (T)0;
+          val.w = ACC::unfix((U)m, k[3]);
+        } else {
+          val.x = ACC::unfix(ts[0][q], k[0]);
+          val.y = ACC::unfix(ts[1][q], k[1]);
+          val.z = DIM == 3 ? ACC::unfix(ts[2][q], k[2]) : (T)0;
+          val.w = ACC::unfix(ts[3][q], k[3]);
+        }
         dst[nd] = val;
-        ts[0][q] = ts[1][q] = ts[2][q] = ts[3][q] = (U)0;
+#pragma unroll
+        for (int a = 0; a < NP; ++a) ts[a][q] = 0ull;
       }
     } else {
       __syncthreads();  // G2P done with the tile before the next item stages it
