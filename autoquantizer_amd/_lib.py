@@ -8,6 +8,27 @@ import os
 
 _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libaqmpm.so")
+# kernel sources read by the run-time specialisation (aq_sim_specialize)
+CSRC_DIR = os.path.join(_PKG, "csrc")
+INCLUDE_DIR = os.path.join(os.path.dirname(_PKG), "include")
+
+
+def jit_cache_dir(path=None):
+    """Directory of cached specialised code objects (created if missing):
+    path, else $AQ_JIT_CACHE, else aqmpm_jit_<uid> in the system temp dir."""
+    import tempfile
+    d = path or os.environ.get("AQ_JIT_CACHE") or os.path.join(tempfile.gettempdir(), f"aqmpm_jit_{os.getuid()}")
+    os.makedirs(d, exist_ok=True)
+    return d
+
+
+def jit_build(desc, cache_dir=None):
+    """Compile (or find cached) the specialised fused kernel for an
+    aq_sim_desc without a device; returns the code object's path."""
+    buf = ctypes.create_string_buffer(4096)
+    check(lib().aq_jit_build(ctypes.byref(desc), CSRC_DIR.encode(), INCLUDE_DIR.encode(),
+                             jit_cache_dir(cache_dir).encode(), buf, len(buf)))
+    return buf.value.decode()
 
 AQ_MAX_COMPS = 32
 AQ_MAX_WORDS = 16
@@ -65,7 +86,7 @@ EXPORTS = [
     "aq_substep", "aq_p2g", "aq_grid_op", "aq_g2p", "aq_advance", "aq_sort",
     "aq_update_limits", "aq_energy", "aq_sim_grid", "aq_advance_timed",
     "aq_adj_workspace_bytes", "aq_adj_substep", "aq_noise_add", "aq_noise_dot", "aq_mlp_forward",
-    "aq_mlp_backward", "aq_adam_step",
+    "aq_mlp_backward", "aq_adam_step", "aq_sim_specialize", "aq_jit_build",
 ]
 
 _lib = None
@@ -106,6 +127,8 @@ def lib():
     L.aq_adj_workspace_bytes.restype = i64
     L.aq_adj_workspace_bytes.argtypes = [P]
     L.aq_adj_substep.argtypes = [P, P, i64, P, P, P, P, i64, P]
+    L.aq_sim_specialize.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]
+    L.aq_jit_build.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, i64]
     L.aq_noise_add.argtypes = [P, P, i64, P, u64, u32, P]
     L.aq_noise_dot.argtypes = [P, P, i64, u64, u32, P, P]
     L.aq_mlp_forward.argtypes = [P, P, i32, P, P, P]

@@ -23,6 +23,8 @@ FLAGS = [
     # no packed FP32 (v_pk_*_f32) instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
+    # hipRTC: run-time specialisation of the fused kernel to a record layout
+    "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib",
 ]
 
 

@@ -16,8 +16,10 @@
 // encode/decode is rounded on its own, as in the reference's unfused form.
 #pragma once
 
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 #include "aq_mpm.h"
 
@@ -66,6 +68,22 @@ struct CodecK {
 typedef const __attribute__((address_space(4))) uint32_t ConstU32;
 constexpr int COMP_BATCH = 4;
 
+#ifdef AQ_JIT_LAYOUT
+// Run-time specialisation of a kernel to one record layout: the generated
+// source defines AQ_JIT_COMPS (the CompK initialisers), AQ_JIT_SHARED,
+// AQ_JIT_GFIRST, AQ_JIT_GLAST (structure masks), AQ_JIT_NW (record words)
+// and AQ_JIT_G2P_MASK (words G2P reads).  Descriptors and masks are then
+// compile-time constants: kinds, offsets and widths fold into the code.
+namespace jit {
+constexpr CompK kComps[] = AQ_JIT_COMPS;
+constexpr uint32_t kShared = AQ_JIT_SHARED, kGFirst = AQ_JIT_GFIRST, kGLast = AQ_JIT_GLAST;
+constexpr int kNW = AQ_JIT_NW;
+constexpr uint32_t kG2PMask = AQ_JIT_G2P_MASK;
+}  // namespace jit
+
+__device__ __forceinline__ ConstU32 *comp_table(const CodecK &) { return nullptr; }
+__device__ __forceinline__ CompK comp_load(ConstU32 *, int c) { return jit::kComps[c]; }
+#else
 __device__ __forceinline__ ConstU32 *comp_table(const CodecK &cod) {
   ConstU32 *p = (ConstU32 *)cod.c;
   asm volatile("" : "+s"(p));
@@ -80,6 +98,7 @@ __device__ __forceinline__ CompK comp_load(ConstU32 *p, int c) {
   __builtin_memcpy(&k, w, sizeof(k));
   return k;
 }
+#endif
 
 __device__ __forceinline__ CompK comp_at(const CodecK &cod, int c) { return comp_load(comp_table(cod), c); }
 
@@ -354,6 +373,17 @@ struct StreamSink {
   }
 };
 
+// RegSink ORs fields into NWD 64-bit registers; with compile-time offsets
+// (a specialised layout) every index is a constant.
+template <int NWD> struct RegSink {
+  uint64_t w[NWD];
+  __device__ __forceinline__ void put(uint32_t off, uint32_t nb, uint32_t v) {
+    const uint32_t i = off >> 6, b = off & 63u;
+    w[i] |= (uint64_t)v << b;
+    if (b + nb > 64u) w[i + 1] |= (uint64_t)v >> (64u - b);
+  }
+};
+
 // Encode NC values (compile-time count) with `sink` and return in dec[] the
 // values the stored fields decode to (the decode of each field's digits,
 // computed from the digits in registers).  Fields are written in component
@@ -362,10 +392,15 @@ struct StreamSink {
 template <int NC, class Sink>
 __device__ __forceinline__ void encode_record(const float (&v)[NC], const CodecK &cod, uint32_t slot,
                                               Sink &sink, float (&dec)[NC]) {
+#ifdef AQ_JIT_LAYOUT
+  constexpr uint32_t sm = jit::kShared, gf = jit::kGFirst, gl = jit::kGLast;
+  const uint32_t kb = cod.key_base;
+#else
   // the structure masks and the key base are re-read here (hidden from the
   // optimiser) rather than kept live across the caller's loops
   uint32_t sm = cod.shared_mask, gf = cod.gfirst_mask, gl = cod.glast_mask, kb = cod.key_base;
   asm volatile("" : "+s"(sm), "+s"(gf), "+s"(gl), "+s"(kb));
+#endif
   // largest f32 exponent of each shared group, spread over its members
   uint32_t gmax[NC];
   uint32_t run = 0u;

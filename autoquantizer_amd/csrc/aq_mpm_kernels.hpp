@@ -79,10 +79,15 @@ __device__ __forceinline__ void fetch_state(const StateK &s, int64_t p, uint32_t
   constexpr int CJ = 2 * DIM + DIM * DIM;
   if constexpr (ST == AQ_STORE_QUANT) {
     const uint64_t *w = (const uint64_t *)s.ptr;
+#ifdef AQ_JIT_LAYOUT
+    constexpr int nw = jit::kNW;
+#else
+    const int nw = s.nw;
+#endif
 #pragma unroll
     for (int i = 0; i < NW32 / 2; ++i) {
       f.w[i] = 0ull;
-      if (i < s.nw && ((mask >> i) & 1u)) f.w[i] = __builtin_nontemporal_load(w + (int64_t)i * s.cap + p);
+      if (i < nw && ((mask >> i) & 1u)) f.w[i] = __builtin_nontemporal_load(w + (int64_t)i * s.cap + p);
     }
   } else {
     const T *vals = (const T *)s.ptr;
@@ -98,6 +103,21 @@ __device__ __forceinline__ void finish_state(const StateK &s, const CodecK &cod,
                                              const Fetched<ST, NW32> &f, uint32_t *row, T (&v)[NC]) {
   if constexpr (ST == AQ_STORE_QUANT) {
     constexpr int CJ = 2 * DIM + DIM * DIM;
+#ifdef AQ_JIT_LAYOUT
+    // compile-time offsets: fields come straight out of the fetched words
+    uint32_t r[NW32];
+#pragma unroll
+    for (int i = 0; i < NW32 / 2; ++i) {
+      r[2 * i] = (uint32_t)f.w[i];
+      r[2 * i + 1] = (uint32_t)(f.w[i] >> 32);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      v[c] = (G2P_ONLY && c >= DIM && c < CJ) ? (T)0 : (T)decode_comp(r, comp_load(nullptr, c));
+    (void)row;
+    (void)mask;
+    return;
+#endif
 #pragma unroll
     for (int i = 0; i < NW32 / 2; ++i) {
       if (i < s.nw && ((mask >> i) & 1u)) {
@@ -138,6 +158,15 @@ __device__ __forceinline__ void store_state(const StateK &s, const CodecK &cod, 
 #pragma unroll
     for (int c = 0; c < NC; ++c) f[c] = (float)v[c];
     uint64_t *w = (uint64_t *)s.ptr;
+#ifdef AQ_JIT_LAYOUT
+    RegSink<NW32 / 2> sink;
+#pragma unroll
+    for (int i = 0; i < NW32 / 2; ++i) sink.w[i] = 0ull;
+    encode_record<NC>(f, cod, noise_slot(s, p), sink, dec);
+#pragma unroll
+    for (int i = 0; i < jit::kNW; ++i) __builtin_nontemporal_store(sink.w[i], w + (int64_t)i * s.cap + p);
+    (void)row;
+#else
     if (cod.stream) {
       StreamSink sink{w, s.cap, p, (uint32_t)s.nw, 0u, 0ull};
       encode_record<NC>(f, cod, noise_slot(s, p), sink, dec);
@@ -153,6 +182,7 @@ __device__ __forceinline__ void store_state(const StateK &s, const CodecK &cod, 
           __builtin_nontemporal_store((uint64_t)row[2 * i] | ((uint64_t)row[2 * i + 1] << 32),
                                       w + (int64_t)i * s.cap + p);
     }
+#endif
 #pragma unroll
     for (int c = 0; c < NC; ++c) v[c] = (T)dec[c];
   } else {

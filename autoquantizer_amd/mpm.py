@@ -136,6 +136,31 @@ class MPMSolver:
     cfg.compute_fp precision."""
 
     def __init__(self, cfg, types=None, groups=None, shared=(), capacity=None, device="cuda"):
+        self.device = device
+        self._setup(cfg, types, groups, shared, capacity)
+        if self.storage == _lib.AQ_STORE_QUANT:
+            self.state = torch.zeros((self.nwords, self.capacity), dtype=torch.int64, device=device)
+        else:
+            self.state = torch.zeros((self.ncomp, self.capacity),
+                                     dtype=torch.float64 if self.storage == _lib.AQ_STORE_F64 else torch.float32,
+                                     device=device)
+        self.ids = torch.arange(self.capacity, dtype=torch.int32, device=device)
+        self.current_idx = 0
+        self.step_count = 0
+        self._make_sim(cfg)
+        self.init_limits()
+
+    @classmethod
+    def describe(cls, cfg, types=None, groups=None, shared=(), capacity=None):
+        """The aq_sim_desc of a solver for cfg, without device work (what
+        aq_jit_build takes)."""
+        self = cls.__new__(cls)
+        self.device = "cpu"
+        self._setup(cfg, types, groups, shared, capacity)
+        return self._build_desc(cfg)
+
+    def _setup(self, cfg, types, groups, shared, capacity):
+        """Physical parameters and the record layout (host only)."""
         self.param = cfg
         self.dim = cfg.dim
         self.fluid = bool(cfg.use_fluid)
@@ -153,7 +178,6 @@ class MPMSolver:
         if cfg.softer:
             self.mu_0 *= 0.3
             self.la_0 *= 0.3
-        self.device = device
         self.names = component_names(self.dim, self.fluid)
         self.ncomp = len(self.names)
         if types is None and cfg.use_quantaichi:
@@ -163,23 +187,15 @@ class MPMSolver:
             if groups is None:
                 groups = [("pack", 0, self.names)]
             self.comps, self.nwords = plan_state(self.names, types, groups, tuple(shared))
-            self.state = torch.zeros((self.nwords, self.capacity), dtype=torch.int64, device=device)
         else:
             self.comps, self.nwords = [], 0
             f64 = str(cfg.compute_fp) in ("f64", "float64")
             self.storage = _lib.AQ_STORE_F64 if f64 else _lib.AQ_STORE_F32
-            self.state = torch.zeros((self.ncomp, self.capacity),
-                                     dtype=torch.float64 if f64 else torch.float32, device=device)
-        self.ids = torch.arange(self.capacity, dtype=torch.int32, device=device)
-        self.current_idx = 0
-        self.step_count = 0
         bound = cfg.bound if cfg.bound is not None else [3] * self.dim
         self.bound = int(bound[0])
-        self._make_sim(cfg)
-        self.init_limits()
 
     # ---- handle -----------------------------------------------------------
-    def _make_sim(self, cfg):
+    def _build_desc(self, cfg):
         d = _lib.SimDesc()
         d.dim, d.material = self.dim, _lib.AQ_FLUID if self.fluid else _lib.AQ_ELASTIC
         d.n_grid = self.n_grid
@@ -196,6 +212,10 @@ class MPMSolver:
         for k, c in enumerate(self.comps):
             for key, val in c.items():
                 setattr(d.comps[k], key, val)
+        return d
+
+    def _make_sim(self, cfg):
+        d = self._build_desc(cfg)
         self._desc = d
         L = _lib.lib()
         nbytes = L.aq_sim_workspace_bytes(ctypes.byref(d))
@@ -216,6 +236,21 @@ class MPMSolver:
                 pass
 
     # ---- state in and out -------------------------------------------------
+    def specialize(self, cache_dir=None):
+        """Compile the fused substep kernel for this solver's record layout
+        (hipRTC, aq_sim_specialize) and use it for every later advance();
+        code objects are cached in cache_dir (default: AQ_JIT_CACHE or a
+        directory under the system temp dir).  Quantized storage only."""
+        if self.storage != _lib.AQ_STORE_QUANT:
+            raise ValueError("specialisation applies to quantized storage")
+        cache = _lib.jit_cache_dir(cache_dir)
+        _lib.check(_lib.lib().aq_sim_specialize(self._h, _lib.CSRC_DIR.encode(), _lib.INCLUDE_DIR.encode(),
+                                                cache.encode()))
+
+    def unspecialize(self):
+        """Back to the generic (descriptor-driven) fused kernel."""
+        _lib.check(_lib.lib().aq_sim_specialize(self._h, None, None, None))
+
     def set_state(self, vals, n=None):
         """vals: [ncomp, n] float array (canonical component order)."""
         vals = np.asarray(vals, dtype=np.float64)

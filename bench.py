@@ -24,6 +24,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-substeps/s + scheme-search time, quantized vs fp32 MPM, 1 MI355X"
 HBM_PEAK = 8.0e12
+# HBM bytes per launch of the specialised fused kernel on this workload, from
+# rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a plain copy's readings
+# (profiles/k_tiled_fused_fluid3d_1M.md); a profiler run, not this script,
+# measures it.
+FUSED_TRAFFIC_BYTES = 103.1e6
 
 
 def make_solver(storage, n, n_grid, seed=0):
@@ -248,6 +253,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-search", action="store_true")
+    ap.add_argument("--generic-kernel", action="store_true",
+                    help="run the descriptor-driven fused kernel instead of the one specialised to the layout")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the substep has no multi-GPU path",
@@ -259,8 +266,13 @@ def main():
     init = initial_state(n)
 
     results = {}
+    jit_s = None
     for storage in ("quant", "f32"):
         sim = make_solver(storage, n, args.grid)
+        if storage == "quant" and not args.generic_kernel:
+            t0 = time.perf_counter()
+            sim.specialize()  # the fused kernel compiled for this record layout (cached)
+            jit_s = time.perf_counter() - t0
         sim.set_state(init)
         if W:
             sim.advance(W, sort_every=args.sort_every)
@@ -314,6 +326,8 @@ def main():
         "config": {"workload": f"3D fluid MLS-MPM, {n} particles uniform in [0.1,0.5]^3 (seed 0), "
                                f"{args.grid}^3 grid, dt 2e-4, quantized 32 B/particle record, dithered",
                    "particles": n, "grid": args.grid, "sort_every": args.sort_every,
+                   "fused_kernel": "generic" if args.generic_kernel else "specialised to the record layout",
+                   "specialise_seconds": jit_s,
                    "bytes_per_particle": q["nwords"] * 8, "fp32_bytes_per_particle": 64},
         "fp32": {"value": fp32_value, "ms_per_step": f["elapsed"] * 1e3 / K,
                  "fused_kernel_us": f_avg * 1e6, "fused_kernel_GBps": f_bytes / f_avg / 1e9},
@@ -321,8 +335,13 @@ def main():
         "kernel_ms": {"fused_g2p2g": q["ms"][0], "grid_update": q["ms"][1], "sort": q["ms"][2],
                       "first_p2g_last_g2p": q["ms"][3], "launches": q["launches"]},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "k_tiled<3,fluid,quant,G2P+P2G> (fused substep kernel)", "kernel_us": avg_s * 1e6,
+                     "frac": achieved / HBM_PEAK,
+                     "traffic": (FUSED_TRAFFIC_BYTES if not args.generic_kernel and n == 1 << 20 and args.grid == 128
+                                 else None),
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (profiles/k_tiled_fused_fluid3d_1M.md)",
+                     "kernel": ("k_tiled<3,fluid,quant,8,float,G2P+P2G>" if args.generic_kernel else
+                                "aq_jit_tiled (k_tiled<3,fluid,quant,8,float,G2P+P2G> specialised to the record)")
+                     + " (fused substep kernel)", "kernel_us": avg_s * 1e6,
                      "algorithmic_bytes": alg_bytes, "active_nodes": q["act"],
                      "bytes_per_particle": per_particle, "bytes_per_node": per_node},
         "scheme_search_s": None,

@@ -23,8 +23,10 @@
 #ifndef AQ_MPM_H
 #define AQ_MPM_H
 
+#ifndef __HIPCC_RTC__
 #include <stddef.h>
 #include <stdint.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -161,6 +163,22 @@ int64_t aq_sim_workspace_bytes(const aq_sim_desc *desc);
 int aq_sim_create(const aq_sim_desc *desc, void *workspace, int64_t workspace_bytes,
                   aq_sim **out);
 int aq_sim_destroy(aq_sim *sim);
+
+/* Run-time specialisation to the simulation's record layout (replaces the
+   per-scheme kernel compilation of the reference's Taichi JIT,
+   TS/taichi/codegen/codegen_llvm_quant.cpp: kernels are generated for each
+   quantization scheme).  The fused G2P/P2G kernel of aq_advance is compiled
+   with hipRTC from the kernel headers in src_dir (autoquantizer_amd/csrc)
+   and include_dir (include/) with the layout's descriptors as constants,
+   and used by every later tiled aq_advance / aq_advance_timed of `sim`
+   (results identical to the generic kernel).  Code objects are cached in
+   cache_dir (NULL: compile, no cache).  src_dir NULL returns to the generic
+   kernel.  Quantized storage only. */
+int aq_sim_specialize(aq_sim *sim, const char *src_dir, const char *include_dir, const char *cache_dir);
+/* Compile (or find in cache_dir) the specialised kernel's code object for a
+   descriptor without a device; path receives its file name. */
+int aq_jit_build(const aq_sim_desc *desc, const char *src_dir, const char *include_dir, const char *cache_dir,
+                 char *path, int64_t path_cap);
 
 /* The reference substep (mpm_solver.py:624-631): build_pid (aq_sort),
    p2g_bls, grid_op, g2p_bls, as separate launches.  `state` is the words
