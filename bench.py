@@ -242,6 +242,46 @@ def adjoint_check(n=65536, n_grid=256, nsteps=256, checkpoint_every=16, dt=2.5e-
                      f"{checkpoint_every}, fp32 adjoint, f64 forward for the differences"}
 
 
+def batched_search(nsims=32, nsteps=300, seed=0):
+    """The batched scheme search: nsims quantized simulations (2D/3D,
+    fluid/elastic, 16k-500k particles, grids 64^2-128^3, each with its own
+    candidate scheme) advanced nsteps substeps; sort intervals from the
+    autotuner; one-by-one library calls vs one captured HIP graph over 4
+    streams; final-position error of each against its FP64 run."""
+    import torch
+    from autoquantizer_amd.batch import Batch, fp64_positions, position_errors, random_specs
+
+    specs = random_specs(nsims, seed=seed)
+    b = Batch(specs)
+    t0 = time.perf_counter()
+    b.autotune()
+    t_tune = time.perf_counter() - t0
+    b.reset()
+    b.run_sequential(8)  # warm-up
+    b.reset()
+    t_seq = b.run_sequential(nsteps)
+    t0 = time.perf_counter()
+    b.capture(nsteps, nstreams=4)
+    t_cap = time.perf_counter() - t0
+    t_graph = b.run_graph()
+    got = b.positions()
+    del b
+    torch.cuda.empty_cache()
+    errs = position_errors(got, fp64_positions(specs, nsteps))
+    total = sum(sp.n for sp in specs)
+    rms = [e["rms"] for e in errs]
+    return {"sims": nsims, "substeps": nsteps, "particles": total,
+            "sequential_s": t_seq, "graph_s": t_graph, "graph_speedup": t_seq / t_graph,
+            "particle_substeps_per_s": total * nsteps / t_graph, "capture_s": t_cap,
+            "autotune_s": t_tune, "sort_every": [sp.sort_every for sp in specs],
+            "error_vs_fp64_rms": {"median": float(np.median(rms)), "max": float(np.max(rms))},
+            "per_sim": [[sp.dim, "fluid" if sp.fluid else "elastic", sp.n, sp.n_grid, sp.bits["x"], sp.bits["v"],
+                         sp.bits["C"], sp.bits["J"] if sp.fluid else sp.bits["F"], round(e["rms"], 9),
+                         round(e["max"], 9)] for sp, e in zip(specs, errs)],
+            "per_sim_fields": ["dim", "material", "particles", "grid", "x bits", "v bits", "C digits",
+                               "J bits | F digits", "x error rms", "x error max"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,6 +293,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-search", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--generic-kernel", action="store_true",
                     help="run the descriptor-driven fused kernel instead of the one specialised to the layout")
     args = ap.parse_args()
@@ -357,6 +398,11 @@ def main():
             out["adjoint_check"] = adjoint_check()
         except Exception as e:
             out["adjoint_check"] = {"error": f"{type(e).__name__}: {e}"}
+    if not args.no_batch:
+        try:
+            out["batched_search"] = batched_search()
+        except Exception as e:
+            out["batched_search"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_fp64:
         sim = make_solver("f64", n, args.grid)
         sim.set_state(init)
