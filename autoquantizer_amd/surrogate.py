@@ -220,12 +220,13 @@ def make_dataset(nscenes, seed, n, n_grid, nsteps, checkpoint_every=16, log=None
     return np.array(X), np.array(G), np.array(R), np.array(P)
 
 
-def search_scheme_surrogate(model, st, E, dt, n_grid, nsteps, ref, tol_rel, judge_g=None):
+def search_scheme_surrogate(model, st, E, dt, n_grid, nsteps, ref, tol_rel, judge=None):
     """Bit allocation from the surrogate: features + MLP + range run +
-    LagrangianEB.  With judge_g (the adjoint's per-component sums) the
-    allocation is checked against the bound under the adjoint's sums and
-    replaced by the adjoint allocation when it misses.  Returns dict(bits,
-    ranges, seconds, met, fell_back)."""
+    LagrangianEB.  With judge (the result of search.search_scheme on the same
+    scene: its per-component sums, ranges and allocation) the allocation is
+    checked against the bound under the adjoint's sums and replaced by the
+    adjoint's allocation when it misses.  Returns dict(bits (sign included),
+    ranges, seconds, met, fell_back, pred)."""
     from .search import detect_ranges, elastic_2d_ranges
     t0 = time.perf_counter()
     n = st.shape[1]
@@ -236,11 +237,12 @@ def search_scheme_surrogate(model, st, E, dt, n_grid, nsteps, ref, tol_rel, judg
     bits, _ = LagrangianEB(component_sums(pred) * ranges ** 2, ref=ref, tol_rel=tol_rel)
     torch.cuda.synchronize()
     seconds = time.perf_counter() - t0
+    bits = bits + 1
     met, fell_back = None, False
-    if judge_g is not None:
-        G = np.asarray(judge_g) * ranges ** 2
-        met = modelled_error(G, bits) <= (ref * tol_rel) ** 2
+    if judge is not None:
+        G = np.asarray(judge["grads"]) * np.asarray(judge["ranges"]) ** 2
+        met = modelled_error(G, bits - 1) <= (ref * tol_rel) ** 2
         if not met:
-            bits, _ = LagrangianEB(G, ref=ref, tol_rel=tol_rel)
+            bits = np.asarray(judge["bits"])
             fell_back = True
-    return dict(bits=bits + 1, ranges=ranges, seconds=seconds, met=met, fell_back=fell_back, pred=pred)
+    return dict(bits=bits, ranges=ranges, seconds=seconds, met=met, fell_back=fell_back, pred=pred)

@@ -166,7 +166,7 @@ def scheme_search(n=65536, n_grid=256, nsteps=256, checkpoint_every=16, dt=2.5e-
         model = Surrogate.load(wpath)
         search_scheme_surrogate(model, init, 1e4, dt, n_grid, 8, ref=1.0, tol_rel=tol_rel)  # warm-up
         r = search_scheme_surrogate(model, init, 1e4, dt, n_grid, nsteps, ref=1.0, tol_rel=tol_rel,
-                                    judge_g=res["grads"])
+                                    judge=res)
         sur_bits = r["bits"] if not r["fell_back"] else None
         out["surrogate"] = {"seconds": r["seconds"], "speedup": total / r["seconds"],
                             "meets_bound_under_adjoint_sums": bool(r["met"]),

@@ -74,14 +74,14 @@ def test_adjoint_judges_and_falls_back(data):
         ref = search_scheme(lambda: make_solver(N, GRID, E, dt), st, STEPS, tol_rel=1e-3, ref=1.0,
                             checkpoint_every=CK)
         t_adj.append(time.perf_counter() - t0)
-        r = search_scheme_surrogate(model, st, E, dt, GRID, STEPS, ref=1.0, tol_rel=1e-3, judge_g=ref["grads"])
+        r = search_scheme_surrogate(model, st, E, dt, GRID, STEPS, ref=1.0, tol_rel=1e-3, judge=ref)
         t_sur.append(r["seconds"])
-        G_true = ref["grads"] * r["ranges"] ** 2
+        G_true = ref["grads"] * ref["ranges"] ** 2
         assert modelled_error(G_true, r["bits"] - 1) <= 1e-6 * (1 + 1e-9)
         if k == 0:
             model.margin = np.full(4, -3.0)
             miss = search_scheme_surrogate(model, st, E, dt, GRID, STEPS, ref=1.0, tol_rel=1e-3,
-                                           judge_g=ref["grads"])
+                                           judge=ref)
             model.margin = np.zeros(4)
             assert miss["fell_back"] and not miss["met"]
             np.testing.assert_array_equal(miss["bits"], ref["bits"])
@@ -119,6 +119,6 @@ def test_trained_surrogate_on_heldout_scenes():
     ref = search_scheme(lambda: make_solver(65536, 256, E, dt), st, 256, tol_rel=1e-3, ref=1.0, checkpoint_every=16)
     torch.cuda.synchronize()
     t_adj = time.perf_counter() - t0
-    r = search_scheme_surrogate(model, st, E, dt, 256, 256, ref=1.0, tol_rel=1e-3, judge_g=ref["grads"])
+    r = search_scheme_surrogate(model, st, E, dt, 256, 256, ref=1.0, tol_rel=1e-3, judge=ref)
     print("search seconds: adjoint", t_adj, "surrogate", r["seconds"], "met", r["met"])
     assert t_adj >= 5 * r["seconds"]
