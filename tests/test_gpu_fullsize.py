@@ -1,0 +1,149 @@
+"""Size-independent properties at the benchmark's full size: 1,048,576
+particles of the 3D fluid on a 128^3 grid with the 32-byte dithered record
+(the oracle would take minutes here, so these check what does not depend on
+the size):
+
+- the tiled P2G conserves mass exactly (to f32 rounding of the sum) and
+  momentum (the APIC affine terms sum to zero over the stencil), and agrees
+  node by node with the untiled global-atomic P2G;
+- the sort is a permutation that carries the records with their ids, and
+  leaves the particles ordered by grid block;
+- the kernel specialised to the record layout is bit-identical to the
+  generic one;
+- 50 substeps stay physical (finite, inside the domain, J inside its range,
+  mass conserved) and close to the fp32-storage run;
+- dithered encoding is unbiased over a million records.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N = 1 << 20
+
+
+def solver(storage="quant"):
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import autoquantizer_amd as aq
+    from autoquantizer_amd import layouts
+    cfg = aq.MPMConfig()
+    cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.la, cfg.g = 3, True, 128, 2e-4, 400.0, 9.8
+    cfg.p_vol = (1 / 128 * 0.5) ** 3
+    cfg.n_particles = N
+    cfg.seed = 1234
+    if storage == "quant":
+        _, t, g, sh = layouts.fluid_3d()
+        return aq.MPMSolver(cfg, types=t, groups=g, shared=sh)
+    cfg.compute_fp = storage
+    return aq.MPMSolver(cfg)
+
+
+def state(seed=0):
+    rng = np.random.default_rng(seed)
+    v = np.zeros((16, N))
+    v[:3] = rng.uniform(0.1, 0.5, (3, N))
+    v[3:6] = rng.uniform(-1.0, 1.0, (3, N))
+    v[6:15] = rng.uniform(-0.5, 0.5, (9, N))
+    v[15] = rng.uniform(0.95, 1.05, N)
+    return v
+
+
+def test_p2g_conservation_and_tiled_vs_untiled():
+    import torch
+    print("[fullsize] p2g", flush=True)
+    a = solver()
+    a.set_state(state())
+    dec = a.get_state(original_order=False)  # what the records hold
+    a.build_pid()
+    dec_sorted = a.get_state(original_order=False)
+    a.p2g_bls()  # tiled (the sort describes the state)
+    ga = a._grid_view().double().clone()
+    m = ga[:, 3].sum().item()
+    assert m == pytest.approx(N * a.p_mass, rel=1e-5)
+    mom = ga[:, :3].sum(0).cpu().numpy()
+    expect = a.p_mass * dec[3:6].sum(axis=1)
+    scale = a.p_mass * np.abs(dec[3:6]).sum()
+    assert np.abs(mom - expect).max() <= 1e-5 * scale
+    # untiled P2G (no sort) of the same records
+    b = solver()
+    b.set_words(a.get_words())
+    assert np.array_equal(b.get_state(original_order=False), dec_sorted)
+    b.p2g_bls()
+    gb = b._grid_view().double()
+    diff = (ga - gb).abs().max(dim=0).values.cpu().numpy()
+    top = ga.abs().max(dim=0).values.cpu().numpy()
+    assert np.all(diff <= 2e-5 * top + 1e-12), (diff, top)
+    torch.cuda.synchronize()
+
+
+def test_sort_is_a_permutation():
+    import torch
+    print("[fullsize] sort", flush=True)
+    s = solver()
+    s.set_state(state(1))
+    w0 = s.state[:, :N].clone()
+    s.build_pid()
+    ids = s.ids[:N].long()
+    assert torch.equal(torch.sort(ids).values, torch.arange(N, device=ids.device))
+    assert torch.equal(s.state[:, :N], w0[:, ids])
+    x = s.get_state(original_order=False)[:3]
+    base = np.floor(x * 128 - 0.5).astype(np.int64) >> 2  # stencil base's block per axis
+    key = (base[0] * 32 + base[1]) * 32 + base[2]
+    assert np.all(np.diff(key) >= 0)
+
+
+def test_specialised_kernel_bit_identical_at_full_size(tmp_path):
+    import torch
+    print("[fullsize] specialised", flush=True)
+    s = solver()
+    s.set_state(state(2))
+    s.advance(2, sort_every=1)
+    w0, ids0, sc = s.state.clone(), s.ids.clone(), s.step_count
+    s.advance(3, sort_every=0)
+    generic = s.state.clone()
+    s.state.copy_(w0)
+    s.ids.copy_(ids0)
+    s.step_count = sc
+    s.specialize(cache_dir=str(tmp_path))
+    s.advance(3, sort_every=0)
+    assert torch.equal(s.state, generic)
+
+
+def test_fifty_substeps_stay_physical(tmp_path):
+    import torch
+    print("[fullsize] 50 substeps", flush=True)
+    init = state(3)
+    init[3:6] *= 0.5
+    init[6:15] = 0.0
+    init[15] = 1.0
+    q = solver()
+    q.set_state(init)
+    q.specialize(cache_dir=str(tmp_path))
+    q.advance(50, sort_every=16)
+    f = solver("f32")
+    f.set_state(init)
+    f.advance(50, sort_every=16)
+    xq, xf = q.get_state(), f.get_state()
+    assert np.all(np.isfinite(xq))
+    assert xq[:3].min() > 0.0 and xq[:3].max() < 1.0
+    assert 0.5 <= xq[15].min() and xq[15].max() <= 1.5
+    assert np.sqrt(np.mean((xq[:3] - xf[:3]) ** 2)) < 1e-4
+    q.p2g_bls()
+    m = q._grid_view().double()[:, 3].sum().item()
+    assert m == pytest.approx(N * q.p_mass, rel=1e-5)
+    torch.cuda.synchronize()
+
+
+def test_dithered_encoding_is_unbiased():
+    print("[fullsize] dither", flush=True)
+    s = solver()
+    vals = state(4)
+    vals[0] = 0.3 + 0.37 / 2 ** 20  # between two codes of x (fixed21, range 1: step 2^-20)
+    vals[15] = 1.0 + 0.61 * 0.5 / 2 ** 15  # between two codes of J (fixed16, range 0.5)
+    s.set_state(vals)
+    dec = s.get_state(original_order=False)
+    for c, step in ((0, 2.0 ** -20), (15, 0.5 / 2 ** 15)):
+        assert abs(dec[c].mean() - vals[c, 0]) < 5 * (step / 2) / np.sqrt(N)
+        assert len(np.unique(np.round((dec[c] - dec[c].min()) / step))) == 2
