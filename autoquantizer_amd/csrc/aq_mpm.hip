@@ -622,9 +622,10 @@ __global__ void __launch_bounds__(256) k_items_fill(const uint32_t *__restrict__
   if (b >= m) return;
   const uint32_t c = counts[b];
   const int32_t base = item_off[b];
-  for (uint32_t j = 0; j * 256u < c; ++j) {
-    const uint32_t left = c - j * 256u;
-    items[base + (int32_t)j] = make_int4(b, (int)(offsets[b] + j * 256u), (int)(left < 256u ? left : 256u), 0);
+  for (uint32_t j = 0; j * (uint32_t)kItem < c; ++j) {
+    const uint32_t left = c - j * (uint32_t)kItem;
+    items[base + (int32_t)j] =
+        make_int4(b, (int)(offsets[b] + j * (uint32_t)kItem), (int)(left < (uint32_t)kItem ? left : (uint32_t)kItem), 0);
   }
 }
 
@@ -693,9 +694,12 @@ __global__ void __launch_bounds__(256) k_sort_keys(StateK s, CodecK cod, ParamsT
 }
 
 // Exclusive scans of counts[0..m) into off[0..m] and of the work items per
-// block, ceil(counts / 256), into ioff[0..m] (totals at [m]): per-chunk scans
+// block, ceil(counts / kItem), into ioff[0..m] (totals at [m]): per-chunk scans
 // of 4096 (k_scan_local), a scan of the chunk totals (k_scan_top) and the
 // chunk prefix added back (k_scan_add).
+// Work items of a block with c particles (k_items_fill writes exactly these).
+__device__ __forceinline__ uint32_t items_of(uint32_t c) { return (c + (uint32_t)kItem - 1u) / (uint32_t)kItem; }
+
 __global__ void __launch_bounds__(256) k_scan_local(const uint32_t *__restrict__ counts, uint32_t *__restrict__ off,
                                                     int32_t *__restrict__ ioff, uint32_t *__restrict__ part,
                                                     int32_t m) {
@@ -709,7 +713,7 @@ __global__ void __launch_bounds__(256) k_scan_local(const uint32_t *__restrict__
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     a += c[i];
-    b += (c[i] + 255u) >> 8;
+    b += items_of(c[i]);
   }
   sa[t] = a;
   sb[t] = b;
@@ -729,7 +733,7 @@ __global__ void __launch_bounds__(256) k_scan_local(const uint32_t *__restrict__
       ioff[base + i] = (int32_t)rb;
     }
     ra += c[i];
-    rb += (c[i] + 255u) >> 8;
+    rb += items_of(c[i]);
   }
   if (t == 255) {
     part[2 * blockIdx.x] = sa[255];
@@ -966,12 +970,12 @@ static int64_t blocks_of(const aq_sim_desc *d) {
   return (d->dim == 3 ? n * n * n : n * n) / 64;
 }
 
-// Work items of a sort: at most ceil(count_b / 256) per block, so at most
-// capacity / 256 plus one per non-empty block.
+// Work items of a sort: at most ceil(count_b / kItem) per block, so at most
+// capacity / kItem plus one per non-empty block.
 static int32_t max_items_of(const aq_sim_desc *d) {
   const int64_t nb = blocks_of(d);
   const int64_t ne = d->capacity < nb ? d->capacity : nb;
-  return (int32_t)(d->capacity / 256 + ne + 1);
+  return (int32_t)(d->capacity / kItem + ne + 1);
 }
 
 static int64_t tile_slots(const aq_sim_desc *d) { return d->dim == 3 ? 512 : 144; }  // Tile<DIM>::TN
@@ -1230,13 +1234,13 @@ static int jit_source(const aq_sim &v, std::string &src) {
                 c.shared_mask, c.gfirst_mask, c.glast_mask, v.lay.nw, k.g2p_mask);
   src += buf;
   std::snprintf(buf, sizeof buf,
-                "extern \"C\" __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(%d)))\n"
+                "extern \"C\" __global__ void __launch_bounds__(%d) __attribute__((amdgpu_waves_per_eu(%d)))\n"
                 "aq_jit_tiled(aq::StateK s, aq::CodecK cod, aq::ParamsT<float> P, int64_t n, const int4 *items,\n"
                 "             const int32_t *nitems, const float4 *grid_v, float4 *tiles, float4 *grid_f,\n"
                 "             uint32_t *fflags) {\n"
                 "  aq::tiled_body<%d, %d, AQ_STORE_QUANT, %d, float, true, true>(s, cod, P, n, items, nitems, grid_v,\n"
                 "                                                               tiles, grid_f, fflags);\n}\n",
-                kJitWaves, v.d.dim, v.d.material, nw32_bucket(v.d.nwords));
+                kItem, kJitWaves, v.d.dim, v.d.material, nw32_bucket(v.d.nwords));
   src += buf;
   return AQ_OK;
 }
@@ -1406,7 +1410,7 @@ static unsigned pblocks(int64_t n) { return (unsigned)((n + 255) / 256); }
 static unsigned gblocks(const aq_sim *s) { return (unsigned)((s->nblocks + 3) / 4); }
 // persistent grid of the tiled kernels: a multiple of 8 (one share per XCD)
 static unsigned tblocks(const aq_sim *s, int64_t n) {
-  int64_t want = (n + 255) / 256 + 64;
+  int64_t want = (n + kItem - 1) / kItem + 64;
   if (want > 4096) want = 4096;
   return (unsigned)((want + 7) / 8 * 8);
 }
@@ -1452,7 +1456,7 @@ static int launch_p2g(aq_sim *s, const void *state, int64_t n, bool *tiled, hipS
   dispatch(s, [&](auto tag_) {
     TAG_TYPES(tag_)
     if (*tiled)
-      hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, false, true>), dim3(tblocks(s, n)), dim3(256), 0, st, sk,
+      hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, false, true>), dim3(tblocks(s, n)), dim3(kItem), 0, st, sk,
                          cod, params_of<T>(s), n, s->items, s->item_off + s->nblocks, (const vec4_t<T> *)nullptr,
                          (vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags);
     else
@@ -1471,7 +1475,7 @@ static int launch_g2p(aq_sim *s, void *state, int64_t n, const int32_t *ids, uin
   dispatch(s, [&](auto tag_) {
     TAG_TYPES(tag_)
     if (tiled)
-      hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, true, false>), dim3(tblocks(s, n)), dim3(256), 0, st, sk,
+      hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, true, false>), dim3(tblocks(s, n)), dim3(kItem), 0, st, sk,
                          cod, params_of<T>(s), n, s->items, s->item_off + s->nblocks,
                          (const vec4_t<T> *)s->grid_v, (vec4_t<T> *)nullptr, (vec4_t<T> *)nullptr,
                          (uint32_t *)nullptr);
@@ -1498,13 +1502,13 @@ static int launch_g2p2g(aq_sim *s, void *state, int64_t n, const int32_t *ids, u
     void *tl = s->tiles, *gf = s->grid_f;
     uint32_t *ff = s->fflags;
     void *args[] = {&sk, &cod, &P, &n, &items, &nitems, &gv, &tl, &gf, &ff};
-    HIP_TRY(hipModuleLaunchKernel(s->jit_fn, tblocks(s, n), 1, 1, 256, 1, 1, 0, st, args, nullptr));
+    HIP_TRY(hipModuleLaunchKernel(s->jit_fn, tblocks(s, n), 1, 1, kItem, 1, 1, 0, st, args, nullptr));
     return AQ_OK;
   }
   dispatch(s, [&](auto tag_) {
     TAG_TYPES(tag_)
     if (*tiled)
-      hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, true, true>), dim3(tblocks(s, n)), dim3(256), 0, st, sk,
+      hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, true, true>), dim3(tblocks(s, n)), dim3(kItem), 0, st, sk,
                          cod, params_of<T>(s), n, s->items, s->item_off + s->nblocks,
                          (const vec4_t<T> *)s->grid_v, (vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags);
     else
@@ -1589,7 +1593,7 @@ extern "C" int aq_sort(aq_sim *s, void *state, int64_t n, int32_t *ids, void *st
   HIP_TRY(hipMemcpy2DAsync(state, (size_t)cap * esz, s->tmp_state, (size_t)cap * esz, (size_t)n * esz,
                            (size_t)planes, hipMemcpyDeviceToDevice, st));
   if (ids) HIP_TRY(hipMemcpyAsync(ids, s->tmp_ids, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
-  // work items: up to 256 particles of one block each
+  // work items: up to kItem particles of one block each
   hipLaunchKernelGGL(k_items_fill, dim3((m + 255) / 256), dim3(256), 0, st, s->scount, s->soff, s->item_off,
                      s->items, m);
   LAUNCH_CHECK();

@@ -8,7 +8,7 @@
 namespace aq {
 
 // ---- LDS-tiled kernels over work items ---------------------------------------
-// A work item is up to 256 particles whose stencil base was in one grid block
+// A work item is up to kItem particles whose stencil base was in one grid block
 // at the last aq_sort: int4 (block, first particle, count, 0).  The item's
 // tile covers the block, the 2-node stencil reach and a 1-cell drift margin
 // on each side: TE = E + 4 nodes per axis from block_origin - 1 (LDS layout
@@ -25,6 +25,12 @@ namespace aq {
 // particle whose stencil leaves the tile (it drifted more than a cell since
 // the sort), or an item with a non-finite bound, reads grid_v and scatters
 // into grid_f.
+// Particles per work item = threads of a tiled workgroup (4 wavefronts).
+// 512 (8 wavefronts) halves the tiles and made the gather 7.5 µs faster on
+// the 3D fluid benchmark, but the fused kernel 6-18 µs slower (fewer
+// workgroups between barriers).
+constexpr int kItem = 256;
+
 template <int DIM> struct Tile {
   static constexpr int E = BlockShape<DIM>::E, LG = BlockShape<DIM>::LOG;
   static constexpr int TE = E + 4;                                // nodes per axis
@@ -149,8 +155,9 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
   __shared__ vec4_t<T> tv[DO_G2P ? TS : 1];
   constexpr int NP = ACC::PLANES;
   __shared__ unsigned long long ts[DO_P2G ? NP : 1][DO_P2G ? TS : 1];
-  __shared__ T bpart[4][4];
-  __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
+  constexpr int NWAVES = kItem / 64;
+  __shared__ T bpart[NWAVES][4];
+  __shared__ uint32_t rows[kItem * RowsOf<ST, NW32>::RS];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
 #ifdef AQ_JIT_LAYOUT
   constexpr uint32_t g2p_mask = jit::kG2PMask;
@@ -159,7 +166,7 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
 #endif
   uint32_t *row = rows + tid * RowsOf<ST, NW32>::RS;
   if constexpr (DO_P2G)
-    for (int q = tid; q < TS; q += 256)
+    for (int q = tid; q < TS; q += kItem)
 #pragma unroll
       for (int a = 0; a < NP; ++a) ts[a][q] = 0ull;
   const int ni = *nitems;
@@ -185,7 +192,7 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
         fetch_state<DIM, ST, NW32, NC, T>(s, p, 0xFFFFFFFFu, fw, v);
     }
     if constexpr (DO_G2P) {
-      for (int nd = tid; nd < TN; nd += 256) {
+      for (int nd = tid; nd < TN; nd += kItem) {
         int t[3];
         TL::node_coords(nd, t);
         const int gi = o[0] + t[0], gj = o[1] + t[1], gk = o[2] + t[2];
@@ -250,7 +257,9 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
       bool finite = true;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const T sum = ((bpart[0][c] + bpart[1][c]) + bpart[2][c]) + bpart[3][c];
+        T sum = bpart[0][c];
+#pragma unroll
+        for (int w = 1; w < NWAVES; ++w) sum += bpart[w][c];
         finite = finite && sum <= (T)1.0e37;
         k[c] = ACC::BITS - texp<T>(sum * ((T)1 + (T)1 / (T)1024));
       }
@@ -280,7 +289,7 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
       }
       __syncthreads();  // tile sums complete
       vec4_t<T> *dst = tiles + (int64_t)it * TN;
-      for (int nd = tid; nd < TN; nd += 256) {
+      for (int nd = tid; nd < TN; nd += kItem) {
         int t[3];
         TL::node_coords(nd, t);
         const int q = TL::slot(t[0], t[1], t[2]);
@@ -310,7 +319,7 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
 }
 
 template <int DIM, int MAT, int ST, int NW32, class T, bool DO_G2P, bool DO_P2G>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TiledWaves<ST>::W)))
+__global__ void __launch_bounds__(kItem) __attribute__((amdgpu_waves_per_eu(TiledWaves<ST>::W)))
 k_tiled(StateK s, CodecK cod, ParamsT<T> P, int64_t n, const int4 *__restrict__ items,
         const int32_t *__restrict__ nitems, const vec4_t<T> *__restrict__ grid_v, vec4_t<T> *__restrict__ tiles,
         vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags) {
