@@ -6,7 +6,15 @@ dt = 2e-4, gravity 9.8, separating walls 3 cells thick; state stored as the
 32-byte quantized record of autoquantizer_amd.layouts.fluid_3d (x 3 x fixed21,
 v 3 x fixed16, C 9 x shared-exponent float(8, 10), J fixed16, dithered),
 f32 compute.  A step is one substep of all particles (P2G, grid update, G2P;
-G2P fused with the next P2G).  Printed: one JSON line.
+G2P fused with the next P2G); the fused kernel is specialised to the record
+layout at run time (--generic-kernel for the descriptor-driven one); the
+same scene with fp32 storage is timed beside it.  Printed: one JSON line,
+with the roofline entry of the fused kernel, the error of both runs against
+an FP64 run, the CPU baseline (the C oracle on the host's cores), the scheme
+search on the adjoint scene (adjoint path and learned surrogate; headline
+`scheme_search_s` is the surrogate's, plus the adjoint's when the adjoint
+rejects its allocation), the adjoint check against finite differences, and
+the batched scheme search.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -390,7 +398,15 @@ def main():
     if not args.no_search:
         try:
             ss = scheme_search()
-            out["scheme_search_s"] = ss["seconds"]
+            # the search as shipped: the surrogate's allocation, with the adjoint search's time added
+            # when the adjoint (run here as the judge) rejects it
+            sur = ss.get("surrogate")
+            if sur is None:
+                out["scheme_search_s"] = ss["seconds"]
+            else:
+                out["scheme_search_s"] = sur["seconds"] + (0.0 if sur["meets_bound_under_adjoint_sums"]
+                                                           else ss["seconds"])
+            out["scheme_search_adjoint_s"] = ss["seconds"]
             out["scheme_search"] = ss
         except Exception as e:  # reported, not hidden: the line still carries the throughput
             out["scheme_search"] = {"error": f"{type(e).__name__}: {e}"}
