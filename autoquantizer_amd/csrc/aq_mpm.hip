@@ -496,6 +496,7 @@ template <int DIM> __device__ __forceinline__ void lane_coords(int lane, int (&l
 // inactive in `act` (and zeroed if they were active).
 template <int DIM, class T, bool OP>
 __global__ void __launch_bounds__(256) k_gather(ParamsT<T> P, int32_t nblocks, const int32_t *__restrict__ item_off,
+                                                const uint32_t *__restrict__ near,
                                                 const vec4_t<T> *__restrict__ tiles, vec4_t<T> *__restrict__ grid_f,
                                                 uint32_t *__restrict__ fflags, vec4_t<T> *__restrict__ grid_v,
                                                 uint32_t *__restrict__ act) {
@@ -504,11 +505,23 @@ __global__ void __launch_bounds__(256) k_gather(ParamsT<T> P, int32_t nblocks, c
   const int32_t b = __builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * 4u + (threadIdx.x >> 6)));
   const int lane = threadIdx.x & 63;
   if (b >= nblocks) return;
+  const bool fall = fflags[b] != 0u;
+  // a block with no items in its neighbourhood and no untiled scatter has
+  // nothing to gather: skip the item ranges of its neighbours
+  if (item_off && !near[b] && !fall) {
+    if (act[b]) {
+      vec4_t<T> z;
+      z.x = z.y = z.z = z.w = (T)0;
+      grid_v[(int64_t)b * 64 + lane] = z;
+      __builtin_amdgcn_wave_barrier();
+      if (lane == 0) act[b] = 0u;
+    }
+    return;
+  }
   const int nb = P.nb;
   int bc[3], loc[3];
   block_coords<DIM>(b, nb, bc);
   lane_coords<DIM>(lane, loc);
-  const bool fall = fflags[b] != 0u;
   int32_t r0[NN], r1[NN], q[NN];
 #pragma unroll
   for (int k = 0; k < NN; ++k) {
@@ -627,6 +640,27 @@ __global__ void __launch_bounds__(256) k_items_fill(const uint32_t *__restrict__
     items[base + (int32_t)j] =
         make_int4(b, (int)(offsets[b] + j * (uint32_t)kItem), (int)(left < (uint32_t)kItem ? left : (uint32_t)kItem), 0);
   }
+}
+
+// near[b]: some block within one block of b (per axis) holds particles, so
+// the gather of b may find tiles (the neighbours whose tiles can cover b).
+template <int DIM>
+__global__ void __launch_bounds__(256) k_near(const uint32_t *__restrict__ counts, int32_t nb,
+                                              uint32_t *__restrict__ near, int32_t m) {
+  const int32_t b = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (b >= m) return;
+  int bc[3];
+  block_coords<DIM>(b, nb, bc);
+  uint32_t any = 0u;
+  for (int dx = -1; dx <= 1; ++dx)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz) {
+        const int x = bc[0] + dx, y = bc[1] + dy, z = bc[2] + dz;
+        if (x < 0 || x >= nb || y < 0 || y >= nb || (DIM == 3 && (z < 0 || z >= nb))) continue;
+        const int32_t c = DIM == 3 ? (x * nb + y) * nb + z : x * nb + y;
+        any |= counts[c];
+      }
+  near[b] = any != 0u ? 1u : 0u;
 }
 
 __global__ void k_fill_u32(uint32_t *a, uint32_t v, int64_t n) {
@@ -913,6 +947,7 @@ struct aq_sim {
   int32_t *tmp_ids;
   int4 *items;
   int32_t *item_off;  // [nblocks + 1]; item_off[nblocks] = item count
+  uint32_t *near;     // [nblocks]: particles within one block (k_near, at each sort)
   int32_t max_items;
   bool have_items;    // items describe the current order of `items_state`
   const void *items_state;
@@ -982,7 +1017,7 @@ static int64_t tile_slots(const aq_sim_desc *d) { return d->dim == 3 ? 512 : 144
 
 enum {
   WS_GRID_V, WS_GRID_F, WS_FLAGS, WS_KEY, WS_RANK, WS_COUNT, WS_OFF, WS_PART, WS_TMP_STATE, WS_TMP_IDS,
-  WS_ITEMS, WS_ITEM_OFF, WS_TILES, WS_LAYOUT, WS_N
+  WS_ITEMS, WS_ITEM_OFF, WS_NEAR, WS_TILES, WS_LAYOUT, WS_N
 };
 
 static void ws_plan(const aq_sim_desc *d, int64_t *offs /*[WS_N]*/, int64_t *total) {
@@ -1006,6 +1041,7 @@ static void ws_plan(const aq_sim_desc *d, int64_t *offs /*[WS_N]*/, int64_t *tot
   sz[WS_TMP_IDS] = cap * 4;
   sz[WS_ITEMS] = (int64_t)max_items_of(d) * 16;
   sz[WS_ITEM_OFF] = (nblocks + 1) * 4;
+  sz[WS_NEAR] = nblocks * 4;
   sz[WS_TILES] = (int64_t)max_items_of(d) * tile_slots(d) * node_bytes;
   sz[WS_LAYOUT] = AQ_MAX_COMPS * (int64_t)sizeof(CompK);
   int64_t o = 0;
@@ -1088,6 +1124,7 @@ extern "C" int aq_sim_create(const aq_sim_desc *d, void *ws, int64_t ws_bytes, a
   s->items = (int4 *)(base + offs[WS_ITEMS]);
   s->item_off = (int32_t *)(base + offs[WS_ITEM_OFF]);
   s->tiles = base + offs[WS_TILES];
+  s->near = (uint32_t *)(base + offs[WS_NEAR]);
   s->lay_dev = (CompK *)(base + offs[WS_LAYOUT]);
   s->max_items = max_items_of(d);
   s->have_items = false;
@@ -1436,11 +1473,11 @@ static int launch_gather(aq_sim *s, bool tiled, bool op, hipStream_t st) {
     (void)NW32;
     if (op)
       hipLaunchKernelGGL((k_gather<DIM, T, true>), dim3(gblocks(s)), dim3(256), 0, st, params_of<T>(s), s->nblocks,
-                         ioff, (const vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags,
+                         ioff, s->near, (const vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags,
                          (vec4_t<T> *)s->grid_v, s->act);
     else
       hipLaunchKernelGGL((k_gather<DIM, T, false>), dim3(gblocks(s)), dim3(256), 0, st, params_of<T>(s), s->nblocks,
-                         ioff, (const vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags,
+                         ioff, s->near, (const vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags,
                          (vec4_t<T> *)s->grid_v, s->act);
   });
   LAUNCH_CHECK();
@@ -1596,6 +1633,10 @@ extern "C" int aq_sort(aq_sim *s, void *state, int64_t n, int32_t *ids, void *st
   // work items: up to kItem particles of one block each
   hipLaunchKernelGGL(k_items_fill, dim3((m + 255) / 256), dim3(256), 0, st, s->scount, s->soff, s->item_off,
                      s->items, m);
+  if (s->d.dim == 3)
+    hipLaunchKernelGGL(k_near<3>, dim3((m + 255) / 256), dim3(256), 0, st, s->scount, (int32_t)s->pf.nb, s->near, m);
+  else
+    hipLaunchKernelGGL(k_near<2>, dim3((m + 255) / 256), dim3(256), 0, st, s->scount, (int32_t)s->pf.nb, s->near, m);
   LAUNCH_CHECK();
   s->have_items = true;
   s->items_state = state;
