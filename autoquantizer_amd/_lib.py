@@ -84,7 +84,7 @@ EXPORTS = [
     "aq_last_error", "aq_version", "aq_layout_plan", "aq_pack", "aq_unpack",
     "aq_store_members", "aq_sim_workspace_bytes", "aq_sim_create", "aq_sim_destroy",
     "aq_substep", "aq_p2g", "aq_grid_op", "aq_g2p", "aq_advance", "aq_sort",
-    "aq_update_limits", "aq_energy", "aq_sim_grid", "aq_advance_timed",
+    "aq_update_limits", "aq_advance_limits", "aq_energy", "aq_sim_grid", "aq_advance_timed",
     "aq_adj_workspace_bytes", "aq_adj_substep", "aq_noise_add", "aq_noise_dot", "aq_mlp_forward",
     "aq_mlp_backward", "aq_adam_step", "aq_sim_specialize", "aq_jit_build",
 ]
@@ -121,6 +121,7 @@ def lib():
     L.aq_advance.argtypes = [P, P, i64, i32, u32, i32, P, P]
     L.aq_sort.argtypes = [P, P, i64, P, P]
     L.aq_update_limits.argtypes = [P, P, i64, P, P]
+    L.aq_advance_limits.argtypes = [P, P, i64, i32, u32, i32, P, P, P]
     L.aq_energy.argtypes = [P, P, i64, P, P]
     L.aq_sim_grid.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(i64)]
     L.aq_advance_timed.argtypes = [P, P, i64, i32, u32, i32, P, P, P, P, P]

@@ -79,7 +79,24 @@ class Noise:
                                            _stream()))
 
 
+def _sorted_copy(solver, state):
+    """A copy of `state` sorted by grid block (aq_sort), so that the tiled
+    kernels run on it, and the original index of each sorted slot."""
+    n = solver.current_idx
+    w = state.clone()
+    ids = torch.arange(w.shape[1], dtype=torch.int32, device=w.device)
+    _lib.check(_lib.lib().aq_sort(solver._h, _ptr(w), n, _ptr(ids), _stream()))
+    return w, ids[:n].long()
+
+
 class _Schedule:
+    """Checkpointed forward and reverse passes.  Without noise each segment
+    runs on a sorted copy of its checkpoint (LDS-tiled kernels, like the
+    forward solver); states, checkpoints and adjoints at segment boundaries
+    stay in the original particle order, and the adjoint is carried through
+    the segment's permutation.  With noise (keyed by particle slot) the
+    segments run unsorted."""
+
     def __init__(self, solver, nsteps, checkpoint_every, noise=None):
         if checkpoint_every < 1:
             raise ValueError("checkpoint_every must be >= 1")
@@ -87,6 +104,7 @@ class _Schedule:
             raise ValueError("sigma needs one entry per component")
         self.solver, self.nsteps, self.k = solver, int(nsteps), int(checkpoint_every)
         self.noise = noise
+        self.sorted = noise is None
         self.checkpoints = []
 
     def _advance(self, s, t0, nsteps):
@@ -101,9 +119,16 @@ class _Schedule:
     def forward(self, state0):
         """Runs the forward from state0 ([ncomp, capacity]); returns s_T."""
         s = state0.clone()
+        n = self.solver.current_idx
         for t0 in range(0, self.nsteps, self.k):
             self.checkpoints.append(s.clone())
-            self._advance(s, t0, min(self.k, self.nsteps - t0))
+            steps = min(self.k, self.nsteps - t0)
+            if self.sorted:
+                w, idx = _sorted_copy(self.solver, s)
+                self._advance(w, t0, steps)
+                s[:, idx] = w[:, :n]
+            else:
+                self._advance(s, t0, steps)
         return s
 
     def backward(self, adj_T, gsum=None, adj=None, noise_dot=None):
@@ -112,19 +137,37 @@ class _Schedule:
         noise_dot (dL/dsigma)."""
         a = adj_T.clone()
         b = torch.empty_like(a)
+        n = self.solver.current_idx
         for j in reversed(range(len(self.checkpoints))):
             t0 = j * self.k
             t1 = min(t0 + self.k, self.nsteps)
-            states = [self.checkpoints[j]]
-            for t in range(t0 + 1, t1):
-                s = states[-1].clone()
-                self._advance(s, t - 1, 1)
-                states.append(s)
+            if self.sorted:
+                # the segment on a sorted copy: its states are kept, and each is copied back into the
+                # sorted buffer (the one the sort's work items describe) for its adjoint step
+                w, idx = _sorted_copy(self.solver, self.checkpoints[j])
+                states = [w.clone()]
+                for t in range(t0 + 1, t1):
+                    self._advance(w, t - 1, 1)
+                    states.append(w.clone())
+                a[:, :n] = a[:, idx]
+            else:
+                states = [self.checkpoints[j]]
+                for t in range(t0 + 1, t1):
+                    s = states[-1].clone()
+                    self._advance(s, t - 1, 1)
+                    states.append(s)
             for t in reversed(range(t0, t1)):
                 if noise_dot is not None:
                     self.noise.dot(self.solver, a, t + 1, noise_dot)
-                adj.step(states[t - t0], a, b, gsum if t >= 1 else None)
+                if self.sorted:
+                    w.copy_(states[t - t0])
+                    adj.step(w, a, b, gsum if t >= 1 else None)
+                else:
+                    adj.step(states[t - t0], a, b, gsum if t >= 1 else None)
                 a, b = b, a
+            if self.sorted:
+                b[:, idx] = a[:, :n]
+                a[:, :n] = b[:, :n]
         return a
 
 

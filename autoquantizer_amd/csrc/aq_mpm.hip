@@ -1715,8 +1715,11 @@ struct Timer {
 // nsteps substeps: [sort] P2G, gather+grid update, then per substep the
 // fused G2P->P2G and gather+grid update ([sort] every sort_every substeps),
 // and a last G2P.
+static int launch_limits(aq_sim *s, const void *state, int64_t n, float *mxmn, hipStream_t st);
+
+// mxmn (optional): running per-component max / min of the state after every substep.
 static int advance_impl(aq_sim *s, void *state, int64_t n, int nsteps, uint32_t step0, int sort_every,
-                        int32_t *ids, hipStream_t st, Timer *tm) {
+                        int32_t *ids, hipStream_t st, Timer *tm, float *mxmn = nullptr) {
   int rc = check_n(s, state, n);
   if (rc) return rc;
   if (nsteps < 0) return fail(AQ_ERR_INVALID, "negative step count");
@@ -1737,11 +1740,20 @@ static int advance_impl(aq_sim *s, void *state, int64_t n, int nsteps, uint32_t 
     }
     TIMED(tm, Timer::FUSED, rc = launch_g2p2g(s, state, n, ids, step0 + (uint32_t)t, &tiled, st));
     if (rc) return rc;
+    if (mxmn && (rc = launch_limits(s, state, n, mxmn, st))) return rc;
     TIMED(tm, Timer::GRID, rc = launch_gather(s, tiled, true, st));
     if (rc) return rc;
   }
   TIMED(tm, Timer::ENDS, rc = launch_g2p(s, state, n, ids, step0 + (uint32_t)(nsteps - 1), st));
+  if (rc) return rc;
+  if (mxmn) rc = launch_limits(s, state, n, mxmn, st);
   return rc;
+}
+
+extern "C" int aq_advance_limits(aq_sim *s, void *state, int64_t n, int nsteps, uint32_t step0, int sort_every,
+                                 int32_t *ids, float *mxmn, void *stream) {
+  if (!mxmn) return fail(AQ_ERR_INVALID, "null output");
+  return advance_impl(s, state, n, nsteps, step0, sort_every, ids, (hipStream_t)stream, nullptr, mxmn);
 }
 
 extern "C" int aq_advance(aq_sim *s, void *state, int64_t n, int nsteps, uint32_t step0, int sort_every,
@@ -1784,8 +1796,11 @@ extern "C" int aq_update_limits(aq_sim *s, const void *state, int64_t n, float *
   int rc = check_n(s, state, n);
   if (rc) return rc;
   if (!mxmn) return fail(AQ_ERR_INVALID, "null output");
+  return launch_limits(s, state, n, mxmn, (hipStream_t)stream);
+}
+
+static int launch_limits(aq_sim *s, const void *state, int64_t n, float *mxmn, hipStream_t st) {
   if (n == 0) return AQ_OK;
-  hipStream_t st = (hipStream_t)stream;
   StateK sk = make_state(s, state, nullptr);
   CodecK cod = make_codec(s, 0u);
   dispatch(s, [&](auto tag_) {
@@ -1933,16 +1948,11 @@ extern "C" int aq_adj_substep(aq_sim *s, const void *state, int64_t n, const voi
   hipStream_t st = (hipStream_t)stream;
   void *pre = ws;
   void *gbar = (char *)ws + s->nodes * s->node_bytes;
-  // forward recompute of s_t's grid: untiled P2G, gather, copy of (m v, m), grid update
-  StateK sk = make_state(s, state, nullptr);
-  CodecK cod = make_codec(s, 0u);
-  dispatch(s, [&](auto tag_) {
-    TAG_TYPES(tag_)
-    hipLaunchKernelGGL((k_p2g<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, cod,
-                       params_of<T>(s), n, (vec4_t<T> *)s->grid_f, s->fflags);
-  });
-  LAUNCH_CHECK();
-  if ((rc = launch_gather(s, false, false, st))) return rc;
+  // forward recompute of s_t's grid: P2G (tiled when the last sort describes
+  // `state`), gather, copy of (m v, m), grid update
+  bool tiled;
+  if ((rc = launch_p2g(s, state, n, &tiled, st))) return rc;
+  if ((rc = launch_gather(s, tiled, false, st))) return rc;
   HIP_TRY(hipMemcpyAsync(pre, s->grid_v, (size_t)(s->nodes * s->node_bytes), hipMemcpyDeviceToDevice, st));
   if ((rc = aq_grid_op(s, stream))) return rc;
   const int64_t cap = s->d.capacity;

@@ -420,6 +420,13 @@ class MPMSolver:
         _lib.check(_lib.lib().aq_update_limits(self._h, _ptr(self.state), self.current_idx,
                                                _ptr(self._mxmn), _stream()))
 
+    def advance_detect(self, nsteps, sort_every=0):
+        """nsteps substeps (as advance) with detect_bound_kernel after each."""
+        _lib.check(_lib.lib().aq_advance_limits(self._h, _ptr(self.state), self.current_idx, int(nsteps),
+                                                self.step_count, int(sort_every), _ptr(self.ids), _ptr(self._mxmn),
+                                                _stream()))
+        self.step_count += int(nsteps)
+
     def detect_bound(self):
         m = self._mxmn.cpu().numpy().astype(np.float64)
         self.limits = np.maximum(np.abs(m[:self.ncomp]), np.abs(m[self.ncomp:]))

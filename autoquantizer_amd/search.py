@@ -21,9 +21,7 @@ def detect_ranges(solver, nsteps, sort_every=8):
     substeps (detect_bound_kernel after every substep), in get_limits order
     (x entries set to 2 as the reference does).  Advances the solver."""
     solver.init_limits()
-    for t in range(int(nsteps)):
-        solver.advance(1, sort_every=sort_every if t % max(sort_every, 1) == 0 else 0)
-        solver.detect_bound_kernel()
+    solver.advance_detect(int(nsteps), sort_every=sort_every)
     return solver.get_limits()
 
 

@@ -226,6 +226,11 @@ int aq_sort(aq_sim *sim, void *state, int64_t n, int32_t *ids, void *stream);
    replaces update_limits_* (mpm_solver.py:960-979). */
 int aq_update_limits(aq_sim *sim, const void *state, int64_t n, float *mx_mn,
                      void *stream);
+/* aq_advance with aq_update_limits after every substep (the range run of
+   run_forward_detect_ranges, run_mpm_elastic_2d.py:174-184: detect_bound
+   after each substep), in one call. */
+int aq_advance_limits(aq_sim *sim, void *state, int64_t n, int nsteps, uint32_t step0, int sort_every,
+                      int32_t *ids, float *mx_mn, void *stream);
 /* compute_ek (mpm_solver.py:1043-1052) reduced in f64 into *out (device). */
 int aq_energy(aq_sim *sim, const void *state, int64_t n, double *out, void *stream);
 
@@ -243,8 +248,9 @@ int64_t aq_adj_workspace_bytes(const aq_sim *sim);
 /* One reverse substep on plain storage (AQ_STORE_F32 or AQ_STORE_F64):
    adj_out = dL/ds_t from adj_next = dL/ds_{t+1}, for the state s_t in
    `state` (value planes, stride capacity; adj_next and adj_out alike).
-   Recomputes s_t's P2G and grid update (untiled), then runs the G2P, grid
-   and P2G adjoint kernels.  If gsum (device, double[ncomp]) is not NULL,
+   Recomputes s_t's P2G (LDS-tiled when the last aq_sort of `sim` sorted
+   this buffer, else untiled) and grid update, then runs the G2P, grid and
+   P2G adjoint kernels.  If gsum (device, double[ncomp]) is not NULL,
    sum_p adj_out[c][p]^2 is added to gsum[c] (the reference's accu_grad). */
 int aq_adj_substep(aq_sim *sim, const void *state, int64_t n, const void *adj_next, void *adj_out,
                    double *gsum, void *workspace, int64_t workspace_bytes, void *stream);

@@ -356,3 +356,22 @@ def test_limits_and_energy(orc):
     assert lim == approx(np.maximum(np.abs(st.max(axis=1)), np.abs(st.min(axis=1))), rel=1e-6)
     e = sim.compute_ek()
     assert e == approx(orc.energy(s, st.astype(np.float32)), rel=1e-6)
+
+
+@pytest.mark.parametrize("storage", ["f32", "quant"])
+def test_range_run_in_one_call(orc, storage):
+    """advance_detect (aq_advance_limits: the limits after every substep
+    inside one advance) against one substep per call with detect_bound_kernel
+    after each: the same per-component ranges (to the rounding level at which
+    the two schedules' tiled sums differ)."""
+    a, _ = scene(orc, "fluid_3d", 6000, seed=13, storage=storage)
+    b, _ = scene(orc, "fluid_3d", 6000, seed=13, storage=storage)
+    a.init_limits()
+    a.advance_detect(40, sort_every=8)
+    b.init_limits()
+    for t in range(40):
+        b.advance(1, sort_every=8 if t % 8 == 0 else 0)
+        b.detect_bound_kernel()
+    la, lb = a.get_limits(), b.get_limits()
+    assert np.all(np.isfinite(la))
+    np.testing.assert_allclose(la, lb, rtol=1e-3)
