@@ -836,32 +836,29 @@ __global__ void __launch_bounds__(256) k_sort_scatter(const E *__restrict__ src,
 }
 
 // ---- reductions ----------------------------------------------------------------
-__device__ __forceinline__ void atomic_max_f32(float *a, float v) {
-  if (v >= 0.0f) atomicMax((int *)a, __float_as_int(v));
-  else atomicMin((unsigned int *)a, __float_as_uint(v));
-}
-__device__ __forceinline__ void atomic_min_f32(float *a, float v) {
-  if (v >= 0.0f) atomicMin((int *)a, __float_as_int(v));
-  else atomicMax((unsigned int *)a, __float_as_uint(v));
-}
+// Per-component max / min of the state: grid-stride blocks write their
+// partial max / min to part[block][2 NC] (no atomics: many blocks updating
+// the same few addresses serialise); k_limits_final folds them into mxmn.
+constexpr int kLimitBlocks = 512;
 
 template <int DIM, int MAT, int ST, int NW32, class T>
-__global__ void __launch_bounds__(256) k_limits(StateK s, CodecK cod, int64_t n, float *__restrict__ mxmn) {
+__global__ void __launch_bounds__(256) k_limits(StateK s, CodecK cod, int64_t n, float *__restrict__ part) {
   constexpr int NC = Comps<DIM, MAT>::NC;
-  __shared__ float smx[NC][8], smn[NC][8];
+  __shared__ float smx[NC][4], smn[NC][4];
   __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  T v[NC];
   float mx[NC], mn[NC];
-  if (p < n) {
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    mx[c] = -INFINITY;
+    mn[c] = INFINITY;
+  }
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+    T v[NC];
     load_state<DIM, ST, NW32, NC, T>(s, cod, p, 0xFFFFFFFFu, rows + threadIdx.x * RowsOf<ST, NW32>::RS, v);
 #pragma unroll
-    for (int c = 0; c < NC; ++c) mx[c] = mn[c] = (float)v[c];
-  } else {
-#pragma unroll
     for (int c = 0; c < NC; ++c) {
-      mx[c] = -INFINITY;
-      mn[c] = INFINITY;
+      mx[c] = fmaxf(mx[c], (float)v[c]);
+      mn[c] = fminf(mn[c], (float)v[c]);
     }
   }
 #pragma unroll
@@ -882,13 +879,32 @@ __global__ void __launch_bounds__(256) k_limits(StateK s, CodecK cod, int64_t n,
   __syncthreads();
   if (threadIdx.x < NC) {
     const int c = threadIdx.x;
+    const float a = fmaxf(fmaxf(smx[c][0], smx[c][1]), fmaxf(smx[c][2], smx[c][3]));
+    const float b = fminf(fminf(smn[c][0], smn[c][1]), fminf(smn[c][2], smn[c][3]));
+    part[(int64_t)blockIdx.x * 2 * NC + c] = a;
+    part[(int64_t)blockIdx.x * 2 * NC + NC + c] = b;
+  }
+}
+
+// mxmn[c] = max(mxmn[c], partial maxima), mxmn[NC + c] = min(...): one
+// workgroup, wavefront w folding components w, w + 4, ...
+__global__ void __launch_bounds__(256) k_limits_final(const float *__restrict__ part, int32_t nparts, int32_t nc,
+                                                      float *__restrict__ mxmn) {
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int c = wv; c < nc; c += 4) {
     float a = -INFINITY, b = INFINITY;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
-      a = fmaxf(a, smx[c][w]);
-      b = fminf(b, smn[c][w]);
+    for (int i = lane; i < nparts; i += 64) {
+      a = fmaxf(a, part[(int64_t)i * 2 * nc + c]);
+      b = fminf(b, part[(int64_t)i * 2 * nc + nc + c]);
     }
-    atomic_max_f32(mxmn + c, a);
-    atomic_min_f32(mxmn + NC + c, b);
+    for (int o = 32; o > 0; o >>= 1) {
+      a = fmaxf(a, __shfl_xor(a, o));
+      b = fminf(b, __shfl_xor(b, o));
+    }
+    if (lane == 0) {
+      mxmn[c] = fmaxf(mxmn[c], a);
+      mxmn[nc + c] = fminf(mxmn[nc + c], b);
+    }
   }
 }
 
@@ -948,6 +964,7 @@ struct aq_sim {
   int4 *items;
   int32_t *item_off;  // [nblocks + 1]; item_off[nblocks] = item count
   uint32_t *near;     // [nblocks]: particles within one block (k_near, at each sort)
+  float *lim_part;    // [kLimitBlocks][2 ncomp]: partials of k_limits
   int32_t max_items;
   bool have_items;    // items describe the current order of `items_state`
   const void *items_state;
@@ -1017,7 +1034,7 @@ static int64_t tile_slots(const aq_sim_desc *d) { return d->dim == 3 ? 512 : 144
 
 enum {
   WS_GRID_V, WS_GRID_F, WS_FLAGS, WS_KEY, WS_RANK, WS_COUNT, WS_OFF, WS_PART, WS_TMP_STATE, WS_TMP_IDS,
-  WS_ITEMS, WS_ITEM_OFF, WS_NEAR, WS_TILES, WS_LAYOUT, WS_N
+  WS_ITEMS, WS_ITEM_OFF, WS_NEAR, WS_TILES, WS_LIMITS, WS_LAYOUT, WS_N
 };
 
 static void ws_plan(const aq_sim_desc *d, int64_t *offs /*[WS_N]*/, int64_t *total) {
@@ -1043,6 +1060,7 @@ static void ws_plan(const aq_sim_desc *d, int64_t *offs /*[WS_N]*/, int64_t *tot
   sz[WS_ITEM_OFF] = (nblocks + 1) * 4;
   sz[WS_NEAR] = nblocks * 4;
   sz[WS_TILES] = (int64_t)max_items_of(d) * tile_slots(d) * node_bytes;
+  sz[WS_LIMITS] = (int64_t)kLimitBlocks * 2 * AQ_MAX_COMPS * 4;  // partial max / min of k_limits
   sz[WS_LAYOUT] = AQ_MAX_COMPS * (int64_t)sizeof(CompK);
   int64_t o = 0;
   for (int i = 0; i < WS_N; ++i) {
@@ -1125,6 +1143,7 @@ extern "C" int aq_sim_create(const aq_sim_desc *d, void *ws, int64_t ws_bytes, a
   s->item_off = (int32_t *)(base + offs[WS_ITEM_OFF]);
   s->tiles = base + offs[WS_TILES];
   s->near = (uint32_t *)(base + offs[WS_NEAR]);
+  s->lim_part = (float *)(base + offs[WS_LIMITS]);
   s->lay_dev = (CompK *)(base + offs[WS_LAYOUT]);
   s->max_items = max_items_of(d);
   s->have_items = false;
@@ -1803,10 +1822,12 @@ static int launch_limits(aq_sim *s, const void *state, int64_t n, float *mxmn, h
   if (n == 0) return AQ_OK;
   StateK sk = make_state(s, state, nullptr);
   CodecK cod = make_codec(s, 0u);
+  const int32_t nparts = (int32_t)(pblocks(n) < (unsigned)kLimitBlocks ? pblocks(n) : (unsigned)kLimitBlocks);
   dispatch(s, [&](auto tag_) {
     TAG_TYPES(tag_)
-    hipLaunchKernelGGL((k_limits<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, cod, n, mxmn);
+    hipLaunchKernelGGL((k_limits<DIM, MAT, ST, NW32, T>), dim3(nparts), dim3(256), 0, st, sk, cod, n, s->lim_part);
   });
+  hipLaunchKernelGGL(k_limits_final, dim3(1), dim3(256), 0, st, s->lim_part, nparts, (int32_t)s->d.ncomp, mxmn);
   LAUNCH_CHECK();
   return AQ_OK;
 }
