@@ -54,7 +54,8 @@ def energy_loss(solver, state):
 
 
 def _advance_plain(solver, state, nsteps):
-    """nsteps untiled plain substeps of `state` (written in place)."""
+    """nsteps plain substeps of `state` (written in place; LDS-tiled when the
+    solver's last sort sorted this buffer, else untiled)."""
     if nsteps <= 0:
         return
     _lib.check(_lib.lib().aq_advance(solver._h, _ptr(state), solver.current_idx, int(nsteps), 0, 0, None,
