@@ -4,7 +4,9 @@ descriptor-driven kernel: from the same words and the same work items, the
 records after two substeps are bit-identical, for the benchmark record (3D
 fluid, dithered), the 2D elastic record (shared exponents for C and F) and a
 record whose fields are out of component order; unspecialize returns to the
-generic kernel."""
+generic kernel.  The sparse scenes spread few particles over the whole
+domain, so that each workgroup walks many work items (the specialised
+kernel loads the next item during the current item's scatter)."""
 import numpy as np
 import pytest
 
@@ -19,7 +21,7 @@ def solver(kind, n):
     from autoquantizer_amd import layouts
     cfg = aq.MPMConfig()
     rng = np.random.default_rng(5)
-    if kind == "elastic_2d":
+    if kind.startswith("elastic_2d"):
         cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.E, cfg.nu, cfg.sticky = 2, False, 128, 1e-4, 1e4, 0.2, True
         _, t, g, sh = layouts.elastic_2d()
     else:
@@ -34,10 +36,11 @@ def solver(kind, n):
     cfg.seed = 3
     sim = aq.MPMSolver(cfg, types=t, groups=g, shared=sh)
     vals = np.zeros((sim.ncomp, n))
-    vals[:d] = rng.uniform(0.2, 0.6, (d, n))
+    lo, hi = (0.08, 0.92) if kind.endswith("_sparse") else (0.2, 0.6)
+    vals[:d] = rng.uniform(lo, hi, (d, n))
     vals[d:2 * d] = rng.uniform(-1, 1, (d, n))
     vals[2 * d:2 * d + d * d] = rng.uniform(-0.5, 0.5, (d * d, n))
-    if kind == "elastic_2d":
+    if kind.startswith("elastic_2d"):
         vals[-4:] = np.eye(2).reshape(-1, 1) + rng.uniform(-0.01, 0.01, (4, n))
     else:
         vals[-1] = rng.uniform(0.99, 1.01, n)
@@ -45,11 +48,12 @@ def solver(kind, n):
     return sim
 
 
-@pytest.mark.parametrize("kind", ["fluid_3d", "elastic_2d", "fluid_3d_reordered"])
-def test_specialised_kernel_is_bit_identical(kind, tmp_path):
+@pytest.mark.parametrize("kind,n", [("fluid_3d", 40000), ("elastic_2d", 40000), ("fluid_3d_reordered", 40000),
+                                    ("fluid_3d_sparse", 20000), ("elastic_2d_sparse", 4000)])
+def test_specialised_kernel_is_bit_identical(kind, n, tmp_path):
     import torch
-    print(f"[jit] {kind}", flush=True)
-    sim = solver(kind, 40000)
+    print(f"[jit] {kind} n={n}", flush=True)
+    sim = solver(kind, n)
     sim.advance(3, sort_every=1)  # sorted: the next substeps run the tiled kernels on these items
     w0, ids0, sc = sim.state.clone(), sim.ids.clone(), sim.step_count
     sim.advance(2, sort_every=0)
