@@ -669,20 +669,22 @@ __global__ void k_fill_u32(uint32_t *a, uint32_t v, int64_t n) {
 }
 
 // ---- sort by grid block of the stencil base --------------------------------
-// Keys and ranks: rank = position among the particles of the same block
-// (counting-sort scatter).  Lanes of a wavefront with equal keys take
-// consecutive ranks with one atomic (particles are mostly sorted already, so
-// a wavefront holds few distinct keys); after four distinct keys the rest
-// take one atomic each.
+// Keys (grid block of the stencil base), block counts and cell ranks: the
+// 64 cells of a block are its stencil bases, and rank = (position among the
+// particles of the same cell) << 6 | cell (k_sort_rank turns it into the
+// position in the block).  Lanes of a wavefront with equal keys add to the
+// block count with one atomic (particles are mostly sorted already, so a
+// wavefront holds few distinct keys); after four distinct keys the rest add
+// one each.
 template <int DIM, int MAT, int ST, int NW32, class T>
 __global__ void __launch_bounds__(256) k_sort_keys(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
                                                    uint32_t *__restrict__ key, uint32_t *__restrict__ rank,
-                                                   uint32_t *__restrict__ counts) {
+                                                   uint32_t *__restrict__ counts, uint32_t *__restrict__ ccounts) {
   __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const bool valid = p < n;
-  uint32_t k = 0u;
+  uint32_t k = 0u, cell = 0u;
   if (valid) {
     T x[DIM];
     if constexpr (ST == AQ_STORE_QUANT) {
@@ -708,23 +710,44 @@ __global__ void __launch_bounds__(256) k_sort_keys(StateK s, CodecK cod, ParamsT
 #pragma unroll
     for (int a = 0; a < DIM; ++a) base[a] = stencil_1d<T>(x[a], P, fx, w);
     k = (uint32_t)block_of_base<DIM>(base, P.nb);
+    constexpr int E = BlockShape<DIM>::E, LG = BlockShape<DIM>::LOG;
+    cell = DIM == 3 ? (uint32_t)(((base[0] & (E - 1)) << (2 * LG)) | ((base[1] & (E - 1)) << LG) | (base[2] & (E - 1)))
+                    : (uint32_t)(((base[0] & (E - 1)) << LG) | (base[1] & (E - 1)));
     key[p] = k;
   }
   uint64_t pending = __ballot(valid);
-  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-  uint32_t r = 0u;
   for (int round = 0; round < 4 && pending; ++round) {
     const int leader = __ffsll((unsigned long long)pending) - 1;
     const uint32_t k0 = __shfl(k, leader);
     const uint64_t same = __ballot(k == k0) & pending;
-    uint32_t base = 0u;
-    if (lane == leader) base = atomicAdd(counts + k0, (uint32_t)__popcll(same));
-    base = __shfl(base, leader);
-    if ((same >> lane) & 1ull) r = base + (uint32_t)__popcll(same & below);
+    if (lane == leader) atomicAdd(counts + k0, (uint32_t)__popcll(same));
     pending &= ~same;
   }
-  if ((pending >> lane) & 1ull) r = atomicAdd(counts + k, 1u);
-  if (valid) rank[p] = r;
+  if ((pending >> lane) & 1ull) atomicAdd(counts + k, 1u);
+  if (valid) rank[p] = (atomicAdd(ccounts + (int64_t)k * 64 + cell, 1u) << 6) | cell;
+}
+
+// Position of each particle in its block, layer by layer: layer r holds the
+// r-th particle of every cell that has more than r, in cell order, so that
+// the 64 lanes of a wavefront of the tiled kernels mostly hold particles of
+// distinct cells (their P2G atomics and G2P reads then hit distinct LDS
+// slots; in particle order within a cell, ~8 lanes would share each slot).
+// pos = sum_c min(count_c, r) + #{c < cell : count_c > r}.
+__global__ void __launch_bounds__(256) k_sort_rank(const uint32_t *__restrict__ key, uint32_t *__restrict__ rank,
+                                                   const uint32_t *__restrict__ ccounts, int64_t n) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t rc = rank[p], r = rc >> 6, cell = rc & 63u;
+  const uint4 *cc = (const uint4 *)(ccounts + (int64_t)key[p] * 64);
+  uint32_t pos = 0u;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const uint4 c4 = cc[i];
+    const uint32_t c[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) pos += (c[j] < r ? c[j] : r) + ((uint32_t)(4 * i + j) < cell && c[j] > r ? 1u : 0u);
+  }
+  rank[p] = pos;
 }
 
 // Exclusive scans of counts[0..m) into off[0..m] and of the work items per
@@ -958,7 +981,7 @@ struct aq_sim {
   int32_t node_bytes;
   void *grid_v, *grid_f, *tiles;
   uint32_t *fflags, *act;
-  uint32_t *skey, *srank, *scount, *soff, *spart;
+  uint32_t *skey, *srank, *scount, *ccount, *soff, *spart;
   void *tmp_state;
   int32_t *tmp_ids;
   int4 *items;
@@ -1033,7 +1056,7 @@ static int32_t max_items_of(const aq_sim_desc *d) {
 static int64_t tile_slots(const aq_sim_desc *d) { return d->dim == 3 ? 512 : 144; }  // Tile<DIM>::TN
 
 enum {
-  WS_GRID_V, WS_GRID_F, WS_FLAGS, WS_KEY, WS_RANK, WS_COUNT, WS_OFF, WS_PART, WS_TMP_STATE, WS_TMP_IDS,
+  WS_GRID_V, WS_GRID_F, WS_FLAGS, WS_KEY, WS_RANK, WS_COUNT, WS_CCOUNT, WS_OFF, WS_PART, WS_TMP_STATE, WS_TMP_IDS,
   WS_ITEMS, WS_ITEM_OFF, WS_NEAR, WS_TILES, WS_LIMITS, WS_LAYOUT, WS_N
 };
 
@@ -1052,6 +1075,7 @@ static void ws_plan(const aq_sim_desc *d, int64_t *offs /*[WS_N]*/, int64_t *tot
   sz[WS_KEY] = cap * 4;
   sz[WS_RANK] = cap * 4;
   sz[WS_COUNT] = (nblocks + 1) * 4;
+  sz[WS_CCOUNT] = nodes * 4;  // particles per cell (64 cells per block)
   sz[WS_OFF] = (nblocks + 1) * 4;
   sz[WS_PART] = chunks * 8;
   sz[WS_TMP_STATE] = state_bytes;
@@ -1135,6 +1159,7 @@ extern "C" int aq_sim_create(const aq_sim_desc *d, void *ws, int64_t ws_bytes, a
   s->skey = (uint32_t *)(base + offs[WS_KEY]);
   s->srank = (uint32_t *)(base + offs[WS_RANK]);
   s->scount = (uint32_t *)(base + offs[WS_COUNT]);
+  s->ccount = (uint32_t *)(base + offs[WS_CCOUNT]);
   s->soff = (uint32_t *)(base + offs[WS_OFF]);
   s->spart = (uint32_t *)(base + offs[WS_PART]);
   s->tmp_state = base + offs[WS_TMP_STATE];
@@ -1614,12 +1639,15 @@ extern "C" int aq_sort(aq_sim *s, void *state, int64_t n, int32_t *ids, void *st
   CodecK cod = make_codec(s, 0u);
   const int32_t m = s->nblocks;
   hipLaunchKernelGGL(k_fill_u32, dim3((m + 1 + 255) / 256), dim3(256), 0, st, s->scount, 0u, (int64_t)m + 1);
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)(((int64_t)m * 64 + 255) / 256)), dim3(256), 0, st, s->ccount, 0u,
+                     (int64_t)m * 64);
   LAUNCH_CHECK();
   dispatch(s, [&](auto tag_) {
     TAG_TYPES(tag_)
     hipLaunchKernelGGL((k_sort_keys<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, cod,
-                       params_of<T>(s), n, s->skey, s->srank, s->scount);
+                       params_of<T>(s), n, s->skey, s->srank, s->scount, s->ccount);
   });
+  hipLaunchKernelGGL(k_sort_rank, dim3(pblocks(n)), dim3(256), 0, st, s->skey, s->srank, s->ccount, n);
   LAUNCH_CHECK();
   const int32_t chunks = (m + 4095) / 4096;
   hipLaunchKernelGGL(k_scan_local, dim3(chunks), dim3(256), 0, st, s->scount, s->soff, s->item_off, s->spart, m);

@@ -303,9 +303,18 @@ def test_sort_is_permutation(orc):
     assert np.array_equal(np.sort(ids), np.arange(7000))
     assert np.array_equal(sim.get_words(original_order=True), w0)
     st = sim.get_state(original_order=False)
-    base = np.clip(np.floor(st[:3] * 64 - 0.5), 0, 61).astype(int) // 4
+    cb = np.clip(np.floor(st[:3] * 64 - 0.5), 0, 61).astype(int)
+    base = cb // 4
     key = (base[0] * 16 + base[1]) * 16 + base[2]
     assert np.all(np.diff(key) >= 0)
+    # inside a block, layer by layer: the r-th particle of every cell holding
+    # more than r, in cell order, then the (r+1)-th ...
+    cell = ((cb[0] & 3) << 4) | ((cb[1] & 3) << 2) | (cb[2] & 3)
+    for k in np.unique(key):
+        seq = cell[key == k]
+        cnt = np.bincount(seq, minlength=64)
+        expect = [c for r in range(cnt.max()) for c in range(64) if cnt[c] > r]
+        assert seq.tolist() == expect
 
 
 def test_mpm88_regression_gpu():
