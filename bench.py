@@ -297,7 +297,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--particles", type=int, default=1 << 20)
     ap.add_argument("--grid", type=int, default=128)
-    ap.add_argument("--sort-every", type=int, default=16)
+    ap.add_argument("--sort-every", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-search", action="store_true")
