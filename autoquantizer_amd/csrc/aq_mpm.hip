@@ -1127,6 +1127,9 @@ static ParamsT<T> make_params(const aq_sim_desc *d) {
 extern "C" int aq_sim_create(const aq_sim_desc *d, void *ws, int64_t ws_bytes, aq_sim **out) {
   int rc = check_desc(d);
   if (rc) return rc;
+  // the tiled P2G packs each node's mass as a non-negative fixed-point sum
+  if (!(d->p_mass > 0.0) || !std::isfinite(d->p_mass))
+    return fail(AQ_ERR_INVALID, "particle mass must be positive and finite");
   if (!out) return fail(AQ_ERR_INVALID, "null output");
   int64_t offs[WS_N], total;
   ws_plan(d, offs, &total);

@@ -146,7 +146,7 @@ typedef struct aq_sim_desc {
   int32_t ncomp;    /* number of state components (2d + d^2 + 1 or 2d + 2d^2) */
   int32_t nwords;   /* AQ_STORE_QUANT: record size in u64 words */
   int64_t capacity; /* particle capacity = plane stride */
-  double dx, inv_dx, dt, p_vol, p_mass, gravity;
+  double dx, inv_dx, dt, p_vol, p_mass, gravity; /* p_mass > 0 (AQ_ERR_INVALID otherwise) */
   double la;        /* fluid bulk modulus (MPMSolver.la) */
   double mu_0, la_0;/* elastic Lame parameters */
   uint64_t seed;    /* dithering seed */

@@ -117,6 +117,11 @@ def test_sim_desc_validation():
     assert L.aq_sim_workspace_bytes(ctypes.byref(d)) < 0
     d.n_grid, d.capacity = 64, 1000
     assert L.aq_sim_workspace_bytes(ctypes.byref(d)) > 64 ** 3 * 16 * 2
+    # creation also needs a positive particle mass (checked before the workspace)
+    h = ctypes.c_void_p()
+    d.p_mass = 0.0
+    assert L.aq_sim_create(ctypes.byref(d), None, 0, ctypes.byref(h)) < 0
+    assert b"particle mass" in L.aq_last_error()
 
 
 def test_header_constants_match_binding():
