@@ -1322,7 +1322,7 @@ static int jit_source(const aq_sim &v, std::string &src) {
                 "aq_jit_tiled(aq::StateK s, aq::CodecK cod, aq::ParamsT<float> P, int64_t n, const int4 *items,\n"
                 "             const int32_t *nitems, const float4 *grid_v, float4 *tiles, float4 *grid_f,\n"
                 "             uint32_t *fflags) {\n"
-                "  aq::tiled_body<%d, %d, AQ_STORE_QUANT, %d, float, true, true, true>(s, cod, P, n, items, nitems, grid_v,\n"
+                "  aq::tiled_body<%d, %d, AQ_STORE_QUANT, %d, float, true, true>(s, cod, P, n, items, nitems, grid_v,\n"
                 "                                                               tiles, grid_f, fflags);\n}\n",
                 kItem, kJitWaves, v.d.dim, v.d.material, nw32_bucket(v.d.nwords));
   src += buf;

@@ -149,13 +149,6 @@ template <class T> __device__ __forceinline__ T wave_sum(T x) {
 // ~11% faster than the 3 the register allocator picks by itself.
 template <int ST> struct TiledWaves { static constexpr int W = ST == AQ_STORE_QUANT ? 4 : 1; };
 
-// Grid velocities of an item's tile, TN nodes from the tile origin o, dealt
-// to the threads of the workgroup (thread tid holds nodes tid + r kItem).
-template <int DIM, class T> struct TileVals {
-  static constexpr int NT = (Tile<DIM>::TN + kItem - 1) / kItem;
-  vec4_t<T> v[NT];
-};
-
 template <int DIM, class T>
 __device__ __forceinline__ void tile_origin(const int4 &item, int nb, int (&o)[3]) {
   int bc[3];
@@ -164,37 +157,8 @@ __device__ __forceinline__ void tile_origin(const int4 &item, int nb, int (&o)[3
   for (int a = 0; a < 3; ++a) o[a] = a < DIM ? bc[a] * Tile<DIM>::E - 1 : 0;
 }
 
-template <int DIM, class T>
-__device__ __forceinline__ void tile_load(const ParamsT<T> &P, const int (&o)[3], int tid,
-                                          const vec4_t<T> *__restrict__ grid_v, TileVals<DIM, T> &tv) {
-  using TL = Tile<DIM>;
-#pragma unroll
-  for (int r = 0; r < TileVals<DIM, T>::NT; ++r) {
-    const int nd = tid + r * kItem;
-    int t[3];
-    TL::node_coords(nd, t);
-    const int gi = o[0] + t[0], gj = o[1] + t[1], gk = o[2] + t[2];
-    tv.v[r].x = tv.v[r].y = tv.v[r].z = tv.v[r].w = (T)0;
-    const bool in = nd < TL::TN && gi >= 0 && gi < P.n && gj >= 0 && gj < P.n &&
-                    (DIM == 2 || (gk >= 0 && gk < P.n));
-    if (in) tv.v[r] = grid_v[node_index<DIM>(gi, gj, gk, P.nb)];
-  }
-}
-
-template <int DIM, class T>
-__device__ __forceinline__ void tile_stage(int tid, const TileVals<DIM, T> &tv, vec4_t<T> *lds) {
-  using TL = Tile<DIM>;
-#pragma unroll
-  for (int r = 0; r < TileVals<DIM, T>::NT; ++r) {
-    const int nd = tid + r * kItem;
-    int t[3];
-    TL::node_coords(nd, t);
-    if (nd < TL::TN) lds[TL::slot(t[0], t[1], t[2])] = tv.v[r];
-  }
-}
-
-// Loads and stages a tile node by node (a rolled loop: fewer live VGPRs
-// than tile_load + tile_stage).
+// Loads and stages an item's grid-velocity tile node by node (a rolled
+// loop keeps few VGPRs live).
 template <int DIM, class T>
 __device__ __forceinline__ void tile_copy(const ParamsT<T> &P, const int (&o)[3], int tid,
                                           const vec4_t<T> *__restrict__ grid_v, vec4_t<T> *lds) {
@@ -211,16 +175,13 @@ __device__ __forceinline__ void tile_copy(const ParamsT<T> &P, const int (&o)[3]
   }
 }
 
-// A workgroup walks its items.  PIPE: the particle words and the grid
-// velocities of item i+1 are loaded while the LDS atomics of item i run and
-// staged before the barrier that completes item i's scatter (two barriers
-// per item: the bounds of the fixed-point scales and the end of the
-// scatter); this holds ~20 more VGPRs across the scatter.  Without PIPE an
-// item is loaded and staged at the top of its iteration, behind a third
-// barrier.  On the 3D fluid benchmark the specialised quantized kernel is
-// ~2% faster with PIPE (same occupancy); the fp32-storage kernel would drop
-// from 4 to 3 waves per SIMD and is 7% faster without.
-template <int DIM, int MAT, int ST, int NW32, class T, bool DO_G2P, bool DO_P2G, bool PIPE = false>
+// A workgroup walks its items; per item three barriers: the tile staged,
+// the bounds of the fixed-point scales, the scatter complete.  (Loading the
+// next item during the current item's scatter saves a barrier but holds
+// ~20 more VGPRs: 2 % faster for the specialised quantized kernel with the
+// particles of a block in random order, 1 % slower with the layered order;
+// the fp32-storage kernel lost a wave per SIMD.)
+template <int DIM, int MAT, int ST, int NW32, class T, bool DO_G2P, bool DO_P2G>
 __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
                                            const int4 *__restrict__ items, const int32_t *__restrict__ nitems,
                                            const vec4_t<T> *__restrict__ grid_v, vec4_t<T> *__restrict__ tiles,
@@ -255,37 +216,17 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
     for (int q = tid; q < TS; q += kItem)
 #pragma unroll
       for (int a = 0; a < NP; ++a) ts[a][q] = 0ull;  // cleared before the first barrier
-  int4 item = items[it];
-  Fetched<ST, NW32> fw;
-  T v[NC];
-  // the particle's loads go out before the tile's, so their latencies overlap
-  auto load_item = [&](const int4 &ni4) {
-    if (tid < ni4.z && (int64_t)ni4.y + tid < n)
-      fetch_state<DIM, ST, NW32, NC, T, G2P_ONLY_LOAD>(s, (int64_t)ni4.y + tid, load_mask, fw, v);
-  };
-  if constexpr (PIPE) {
-    load_item(item);
-    if constexpr (DO_G2P) {
-      int o[3];
-      tile_origin<DIM, T>(item, P.nb, o);
-      TileVals<DIM, T> tl;
-      tile_load<DIM, T>(P, o, tid, grid_v, tl);
-      tile_stage<DIM, T>(tid, tl, tv);
-    }
-    __syncthreads();  // first tile staged, scatter tile cleared
-  }
-  for (;;) {
-    const int next = it + per;
-    const bool more = next < hi;
+  for (; it < hi; it += per) {
+    const int4 item = items[it];
     int o[3];
     tile_origin<DIM, T>(item, P.nb, o);
-    if constexpr (!PIPE) {
-      load_item(item);
-      if constexpr (DO_G2P) tile_copy<DIM, T>(P, o, tid, grid_v, tv);
-      __syncthreads();  // tile staged; the previous item's tile written out
-    }
-    int4 nitem = make_int4(0, 0, 0, 0);
-    if (more) nitem = items[next];
+    // the particle's loads go out before the tile's, so their latencies overlap
+    Fetched<ST, NW32> fw;
+    T v[NC];
+    if (tid < item.z && (int64_t)item.y + tid < n)
+      fetch_state<DIM, ST, NW32, NC, T, G2P_ONLY_LOAD>(s, (int64_t)item.y + tid, load_mask, fw, v);
+    if constexpr (DO_G2P) tile_copy<DIM, T>(P, o, tid, grid_v, tv);
+    __syncthreads();  // tile staged; the previous item's tile written out
     const bool has = tid < item.z && (int64_t)item.y + tid < n;
     const int64_t p = (int64_t)item.y + tid;
     T out[NC];
@@ -338,16 +279,6 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
       }
     }
     __syncthreads();  // item bounds complete; G2P done with the tile
-    // PIPE: the next item's particle words and tile go out before this item's scatter
-    TileVals<DIM, T> tl;
-    if (PIPE && more) {
-      load_item(nitem);
-      if constexpr (DO_G2P) {
-        int no[3];
-        tile_origin<DIM, T>(nitem, P.nb, no);
-        tile_load<DIM, T>(P, no, tid, grid_v, tl);
-      }
-    }
     int k[4] = {0, 0, 0, 0};
     if constexpr (DO_P2G) {
       bool finite = true;
@@ -384,10 +315,8 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
         }
       }
     }
-    if constexpr (PIPE && DO_G2P)
-      if (more) tile_stage<DIM, T>(tid, tl, tv);
-    __syncthreads();  // tile sums complete (PIPE: the next item's tile staged)
     if constexpr (DO_P2G) {
+      __syncthreads();  // tile sums complete
       vec4_t<T> *dst = tiles + (int64_t)it * TN;
       for (int nd = tid; nd < TN; nd += kItem) {
         int t[3];
@@ -413,9 +342,6 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
         for (int a = 0; a < NP; ++a) ts[a][q] = 0ull;
       }
     }
-    if (!more) break;
-    it = next;
-    item = nitem;
   }
 }
 

@@ -36,7 +36,7 @@ HBM_PEAK = 8.0e12
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a plain copy's readings
 # (profiles/k_tiled_fused_fluid3d_1M.md); a profiler run, not this script,
 # measures it.
-FUSED_TRAFFIC_BYTES = 103.1e6
+FUSED_TRAFFIC_BYTES = 102.8e6
 
 
 def make_solver(storage, n, n_grid, seed=0):
