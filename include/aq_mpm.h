@@ -217,8 +217,10 @@ int aq_advance_timed(aq_sim *sim, void *state, int64_t n, int nsteps, uint32_t s
                      int64_t *active_nodes, void *stream);
 
 /* Sort particles by grid block of their stencil base (replaces build_pid's
-   per-block lists, mpm_solver.py:578-588, node_dynamic.h:61-90); permutes
-   the state and ids in place through the workspace. */
+   per-block lists, mpm_solver.py:578-588, node_dynamic.h:61-90); inside a
+   block, in layers over its 64 cells (the r-th particle of every cell that
+   has more than r, in cell order, then the (r+1)-th).  Permutes the state
+   and ids in place through the workspace. */
 int aq_sort(aq_sim *sim, void *state, int64_t n, int32_t *ids, void *stream);
 
 /* Per-component max and min over particles, accumulated into
