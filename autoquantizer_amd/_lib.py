@@ -85,7 +85,7 @@ EXPORTS = [
     "aq_store_members", "aq_sim_workspace_bytes", "aq_sim_create", "aq_sim_destroy",
     "aq_substep", "aq_p2g", "aq_grid_op", "aq_g2p", "aq_advance", "aq_sort",
     "aq_update_limits", "aq_advance_limits", "aq_energy", "aq_sim_grid", "aq_advance_timed",
-    "aq_adj_workspace_bytes", "aq_adj_substep", "aq_noise_add", "aq_noise_dot", "aq_mlp_forward",
+    "aq_adj_workspace_bytes", "aq_adj_substep", "aq_noise_add", "aq_noise_dot", "aq_codec_selftest", "aq_mlp_forward",
     "aq_mlp_backward", "aq_adam_step", "aq_sim_specialize", "aq_jit_build",
 ]
 
@@ -132,6 +132,7 @@ def lib():
     L.aq_jit_build.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, i64]
     L.aq_noise_add.argtypes = [P, P, i64, P, u64, u32, P]
     L.aq_noise_dot.argtypes = [P, P, i64, u64, u32, P, P]
+    L.aq_codec_selftest.argtypes = [P, P]
     L.aq_mlp_forward.argtypes = [P, P, i32, P, P, P]
     f32 = ctypes.c_float
     L.aq_mlp_backward.argtypes = [P, P, P, P, P, i32, f32, P, P, P, P]

@@ -269,6 +269,27 @@ __device__ __forceinline__ float dec_shared(uint32_t e, uint32_t q, const CompK 
   return __uint_as_float(fb | sign);
 }
 
+// dec_shared of the encoder's own output, for signed members: when the
+// group's unit digit 2^(e + conv - 126 - db) and every value it scales are
+// normal floats (e + conv + 1 - db >= 3 and e + conv <= 254), the decode is
+// the digit magnitude times that unit (a sign-only pattern decodes to a
+// quarter unit, as dec_shared's renormalisation gives); otherwise
+// dec_shared.  About half the instructions of dec_shared per member.
+__device__ __forceinline__ float dec_shared_enc(uint32_t e, uint32_t q, const CompK &c) {
+  if (c.is_signed) {
+    const uint32_t db = c.nbits - 1u;
+    const int32_t ue = (int32_t)(e + c.conv) + 1 - (int32_t)db;
+    if (e != 0u && ue >= 3 && e + c.conv <= 254u) {
+      q &= lowmask(c.nbits);
+      const uint32_t mag = q & lowmask(db), sign = (q >> db) & 1u;
+      const float m = mag ? (float)mag : (sign ? 0.25f : 0.0f);
+      const float r = m * __uint_as_float((uint32_t)ue << 23);
+      return __uint_as_float(__float_as_uint(r) | (sign << 31));
+    }
+  }
+  return dec_shared(e, q, c);
+}
+
 // Decode a member given a field reader get(off, nbits).
 template <class G>
 __device__ __forceinline__ float decode_with(G &&get, const CompK &c) {
@@ -430,7 +451,7 @@ __device__ __forceinline__ void encode_record(const float (&v)[NC], const CodecK
       if ((gf >> c) & 1u) sink.put(k.eoff, k.exp_bits, e);
       const uint32_t d = enc_shared(v[c], gmax[c], k);
       sink.put(k.off, k.nbits, d);
-      dec[c] = dec_shared(e, d, k);
+      dec[c] = dec_shared_enc(e, d, k);
     } else if (k.kind == K_FIXED) {
       const float u = k.dither ? noise_u(slot, hash32((uint32_t)c ^ kb)) : 0.0f;
       const uint32_t q = enc_fixed(v[c], k, u);

@@ -2025,3 +2025,34 @@ extern "C" int aq_adj_substep(aq_sim *s, const void *state, int64_t n, const voi
   LAUNCH_CHECK();
   return AQ_OK;
 }
+
+// ---- codec self-check ------------------------------------------------------------
+// dec_shared_enc (the encoder's fast decode of signed shared-exponent digits)
+// against dec_shared, bit for bit, for every exponent field and digit
+// pattern of one (digit bits, exponent bits) shape.
+__global__ void __launch_bounds__(256) k_dec_shared_check(int nbits, int exp_bits, unsigned long long *bad) {
+  const uint32_t n = 1u << (nbits + exp_bits);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  CompK c = {};
+  c.kind = K_FLOAT;
+  c.nbits = (uint8_t)nbits;
+  c.is_signed = 1;
+  c.exp_bits = (uint8_t)exp_bits;
+  c.shared = 1;
+  c.conv = 127u - (1u << (exp_bits - 1)) + 1u;
+  const uint32_t e = i >> nbits, q = i & ((1u << nbits) - 1u);
+  if (__float_as_uint(dec_shared_enc(e, q, c)) != __float_as_uint(dec_shared(e, q, c))) atomicAdd(bad, 1ull);
+}
+
+extern "C" int aq_codec_selftest(unsigned long long *mismatches, void *stream) {
+  if (!mismatches) return fail(AQ_ERR_INVALID, "null output");
+  hipStream_t st = (hipStream_t)stream;
+  for (int eb = 4; eb <= 8; ++eb)
+    for (int nb = 2; nb <= 20; ++nb) {
+      const uint32_t n = 1u << (nb + eb);
+      hipLaunchKernelGGL(k_dec_shared_check, dim3((n + 255) / 256), dim3(256), 0, st, nb, eb, mismatches);
+    }
+  LAUNCH_CHECK();
+  return AQ_OK;
+}

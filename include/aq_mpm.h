@@ -272,6 +272,15 @@ int aq_noise_add(aq_sim *sim, void *state, int64_t n, const double *sigma, uint6
                  void *stream);
 int aq_noise_dot(aq_sim *sim, const void *adj, int64_t n, uint64_t seed, uint32_t step, double *out, void *stream);
 
+/* Codec self-check (no counterpart in the reference): the fused kernels'
+   encoder decodes the shared-exponent digits it writes with a shorter
+   formula than the record decode (reconstruct_custom_float_with_exponent,
+   TS/taichi/codegen/codegen_llvm_quant.cpp:660-757); adds to *mismatches
+   (device u64, zero it first) the digit patterns, over every exponent field
+   and digit pattern of signed members with 4..8 exponent bits and 2..20
+   digit bits, on which the two differ in any bit. */
+int aq_codec_selftest(unsigned long long *mismatches, void *stream);
+
 /* ---- surrogate MLP of the scheme search: training and inference kernels
    (no counterpart in the reference, whose search always runs the adjoint) ---
    in -> 64 (tanh) -> 64 (tanh) -> 64 (tanh) -> out; parameters one flat fp32

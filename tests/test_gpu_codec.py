@@ -247,3 +247,19 @@ def test_large_roundtrip(orc):
     v2 = gpu_unpack(comps, nw, w)
     w2 = gpu_pack(comps, nw, v2)
     assert np.array_equal(w, w2)
+
+
+def test_encoder_shared_exponent_decode_matches_record_decode():
+    """The fused kernels' encoder decodes the shared-exponent digits it writes
+    (the values the next P2G sees) with a shorter formula than the record
+    decode; aq_codec_selftest compares the two bit for bit over every
+    exponent field and digit pattern of signed members (4..8 exponent bits,
+    2..20 digit bits)."""
+    torch = _torch()
+    from autoquantizer_amd import _lib
+    print("[codec] encoder decode self-check", flush=True)
+    bad = torch.zeros(1, dtype=torch.int64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    _lib.check(_lib.lib().aq_codec_selftest(ctypes.c_void_p(bad.data_ptr()), ctypes.c_void_p(stream)))
+    torch.cuda.synchronize()
+    assert int(bad.item()) == 0
