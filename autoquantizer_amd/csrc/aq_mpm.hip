@@ -534,7 +534,7 @@ __global__ void __launch_bounds__(256) k_gather(ParamsT<T> P, int32_t nblocks, c
       t[a] = loc[a] + 1 - d[a] * E;
       ok = ok && c >= 0 && c < nb && t[a] < TL::TE;  // 2D: only 3 nodes reach past the edge
     }
-    q[k] = TL::node_of(t[0], t[1], t[2]);
+    q[k] = TL::store_pos(t[0], t[1], t[2]);
     r0[k] = r1[k] = 0;
     if (ok) {
       const int32_t nbid = DIM == 3 ? ((bc[0] + d[0]) * nb + (bc[1] + d[1])) * nb + (bc[2] + d[2])

@@ -41,7 +41,9 @@ template <int DIM> struct Tile {
   static constexpr int SY = DIM == 3 ? 12 : 1;
   static constexpr int SX = DIM == 3 ? 112 : 72;
   static constexpr int TS = TE * SX;                              // LDS slots
-  __device__ static int slot(int x, int y, int z) { return DIM == 3 ? x * SX + y * SY + z : x * SX + y; }
+  __host__ __device__ static constexpr int slot(int x, int y, int z) {
+    return DIM == 3 ? x * SX + y * SY + z : x * SX + y;
+  }
   __device__ static void node_coords(int nd, int (&t)[3]) {
     if constexpr (DIM == 3) {
       t[0] = nd / (TE * TE);
@@ -54,7 +56,57 @@ template <int DIM> struct Tile {
     }
   }
   __device__ static int node_of(int x, int y, int z) { return DIM == 3 ? (x * TE + y) * TE + z : x * TE + y; }
+  // Position of tile node (x, y, z) in the tile buffer.  Per axis the tile
+  // covers the last node of the previous block (t = 0), the E nodes of its
+  // own block (1..E) and the first 3 of the next (E+1..E+3); the buffer
+  // groups the nodes by the block they belong to (3^DIM groups, in group
+  // order) and stores each group row-major, which is the gathering
+  // wavefront's lane order: a wavefront reads each covering tile as one
+  // contiguous run.
+  __host__ __device__ static constexpr int store_pos(int x, int y, int z) {
+    int g[3] = {0, 0, 0}, u[3] = {0, 0, 0}, sz[3] = {0, 0, 0};
+    const int t[3] = {x, y, z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      g[a] = t[a] == 0 ? 0 : (t[a] <= E ? 1 : 2);
+      u[a] = t[a] == 0 ? 0 : (t[a] <= E ? t[a] - 1 : t[a] - E - 1);
+      sz[a] = g[a] == 0 ? 1 : (g[a] == 1 ? E : 3);
+    }
+    // nodes of the groups before (g0, g1, g2): the slab below g0, the rows
+    // of slab g0 below g1, the groups of row (g0, g1) below g2
+    const int cum[3] = {0, 1, E + 1};  // nodes per axis before group g
+    if constexpr (DIM == 3) {
+      const int before = cum[g[0]] * TE * TE + sz[0] * cum[g[1]] * TE + sz[0] * sz[1] * cum[g[2]];
+      return before + (u[0] * sz[1] + u[1]) * sz[2] + u[2];
+    } else {
+      const int before = cum[g[0]] * TE + sz[0] * cum[g[1]];
+      return before + u[0] * sz[1] + u[1];
+    }
+  }
 };
+
+// LDS slot of the node at each position of the tile buffer (the inverse of
+// Tile::store_pos), built at compile time: the write-out walks the buffer
+// in order, so each wavefront stores a contiguous run.
+template <int DIM> struct TileOrder {
+  uint16_t slot[Tile<DIM>::TN];
+};
+template <int DIM> constexpr TileOrder<DIM> make_tile_order() {
+  using TL = Tile<DIM>;
+  TileOrder<DIM> o{};
+  for (int x = 0; x < TL::TE; ++x)
+    for (int y = 0; y < TL::TE; ++y)
+      for (int z = 0; z < (DIM == 3 ? TL::TE : 1); ++z) o.slot[TL::store_pos(x, y, z)] = (uint16_t)TL::slot(x, y, z);
+  return o;
+}
+__constant__ constexpr TileOrder<3> kTileOrder3 = make_tile_order<3>();
+__constant__ constexpr TileOrder<2> kTileOrder2 = make_tile_order<2>();
+template <int DIM> __device__ __forceinline__ int tile_slot_at(int pos) {
+  if constexpr (DIM == 3)
+    return kTileOrder3.slot[pos];
+  else
+    return kTileOrder2.slot[pos];
+}
 
 template <int DIM>
 __device__ __forceinline__ void block_coords(int32_t b, int nb, int (&bc)[3]) {
@@ -212,6 +264,12 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
   const int lo = (int)((int64_t)ni * xcd / 8), hi = (int)((int64_t)ni * (xcd + 1) / 8);
   int it = lo + (int)(blockIdx.x >> 3);
   if (it >= hi) return;  // uniform over the workgroup
+  // LDS slot of each buffer position of the write-out, copied to LDS once
+  // per workgroup (visible after the first barrier): the write-out then
+  // looks it up with an LDS read instead of a global load per item
+  __shared__ uint16_t wsl[DO_P2G ? TN : 1];
+  if constexpr (DO_P2G)
+    for (int q = tid; q < TN; q += kItem) wsl[q] = (uint16_t)tile_slot_at<DIM>(q);
   if constexpr (DO_P2G)
     for (int q = tid; q < TS; q += kItem)
 #pragma unroll
@@ -318,10 +376,8 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
     if constexpr (DO_P2G) {
       __syncthreads();  // tile sums complete
       vec4_t<T> *dst = tiles + (int64_t)it * TN;
-      for (int nd = tid; nd < TN; nd += kItem) {
-        int t[3];
-        TL::node_coords(nd, t);
-        const int q = TL::slot(t[0], t[1], t[2]);
+      for (int nd = tid; nd < TN; nd += kItem) {  // nd: position in the buffer
+        const int q = wsl[nd];
         vec4_t<T> val;
         if constexpr (NP == 2) {
           int32_t mx, m, my, mz;
