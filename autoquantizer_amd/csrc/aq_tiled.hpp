@@ -12,7 +12,8 @@ namespace aq {
 // at the last aq_sort: int4 (block, first particle, count, 0).  The item's
 // tile covers the block, the 2-node stencil reach and a 1-cell drift margin
 // on each side: TE = E + 4 nodes per axis from block_origin - 1 (LDS layout
-// in Tile; in the tile buffer, TE^DIM nodes in (x, y, z) order).
+// in Tile; in the tile buffer, TE^DIM nodes grouped by the grid block they
+// belong to, Tile::store_pos).
 //
 // G2P reads grid velocities staged in LDS.  P2G accumulates in LDS in fixed
 // point (integer LDS atomics; float LDS atomics are ~30x slower on this
@@ -55,7 +56,6 @@ template <int DIM> struct Tile {
       t[2] = 0;
     }
   }
-  __device__ static int node_of(int x, int y, int z) { return DIM == 3 ? (x * TE + y) * TE + z : x * TE + y; }
   // Position of tile node (x, y, z) in the tile buffer.  Per axis the tile
   // covers the last node of the previous block (t = 0), the E nodes of its
   // own block (1..E) and the first 3 of the next (E+1..E+3); the buffer
