@@ -144,7 +144,7 @@ class Batch:
             sim.ids[:sim.current_idx] = torch.arange(sim.current_idx, dtype=torch.int32, device=sim.state.device)
             sim.step_count = 0
 
-    def autotune(self, candidates=(4, 8, 16), steps=16):
+    def autotune(self, candidates=(8, 16, 32), steps=32):
         """Sort interval per simulation: the candidate with the least time
         for `steps` substeps from the initial state (HIP events)."""
         for sp, sim in zip(self.specs, self.sims):
