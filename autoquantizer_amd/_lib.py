@@ -14,11 +14,20 @@ INCLUDE_DIR = os.path.join(os.path.dirname(_PKG), "include")
 
 
 def jit_cache_dir(path=None):
-    """Directory of cached specialised code objects (created if missing):
-    path, else $AQ_JIT_CACHE, else aqmpm_jit_<uid> in the system temp dir."""
+    """Directory of cached specialised code objects (created with mode 0700
+    if missing): path, else $AQ_JIT_CACHE, else aqmpm_jit_<uid> in the
+    system temp dir.  The library loads and runs the code objects it finds
+    there, so a directory owned by another user or writable by others is
+    refused."""
+    import stat
     import tempfile
     d = path or os.environ.get("AQ_JIT_CACHE") or os.path.join(tempfile.gettempdir(), f"aqmpm_jit_{os.getuid()}")
-    os.makedirs(d, exist_ok=True)
+    os.makedirs(d, mode=0o700, exist_ok=True)
+    st = os.stat(d)
+    if st.st_uid != os.geteuid():
+        raise AQError(f"JIT cache directory {d} is not owned by the current user")
+    if st.st_mode & (stat.S_IWGRP | stat.S_IWOTH):
+        raise AQError(f"JIT cache directory {d} is writable by other users")
     return d
 
 
@@ -87,6 +96,7 @@ EXPORTS = [
     "aq_update_limits", "aq_advance_limits", "aq_energy", "aq_sim_grid", "aq_advance_timed",
     "aq_adj_workspace_bytes", "aq_adj_substep", "aq_noise_add", "aq_noise_dot", "aq_codec_selftest", "aq_mlp_forward",
     "aq_mlp_backward", "aq_adam_step", "aq_sim_specialize", "aq_jit_build",
+    "aq_sim_count_wraps",
 ]
 
 _lib = None
@@ -130,12 +140,13 @@ def lib():
     L.aq_adj_substep.argtypes = [P, P, i64, P, P, P, P, i64, P]
     L.aq_sim_specialize.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]
     L.aq_jit_build.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, i64]
-    L.aq_noise_add.argtypes = [P, P, i64, P, u64, u32, P]
-    L.aq_noise_dot.argtypes = [P, P, i64, u64, u32, P, P]
+    L.aq_noise_add.argtypes = [P, P, i64, P, P, u64, u32, P]
+    L.aq_noise_dot.argtypes = [P, P, i64, P, u64, u32, P, P]
     L.aq_codec_selftest.argtypes = [P, P]
     L.aq_mlp_forward.argtypes = [P, P, i32, P, P, P]
     f32 = ctypes.c_float
     L.aq_mlp_backward.argtypes = [P, P, P, P, P, i32, f32, P, P, P, P]
+    L.aq_sim_count_wraps.argtypes = [P, P]
     L.aq_adam_step.argtypes = [P, P, P, P, i32, i32, f32, f32, f32, f32, P]
     for name in EXPORTS:
         if name not in ("aq_last_error", "aq_version", "aq_sim_workspace_bytes", "aq_adj_workspace_bytes"):

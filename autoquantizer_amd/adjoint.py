@@ -64,58 +64,74 @@ def _advance_plain(solver, state, nsteps):
 
 class Noise:
     """Modelled rounding noise: after substep t (1-based) the state gets
-    sigma[c] * xi(seed, t, c, p), xi uniform in [-1/2, 1/2) (aq_noise_add)."""
+    sigma[c] * xi(seed, t, c, id), xi uniform in [-1/2, 1/2), keyed by the
+    particle id (aq_noise_add), so that a sorted copy draws the same noise
+    per particle as the unsorted state."""
 
     def __init__(self, sigma, seed=0):
         self.sigma = np.ascontiguousarray(sigma, dtype=np.float64)
         self.seed = int(seed)
 
-    def add(self, solver, state, step):
+    def add(self, solver, state, step, ids=None):
         _lib.check(_lib.lib().aq_noise_add(solver._h, _ptr(state), solver.current_idx,
+                                           _ptr(ids) if ids is not None else None,
                                            self.sigma.ctypes.data_as(ctypes.c_void_p), self.seed, int(step),
                                            _stream()))
 
-    def dot(self, solver, adj, step, out):
-        _lib.check(_lib.lib().aq_noise_dot(solver._h, _ptr(adj), solver.current_idx, self.seed, int(step), _ptr(out),
+    def dot(self, solver, adj, step, out, ids=None):
+        _lib.check(_lib.lib().aq_noise_dot(solver._h, _ptr(adj), solver.current_idx,
+                                           _ptr(ids) if ids is not None else None, self.seed, int(step), _ptr(out),
                                            _stream()))
 
 
 def _sorted_copy(solver, state):
     """A copy of `state` sorted by grid block (aq_sort), so that the tiled
-    kernels run on it, and the original index of each sorted slot."""
+    kernels run on it; returns (copy, original index of each sorted slot
+    (int64 [n]), particle ids of the slots (int32 [capacity]))."""
     n = solver.current_idx
     w = state.clone()
     ids = torch.arange(w.shape[1], dtype=torch.int32, device=w.device)
     _lib.check(_lib.lib().aq_sort(solver._h, _ptr(w), n, _ptr(ids), _stream()))
-    return w, ids[:n].long()
+    return w, ids[:n].long(), ids
 
 
 class _Schedule:
-    """Checkpointed forward and reverse passes.  Without noise each segment
-    runs on a sorted copy of its checkpoint (LDS-tiled kernels, like the
-    forward solver); states, checkpoints and adjoints at segment boundaries
-    stay in the original particle order, and the adjoint is carried through
-    the segment's permutation.  With noise (keyed by particle slot) the
-    segments run unsorted."""
+    """Checkpointed forward and reverse passes.  Each segment runs on a
+    sorted copy of its checkpoint (LDS-tiled kernels, like the forward
+    solver); states, checkpoints and adjoints at segment boundaries stay in
+    the original particle order, and the adjoint is carried through the
+    segment's permutation.  Noise is keyed by particle id, so it does not
+    depend on the order.
 
-    def __init__(self, solver, nsteps, checkpoint_every, noise=None):
+    adjoint_solver (optional): a solver of the same scene in another
+    precision that runs the reverse pass (e.g. the fp32 adjoint kernels
+    around an f64 forward): each of its segments sorts a copy of the
+    checkpoint with its own sort, and the forward's states are cast and
+    permuted into that order."""
+
+    def __init__(self, solver, nsteps, checkpoint_every, noise=None, adjoint_solver=None):
         if checkpoint_every < 1:
             raise ValueError("checkpoint_every must be >= 1")
         if noise is not None and len(noise.sigma) != solver.ncomp:
             raise ValueError("sigma needs one entry per component")
+        if adjoint_solver is not None and (adjoint_solver.ncomp != solver.ncomp
+                                           or adjoint_solver.current_idx != solver.current_idx
+                                           or adjoint_solver.capacity != solver.capacity):
+            raise ValueError("the adjoint solver must hold the same scene")
         self.solver, self.nsteps, self.k = solver, int(nsteps), int(checkpoint_every)
         self.noise = noise
-        self.sorted = noise is None
+        self.adj_solver = adjoint_solver if adjoint_solver is not None else solver
         self.checkpoints = []
 
-    def _advance(self, s, t0, nsteps):
-        """nsteps substeps of s, the first producing state t0 + 1."""
+    def _advance(self, s, ids, t0, nsteps):
+        """nsteps substeps of s (sorted, particle ids `ids`), the first
+        producing state t0 + 1."""
         if self.noise is None:
             _advance_plain(self.solver, s, nsteps)
             return
         for t in range(t0 + 1, t0 + nsteps + 1):
             _advance_plain(self.solver, s, 1)
-            self.noise.add(self.solver, s, t)
+            self.noise.add(self.solver, s, t, ids)
 
     def forward(self, state0):
         """Runs the forward from state0 ([ncomp, capacity]); returns s_T."""
@@ -124,65 +140,64 @@ class _Schedule:
         for t0 in range(0, self.nsteps, self.k):
             self.checkpoints.append(s.clone())
             steps = min(self.k, self.nsteps - t0)
-            if self.sorted:
-                w, idx = _sorted_copy(self.solver, s)
-                self._advance(w, t0, steps)
-                s[:, idx] = w[:, :n]
-            else:
-                self._advance(s, t0, steps)
+            w, idx, ids = _sorted_copy(self.solver, s)
+            self._advance(w, ids, t0, steps)
+            s[:, idx] = w[:, :n]
         return s
 
     def backward(self, adj_T, gsum=None, adj=None, noise_dot=None):
-        """dL/ds_0 from dL/ds_T, adding the squared adjoints of states
-        1..T-1 into gsum and, with noise, sum_p dL/ds_t xi_t (t = 1..T) into
-        noise_dot (dL/dsigma)."""
+        """dL/ds_0 from dL/ds_T (tensors of the adjoint solver's dtype),
+        adding the squared adjoints of states 1..T-1 into gsum and, with
+        noise, sum_p dL/ds_t xi_t (t = 1..T) into noise_dot (dL/dsigma)."""
         a = adj_T.clone()
         b = torch.empty_like(a)
         n = self.solver.current_idx
+        mixed = self.adj_solver is not self.solver
         for j in reversed(range(len(self.checkpoints))):
             t0 = j * self.k
             t1 = min(t0 + self.k, self.nsteps)
-            if self.sorted:
-                # the segment on a sorted copy: its states are kept, and each is copied back into the
-                # sorted buffer (the one the sort's work items describe) for its adjoint step
-                w, idx = _sorted_copy(self.solver, self.checkpoints[j])
-                states = [w.clone()]
-                for t in range(t0 + 1, t1):
-                    self._advance(w, t - 1, 1)
-                    states.append(w.clone())
-                a[:, :n] = a[:, idx]
+            # the segment's states, in the order of the forward's sorted copy
+            w, idx, ids = _sorted_copy(self.solver, self.checkpoints[j])
+            states = [w.clone()]
+            for t in range(t0 + 1, t1):
+                self._advance(w, ids, t - 1, 1)
+                states.append(w.clone())
+            if mixed:
+                # the adjoint solver's own sorted copy, and where each of its slots sits in the forward's order
+                wa, idx_a, ids_a = _sorted_copy(self.adj_solver, self.checkpoints[j].to(a.dtype))
+                slot = torch.empty_like(idx)
+                slot[idx] = torch.arange(n, device=idx.device)
+                gather = slot[idx_a]
             else:
-                states = [self.checkpoints[j]]
-                for t in range(t0 + 1, t1):
-                    s = states[-1].clone()
-                    self._advance(s, t - 1, 1)
-                    states.append(s)
+                wa, idx_a, ids_a = w, idx, ids
+            a[:, :n] = a[:, idx_a]
             for t in reversed(range(t0, t1)):
                 if noise_dot is not None:
-                    self.noise.dot(self.solver, a, t + 1, noise_dot)
-                if self.sorted:
-                    w.copy_(states[t - t0])
-                    adj.step(w, a, b, gsum if t >= 1 else None)
+                    self.noise.dot(self.adj_solver, a, t + 1, noise_dot, ids_a)
+                st = states[t - t0]
+                if mixed:
+                    wa[:, :n] = st[:, gather].to(a.dtype)
                 else:
-                    adj.step(states[t - t0], a, b, gsum if t >= 1 else None)
+                    wa.copy_(st)
+                adj.step(wa, a, b, gsum if t >= 1 else None)
                 a, b = b, a
-            if self.sorted:
-                b[:, idx] = a[:, :n]
-                a[:, :n] = b[:, :n]
+            b[:, idx_a] = a[:, :n]
+            a[:, :n] = b[:, :n]
         return a
 
 
 class Rollout(torch.autograd.Function):
-    """state_T = Rollout.apply(state_0, solver, nsteps, checkpoint_every, noise):
-    nsteps plain substeps (with a Noise after each when noise is given) as a
-    differentiable torch op.  backward runs the hand-written adjoint; the
-    error-contribution sums of the backward pass are left in
-    solver.adjoint_gsum and, with noise, dL/dsigma in solver.adjoint_dsigma
-    (float64 [ncomp] on the device)."""
+    """state_T = Rollout.apply(state_0, solver, nsteps, checkpoint_every, noise,
+    adjoint_solver): nsteps plain substeps (with a Noise after each when
+    noise is given) as a differentiable torch op.  backward runs the
+    hand-written adjoint (of adjoint_solver when given: e.g. the fp32 adjoint
+    kernels around an f64 forward); the error-contribution sums of the
+    backward pass are left in solver.adjoint_gsum and, with noise, dL/dsigma
+    in solver.adjoint_dsigma (float64 [ncomp] on the device)."""
 
     @staticmethod
-    def forward(ctx, state0, solver, nsteps, checkpoint_every=16, noise=None):
-        sched = _Schedule(solver, nsteps, checkpoint_every, noise)
+    def forward(ctx, state0, solver, nsteps, checkpoint_every=16, noise=None, adjoint_solver=None):
+        sched = _Schedule(solver, nsteps, checkpoint_every, noise, adjoint_solver)
         out = sched.forward(state0.detach())
         ctx.sched = sched
         return out
@@ -191,13 +206,14 @@ class Rollout(torch.autograd.Function):
     def backward(ctx, grad_T):
         sched = ctx.sched
         solver = sched.solver
-        adj = SubstepAdjoint(solver)
+        adj = SubstepAdjoint(sched.adj_solver)
         gsum = torch.zeros(solver.ncomp, dtype=torch.float64, device=solver.device)
         dsig = None if sched.noise is None else torch.zeros_like(gsum)
-        grad0 = sched.backward(grad_T.contiguous().to(solver.state.dtype), gsum, adj, dsig)
+        adt = sched.adj_solver.state.dtype
+        grad0 = sched.backward(grad_T.contiguous().to(adt), gsum, adj, dsig)
         solver.adjoint_gsum = gsum
         solver.adjoint_dsigma = dsig
-        return grad0, None, None, None, None
+        return grad0.to(grad_T.dtype), None, None, None, None, None
 
 
 def error_contributions(solver, nsteps, loss=energy_loss, checkpoint_every=16):
@@ -224,16 +240,18 @@ def final_position_error(solver, state, xref):
     return (e * e).sum()
 
 
-def noise_scale_gradients(solver, nsteps, sigma, xref, seed=0, checkpoint_every=16):
+def noise_scale_gradients(solver, nsteps, sigma, xref, seed=0, checkpoint_every=16, adjoint_solver=None):
     """The adjoint error-gradient run: nsteps substeps from the solver's
     current state with modelled rounding noise sigma[c] * xi after every
     substep, L = |x_T - xref|^2, and dL/dsigma[c] = sum_t sum_p
     dL/ds_t[c,p] xi_t[c,p] from the hand-written adjoint (products and sums
-    in f64).  Returns (L, dL/dsigma as numpy [ncomp]).  The solver's state
-    is left unchanged."""
+    in f64).  adjoint_solver (optional, same scene): runs the reverse pass
+    in its precision (the fp32 adjoint around an f64 forward, whose
+    x_T - xref resolves noise far below fp32's rounding of x).  Returns (L,
+    dL/dsigma as numpy [ncomp]).  The solver's state is left unchanged."""
     xr = torch.as_tensor(np.asarray(xref), dtype=torch.float64, device=solver.device)
     s0 = solver.state.clone().requires_grad_(True)
-    sT = Rollout.apply(s0, solver, nsteps, checkpoint_every, Noise(sigma, seed))
+    sT = Rollout.apply(s0, solver, nsteps, checkpoint_every, Noise(sigma, seed), adjoint_solver)
     L = final_position_error(solver, sT, xr)
     L.backward()
     torch.cuda.current_stream().synchronize()

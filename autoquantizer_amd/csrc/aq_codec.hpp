@@ -55,9 +55,10 @@ struct CodecK {
   uint32_t shared_mask;  // bit c: member of a shared-exponent group
   uint32_t gfirst_mask;  // bit c: first member of its group
   uint32_t glast_mask;   // bit c: last member of its group
-  uint32_t key_base;     // dithering key of component c: hash32(c ^ key_base)
+  uint32_t key_base;     // dithering key of component pair j: hash32(j ^ key_base)
   uint32_t stream;       // fields lie in encode order at increasing offsets (StreamSink applies)
   uint32_t pad;
+  unsigned long long *wraps;  // per component: stores whose code wrapped (null: not counted)
 };
 
 // Descriptors are loaded where they are used, a few at a time: the empty asm
@@ -152,22 +153,22 @@ __device__ __forceinline__ void row_set(uint32_t *row, uint32_t off, uint32_t nb
 }
 
 // fptosi / fptoui with saturation and NaN -> 0 (the reference's values are
-// LLVM poison outside the int range); branch-free: the conversion runs on the
-// value clamped to the largest float inside the range, and the saturated and
-// NaN cases are selected afterwards.
+// LLVM poison outside the int range): the hardware conversions
+// v_cvt_i32_f32 / v_cvt_u32_f32 truncate toward zero, clamp to the integer
+// range and turn NaN into 0 (CDNA3 ISA, VOP1 conversions), which is exactly
+// this semantics, in one instruction.  (The C conversion would let the
+// compiler treat out-of-range inputs as undefined.)
 __device__ __forceinline__ uint32_t cvt_sat(float y, bool is_signed) {
-  if (is_signed) {
-    int32_t r = (int32_t)fminf(fmaxf(y, -2147483648.0f), 2147483520.0f);
-    r = y >= 2147483648.0f ? 0x7FFFFFFF : r;
-    return y == y ? (uint32_t)r : 0u;
-  }
-  uint32_t r = (uint32_t)fminf(fmaxf(y, 0.0f), 4294967040.0f);
-  r = y >= 4294967296.0f ? 0xFFFFFFFFu : r;
-  return y > 0.0f ? r : 0u;
+  uint32_t r;
+  if (is_signed)
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(y));
+  else
+    asm("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(y));
+  return r;
 }
 
-// Dithering noise in [0, 1): lowbias32 (C. Wellons, hash-prospector) of the
-// record slot xor a per-(seed, step, component) key.
+// lowbias32 (C. Wellons, hash-prospector): xorshift-multiply rounds with
+// constants 0x7feb352d and 0x846ca68b.
 __device__ __forceinline__ uint32_t hash32(uint32_t x) {
   x ^= x >> 16;
   x *= 0x7feb352du;
@@ -176,25 +177,60 @@ __device__ __forceinline__ uint32_t hash32(uint32_t x) {
   x ^= x >> 16;
   return x;
 }
+// Uniform noise in [0, 1) on a 2^-24 grid: the hash of the record slot xor a
+// per-(seed, step, component) key (the modelled rounding noise of the
+// adjoint run).
 __device__ __forceinline__ float noise_u(uint32_t slot, uint32_t key) {
   return (float)(hash32(slot ^ key) >> 8) * (1.0f / 16777216.0f);
 }
+// Dithering noise of component c: one hash serves the pair of components
+// (2j, 2j+1), key_j = hash32(j ^ key_base); component c takes half (c & 1)
+// of it, u = (h16 + 1/2) 2^-16, exact, in (0, 1) with mean 1/2.
+__device__ __forceinline__ float dither_u(uint32_t slot, uint32_t pair_key, int c) {
+  const uint32_t h = hash32(slot ^ pair_key);
+  const uint32_t b = (c & 1) ? (h >> 16) : (h & 0xFFFFu);
+  return __uint_as_float(0x3F800040u | (b << 7)) - 1.0f;
+}
 
-__device__ __forceinline__ uint32_t enc_fixed(float v, const CompK &c, float u) {
-  float y = v - c.offs;
+// Fixed-point code of v before masking to its width: the saturated integer
+// (cvt_sat) of y, the rounded scaled value it truncates.
+__device__ __forceinline__ uint32_t enc_fixed_sat(float v, const CompK &c, float u, float &y) {
+  y = v - c.offs;
   y = y * c.enc_s;
   if (c.dither) {
     y = y + u;
     y = y - 0.5f;
   }
   y = y + __builtin_copysignf(0.5f, y);
-  return cvt_sat(y, c.is_signed) & lowmask(c.nbits);
+  return cvt_sat(y, c.is_signed);
+}
+
+__device__ __forceinline__ uint32_t enc_fixed(float v, const CompK &c, float u) {
+  float y;
+  return enc_fixed_sat(v, c, u, y) & lowmask(c.nbits);
+}
+
+// Whether the code of a fixed or int member wraps: trunc(y) outside the
+// nb-bit range (the stored code then aliases another value), or y is NaN.
+// q is cvt_sat(y).
+__device__ __forceinline__ bool code_wraps(float y, uint32_t q, uint32_t nb, bool is_signed) {
+  if (y != y) return true;
+  if (is_signed) {
+    if (nb >= 32u) return y >= 2147483648.0f || y < -2147483648.0f;
+    const uint32_t sh = 32u - nb;
+    return (int32_t)(q << sh) >> sh != (int32_t)q;
+  }
+  if (y <= -1.0f) return true;
+  if (nb >= 32u) return y >= 4294967296.0f;
+  return (q >> nb) != 0u;
 }
 
 __device__ __forceinline__ float dec_fixed(uint32_t q, const CompK &c) {
   const float cf = c.is_signed ? (float)sext(q, c.nbits) : (float)q;
   const float t = cf * c.dec_s;
-  return t + c.offs;
+  // cf is never -0 and dec_s > 0 (checked at layout time), so t is never
+  // -0 and t + 0 == t: a zero offset needs no add
+  return c.offs == 0.0f ? t : t + c.offs;
 }
 
 __device__ __forceinline__ uint32_t f32_exp(float f) {
@@ -344,11 +380,6 @@ __device__ __forceinline__ void encode_member(float v, const CompK &k, float u, 
   }
 }
 
-// Dithering noise of component c of the record with noise key `slot`.
-__device__ __forceinline__ float comp_noise(const CodecK &cod, int c, uint32_t slot) {
-  return noise_u(slot, hash32((uint32_t)c ^ cod.key_base));
-}
-
 // Field writers of the record encoder: put(off, nbits, value) with value
 // masked to nbits.
 // RowSink ORs fields into a zeroed per-thread LDS row (any layout).
@@ -441,6 +472,8 @@ __device__ __forceinline__ void encode_record(const float (&v)[NC], const CodecK
       gmax[c] = run;
     }
   }
+  const bool count_wraps = cod.wraps != nullptr;  // uniform
+  uint32_t wrapped = 0u;
   ConstU32 *table = nullptr;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
@@ -453,8 +486,12 @@ __device__ __forceinline__ void encode_record(const float (&v)[NC], const CodecK
       sink.put(k.off, k.nbits, d);
       dec[c] = dec_shared_enc(e, d, k);
     } else if (k.kind == K_FIXED) {
-      const float u = k.dither ? noise_u(slot, hash32((uint32_t)c ^ kb)) : 0.0f;
-      const uint32_t q = enc_fixed(v[c], k, u);
+      // both members of a component pair hash the same (slot, key): one hash per pair
+      const float u = k.dither ? dither_u(slot, hash32((uint32_t)c >> 1 ^ kb), c) : 0.0f;
+      float y;
+      const uint32_t qs = enc_fixed_sat(v[c], k, u, y);
+      const uint32_t q = qs & lowmask(k.nbits);
+      if (count_wraps && code_wraps(y, qs, k.nbits, k.is_signed)) wrapped |= 1u << c;
       sink.put(k.off, k.nbits, q);
       dec[c] = dec_fixed(q, k);
     } else if (k.kind == K_FLOAT) {
@@ -467,11 +504,17 @@ __device__ __forceinline__ void encode_record(const float (&v)[NC], const CodecK
       sink.put(k.off, 32u, __float_as_uint(v[c]));
       dec[c] = v[c];
     } else {
-      const uint32_t q = cvt_sat(v[c], k.is_signed) & lowmask(k.nbits);
+      const uint32_t qs = cvt_sat(v[c], k.is_signed);
+      const uint32_t q = qs & lowmask(k.nbits);
+      if (count_wraps && code_wraps(v[c], qs, k.nbits, k.is_signed)) wrapped |= 1u << c;
       sink.put(k.off, k.nbits, q);
       dec[c] = k.is_signed ? (float)sext(q, k.nbits) : (float)q;
     }
   }
+  // range-violation counter: the members whose code wrapped (rare; the
+  // whole test is skipped when no counter is set)
+  if (count_wraps)
+    for (uint32_t m = wrapped; m; m &= m - 1u) atomicAdd(cod.wraps + __builtin_ctz(m), 1ull);
 }
 
 }  // namespace aq

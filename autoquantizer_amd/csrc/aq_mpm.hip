@@ -2,6 +2,8 @@
 // Built for gfx950 only (see autoquantizer_amd/build.py).
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <cmath>
 #include <cstdio>
@@ -64,11 +66,14 @@ static int build_layout(const aq_component *comps, int nc, int nw, LayoutK *out)
     k.dither = (c.kind == AQ_FIXED && c.use_dithering) ? 1 : 0;
     k.off = (uint16_t)c.bit_offset;
     if (c.kind == AQ_FIXED) {
-      if (!(c.scale != 0.0) || !std::isfinite(c.scale) || !std::isfinite(c.offset))
-        return fail(AQ_ERR_INVALID, "fixed-point scale must be finite and non-zero");
+      if (!(c.scale > 0.0) || !std::isfinite(c.scale) || !std::isfinite(c.offset))
+        return fail(AQ_ERR_INVALID, "fixed-point scale must be positive and finite");
       k.enc_s = (float)(1.0 / c.scale);
       k.dec_s = (float)c.scale;
       k.offs = (float)c.offset;
+      // the decode relies on a positive normal f32 scale (dec_fixed)
+      if (!(k.dec_s >= 1.17549435e-38f) || !std::isfinite(k.enc_s) || !std::isfinite(k.dec_s))
+        return fail(AQ_ERR_INVALID, "fixed-point scale outside the f32 normal range");
     }
     if (c.kind == AQ_FLOAT) {
       if (c.exp_bits < 1 || c.exp_bits > 8) return fail(AQ_ERR_INVALID, "exponent bits must be in [1, 8]");
@@ -105,8 +110,10 @@ struct KeysK {
   uint32_t k[AQ_MAX_COMPS];
 };
 
-// Dithering keys of one (seed, step): key_c = h(c ^ key_base),
-// key_base = h(step ^ h(seed_lo ^ h(seed_hi))).
+// Keys of one (seed, step): key_j = h(j ^ key_base), key_base = h(step ^
+// h(seed_lo ^ h(seed_hi))).  Dithering: component c uses key_{c >> 1} (one
+// hash per component pair, dither_u); modelled noise: component c uses
+// key_c (noise_u).
 static uint32_t host_hash32(uint32_t x) {
   x ^= x >> 16;
   x *= 0x7feb352du;
@@ -163,7 +170,7 @@ __global__ void __launch_bounds__(256) k_pack(LayoutK lay, KeysK keys, const flo
       row_put(row, k.off, k.nbits, enc_shared(v, gmax, k));
     } else {
       float u = 0.0f;
-      if (k.kind == K_FIXED && k.dither) u = noise ? noise[(int64_t)c * n + p] : noise_u((uint32_t)p, keys.k[c]);
+      if (k.kind == K_FIXED && k.dither) u = noise ? noise[(int64_t)c * n + p] : dither_u((uint32_t)p, keys.k[c >> 1], c);
       encode_member(v, k, u, row, false);
     }
   }
@@ -252,7 +259,7 @@ __global__ void __launch_bounds__(256) k_store_members(LayoutK lay, KeysK keys, 
       }
       row_set(row, k.off, k.nbits, enc_shared(vv[c], gmax, k));
     } else {
-      const float u = (k.kind == K_FIXED && k.dither) ? noise_u((uint32_t)p, keys.k[c]) : 0.0f;
+      const float u = (k.kind == K_FIXED && k.dither) ? dither_u((uint32_t)p, keys.k[c >> 1], c) : 0.0f;
       encode_member(vv[c], k, u, row, true);
     }
   }
@@ -490,10 +497,12 @@ template <int DIM> __device__ __forceinline__ void lane_coords(int lane, int (&l
 // block and the one on the side of the block edge it is nearest); each lane
 // reads the item ranges of its 2^DIM blocks and the first two items' tile
 // values all at once (independent loads), then any further items, and sums
-// in a fixed order (neighbour, then item); grid_f is added for blocks flagged
-// by untiled scatters (and zeroed after use).  OP applies the grid update;
-// without it grid_v holds (m v, m).  Blocks with no contribution are marked
-// inactive in `act` (and zeroed if they were active).
+// in a fixed order (neighbour, then item).  A node in a tile's rim is read
+// only when the item's rim flags say it was written (otherwise it is zero);
+// grid_f is added for blocks flagged by untiled scatters (and zeroed after
+// use).  OP applies the grid update; without it grid_v holds (m v, m).
+// Blocks with no contribution are marked inactive in `act` (and zeroed if
+// they were active).
 template <int DIM, class T, bool OP>
 __global__ void __launch_bounds__(256) k_gather(ParamsT<T> P, int32_t nblocks, const int32_t *__restrict__ item_off,
                                                 const uint32_t *__restrict__ near,
@@ -501,7 +510,7 @@ __global__ void __launch_bounds__(256) k_gather(ParamsT<T> P, int32_t nblocks, c
                                                 uint32_t *__restrict__ fflags, vec4_t<T> *__restrict__ grid_v,
                                                 uint32_t *__restrict__ act) {
   using TL = Tile<DIM>;
-  constexpr int E = TL::E, TN = TL::TN, NN = 1 << DIM;
+  constexpr int E = TL::E, TN = TL::TN, ST = TL::STRIDE, NN = 1 << DIM;
   const int32_t b = __builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * 4u + (threadIdx.x >> 6)));
   const int lane = threadIdx.x & 63;
   if (b >= nblocks) return;
@@ -523,6 +532,7 @@ __global__ void __launch_bounds__(256) k_gather(ParamsT<T> P, int32_t nblocks, c
   block_coords<DIM>(b, nb, bc);
   lane_coords<DIM>(lane, loc);
   int32_t r0[NN], r1[NN], q[NN];
+  uint32_t need[NN];
 #pragma unroll
   for (int k = 0; k < NN; ++k) {
     int d[3] = {0, 0, 0}, t[3] = {0, 0, 0};
@@ -534,7 +544,8 @@ __global__ void __launch_bounds__(256) k_gather(ParamsT<T> P, int32_t nblocks, c
       t[a] = loc[a] + 1 - d[a] * E;
       ok = ok && c >= 0 && c < nb && t[a] < TL::TE;  // 2D: only 3 nodes reach past the edge
     }
-    q[k] = TL::store_pos(t[0], t[1], t[2]);
+    q[k] = ok ? tile_pos<DIM>(t[0], t[1], t[2]) : 0;
+    need[k] = TL::need(t[0], t[1], t[2]);
     r0[k] = r1[k] = 0;
     if (ok) {
       const int32_t nbid = DIM == 3 ? ((bc[0] + d[0]) * nb + (bc[1] + d[1])) * nb + (bc[2] + d[2])
@@ -558,13 +569,17 @@ __global__ void __launch_bounds__(256) k_gather(ParamsT<T> P, int32_t nblocks, c
     }
     return;
   }
+  // rim faces of item `it` reached by its particles (stored after its nodes)
+  auto rim = [&](int32_t it) { return *(const uint32_t *)(tiles + (int64_t)it * ST + TN); };
   vec4_t<T> c0[NN], c1[NN];
 #pragma unroll
   for (int k = 0; k < NN; ++k) {
     c0[k].x = c0[k].y = c0[k].z = c0[k].w = (T)0;
     c1[k] = c0[k];
-    if (r1[k] > r0[k]) c0[k] = tiles[(int64_t)r0[k] * TN + q[k]];
-    if (r1[k] > r0[k] + 1) c1[k] = tiles[(int64_t)(r0[k] + 1) * TN + q[k]];
+    if (r1[k] > r0[k] && (need[k] == 0u || (rim(r0[k]) & need[k]) == need[k]))
+      c0[k] = tiles[(int64_t)r0[k] * ST + q[k]];
+    if (r1[k] > r0[k] + 1 && (need[k] == 0u || (rim(r0[k] + 1) & need[k]) == need[k]))
+      c1[k] = tiles[(int64_t)(r0[k] + 1) * ST + q[k]];
   }
   vec4_t<T> sum;
   sum.x = sum.y = sum.z = sum.w = (T)0;
@@ -579,7 +594,8 @@ __global__ void __launch_bounds__(256) k_gather(ParamsT<T> P, int32_t nblocks, c
     sum.z += c1[k].z;
     sum.w += c1[k].w;
     for (int32_t it = r0[k] + 2; it < r1[k]; ++it) {
-      const vec4_t<T> c = tiles[(int64_t)it * TN + q[k]];
+      if (need[k] != 0u && (rim(it) & need[k]) != need[k]) continue;
+      const vec4_t<T> c = tiles[(int64_t)it * ST + q[k]];
       sum.x += c.x;
       sum.y += c.y;
       sum.z += c.z;
@@ -997,6 +1013,7 @@ struct aq_sim {
   int64_t ws_bytes;
   hipModule_t jit_mod = nullptr;      // fused kernel specialised to the layout (aq_sim_specialize)
   hipFunction_t jit_fn = nullptr;
+  unsigned long long *wraps = nullptr;  // range-violation counters (aq_sim_count_wraps)
 };
 
 // Whether the record's fields, in the encoder's order (components in order,
@@ -1053,7 +1070,7 @@ static int32_t max_items_of(const aq_sim_desc *d) {
   return (int32_t)(d->capacity / kItem + ne + 1);
 }
 
-static int64_t tile_slots(const aq_sim_desc *d) { return d->dim == 3 ? 512 : 144; }  // Tile<DIM>::TN
+static int64_t tile_slots(const aq_sim_desc *d) { return d->dim == 3 ? Tile<3>::STRIDE : Tile<2>::STRIDE; }
 
 enum {
   WS_GRID_V, WS_GRID_F, WS_FLAGS, WS_KEY, WS_RANK, WS_COUNT, WS_CCOUNT, WS_OFF, WS_PART, WS_TMP_STATE, WS_TMP_IDS,
@@ -1241,6 +1258,7 @@ static CodecK make_codec(const aq_sim *s, uint32_t step) {
   c.key_base = key_base(s->d.seed, step);
   c.stream = s->stream_order ? 1u : 0u;
   c.pad = 0u;
+  c.wraps = s->wraps;
   return c;
 }
 
@@ -1347,8 +1365,23 @@ static int jit_code(const aq_sim &v, const char *src_dir, const char *include_di
   }
   if (!read_file(std::string(include_dir) + "/aq_mpm.h", text)) return fail(AQ_ERR_INVALID, "aq_mpm.h not found");
   h = fnv1a(h, text.data(), text.size());
+  // a different compiler gives a different code object
+  int rtc_major = 0, rtc_minor = 0;
+  if (hiprtcVersion(&rtc_major, &rtc_minor) != HIPRTC_SUCCESS) return fail(AQ_ERR_HIP, "hiprtcVersion failed");
+  const int rtc_ver[2] = {rtc_major, rtc_minor};
+  h = fnv1a(h, rtc_ver, sizeof rtc_ver);
+  const int hip_ver = HIP_VERSION;
+  h = fnv1a(h, &hip_ver, sizeof hip_ver);
   std::string file;
   if (cache_dir) {
+    // code objects found here are loaded and run: only a directory that no
+    // other user can write to is trusted
+    struct stat cs;
+    if (::stat(cache_dir, &cs) != 0 || !S_ISDIR(cs.st_mode))
+      return fail(AQ_ERR_INVALID, "JIT cache directory does not exist");
+    if (cs.st_uid != ::geteuid()) return fail(AQ_ERR_INVALID, "JIT cache directory is not owned by the current user");
+    if (cs.st_mode & (S_IWGRP | S_IWOTH))
+      return fail(AQ_ERR_INVALID, "JIT cache directory is writable by other users");
     char name[64];
     std::snprintf(name, sizeof name, "/aqjit_%016llx.co", (unsigned long long)h);
     file = std::string(cache_dir) + name;
@@ -1884,19 +1917,22 @@ extern "C" int aq_energy(aq_sim *s, const void *state, int64_t n, double *out, v
 
 // ---- modelled quantization noise of the adjoint configuration ------------
 // xi(step, c, p) = u - 1/2 with u the dithering noise of (seed, step,
-// component c, particle slot p): uniform in [-1/2, 1/2), 24-bit grid.
+// component c, particle id p): uniform in [-1/2, 1/2), 24-bit grid.  The id
+// is ids[slot] when ids is given (a sorted state carries its particles'
+// ids), else the slot.
 struct SigmaK {
   double s[AQ_MAX_COMPS];
 };
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_noise_add(T *__restrict__ state, int64_t cap, int64_t n, int32_t nc,
-                                                   SigmaK sig, KeysK keys) {
+                                                   const int32_t *__restrict__ ids, SigmaK sig, KeysK keys) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
+  const uint32_t id = ids ? (uint32_t)ids[p] : (uint32_t)p;
   for (int c = 0; c < nc; ++c) {
     if (sig.s[c] == 0.0) continue;
-    const T xi = (T)(noise_u((uint32_t)p, keys.k[c]) - 0.5f);
+    const T xi = (T)(noise_u(id, keys.k[c]) - 0.5f);
     state[(int64_t)c * cap + p] += (T)sig.s[c] * xi;
   }
 }
@@ -1904,13 +1940,15 @@ __global__ void __launch_bounds__(256) k_noise_add(T *__restrict__ state, int64_
 // out[c] += sum_p adj[c][p] xi(step, c, p), products and sums in double.
 template <typename T>
 __global__ void __launch_bounds__(256) k_noise_dot(const T *__restrict__ adj, int64_t cap, int64_t n, int32_t nc,
-                                                   KeysK keys, double *__restrict__ out) {
+                                                   const int32_t *__restrict__ ids, KeysK keys,
+                                                   double *__restrict__ out) {
   __shared__ double part[4][AQ_MAX_COMPS];
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t id = p < n ? (ids ? (uint32_t)ids[p] : (uint32_t)p) : 0u;
   for (int c = 0; c < nc; ++c) {
     double q = 0.0;
-    if (p < n) q = (double)adj[(int64_t)c * cap + p] * (double)(noise_u((uint32_t)p, keys.k[c]) - 0.5f);
+    if (p < n) q = (double)adj[(int64_t)c * cap + p] * (double)(noise_u(id, keys.k[c]) - 0.5f);
     for (int o = 32; o > 0; o >>= 1) q += __shfl_xor(q, o);
     if (lane == 0) part[wave][c] = q;
   }
@@ -1921,8 +1959,8 @@ __global__ void __launch_bounds__(256) k_noise_dot(const T *__restrict__ adj, in
   }
 }
 
-extern "C" int aq_noise_add(aq_sim *s, void *state, int64_t n, const double *sigma, uint64_t seed, uint32_t step,
-                            void *stream) {
+extern "C" int aq_noise_add(aq_sim *s, void *state, int64_t n, const int32_t *ids, const double *sigma,
+                            uint64_t seed, uint32_t step, void *stream) {
   int rc = check_n(s, state, n);
   if (rc) return rc;
   if (s->d.storage == AQ_STORE_QUANT) return fail(AQ_ERR_INVALID, "noise is added to plain (f32/f64) storage");
@@ -1937,16 +1975,16 @@ extern "C" int aq_noise_add(aq_sim *s, void *state, int64_t n, const double *sig
   hipStream_t st = (hipStream_t)stream;
   if (s->d.storage == AQ_STORE_F64)
     hipLaunchKernelGGL(k_noise_add<double>, dim3(pblocks(n)), dim3(256), 0, st, (double *)state, s->d.capacity, n,
-                       s->d.ncomp, sig, keys);
+                       s->d.ncomp, ids, sig, keys);
   else
     hipLaunchKernelGGL(k_noise_add<float>, dim3(pblocks(n)), dim3(256), 0, st, (float *)state, s->d.capacity, n,
-                       s->d.ncomp, sig, keys);
+                       s->d.ncomp, ids, sig, keys);
   LAUNCH_CHECK();
   return AQ_OK;
 }
 
-extern "C" int aq_noise_dot(aq_sim *s, const void *adj, int64_t n, uint64_t seed, uint32_t step, double *out,
-                            void *stream) {
+extern "C" int aq_noise_dot(aq_sim *s, const void *adj, int64_t n, const int32_t *ids, uint64_t seed, uint32_t step,
+                            double *out, void *stream) {
   int rc = check_n(s, adj, n);
   if (rc) return rc;
   if (s->d.storage == AQ_STORE_QUANT) return fail(AQ_ERR_INVALID, "the adjoint runs on plain (f32/f64) storage");
@@ -1956,11 +1994,18 @@ extern "C" int aq_noise_dot(aq_sim *s, const void *adj, int64_t n, uint64_t seed
   hipStream_t st = (hipStream_t)stream;
   if (s->d.storage == AQ_STORE_F64)
     hipLaunchKernelGGL(k_noise_dot<double>, dim3(pblocks(n)), dim3(256), 0, st, (const double *)adj, s->d.capacity,
-                       n, s->d.ncomp, keys, out);
+                       n, s->d.ncomp, ids, keys, out);
   else
     hipLaunchKernelGGL(k_noise_dot<float>, dim3(pblocks(n)), dim3(256), 0, st, (const float *)adj, s->d.capacity, n,
-                       s->d.ncomp, keys, out);
+                       s->d.ncomp, ids, keys, out);
   LAUNCH_CHECK();
+  return AQ_OK;
+}
+
+extern "C" int aq_sim_count_wraps(aq_sim *s, unsigned long long *counts) {
+  if (!s) return fail(AQ_ERR_INVALID, "null handle");
+  if (counts && s->d.storage != AQ_STORE_QUANT) return fail(AQ_ERR_INVALID, "only quantized records wrap");
+  s->wraps = counts;
   return AQ_OK;
 }
 

@@ -10,6 +10,7 @@ Every phase runs in libaqmpm on the GPU; the state lives in torch tensors.
 """
 import ctypes
 import math
+import warnings
 
 import numpy as np
 import torch
@@ -183,6 +184,11 @@ class MPMSolver:
         if types is None and cfg.use_quantaichi:
             types, groups = reference_layout(cfg)
         if types is not None:
+            if str(cfg.compute_fp) in ("f64", "float64"):
+                # init_quantaichi (mpm_solver.py:372-374) computes in cfg.compute_fp, f64 by default; the
+                # quantized kernels here compute in f32
+                warnings.warn("quantized storage computes in f32; cfg.compute_fp='f64' applies to plain storage only",
+                              stacklevel=3)
             self.storage = _lib.AQ_STORE_QUANT
             if groups is None:
                 groups = [("pack", 0, self.names)]
@@ -251,8 +257,10 @@ class MPMSolver:
         """Back to the generic (descriptor-driven) fused kernel."""
         _lib.check(_lib.lib().aq_sim_specialize(self._h, None, None, None))
 
-    def set_state(self, vals, n=None):
-        """vals: [ncomp, n] float array (canonical component order)."""
+    def set_state(self, vals, n=None, dither_step=0xFFFFFFFF):
+        """vals: [ncomp, n] float array (canonical component order).  With
+        quantized storage the values are encoded with the dithering noise of
+        step `dither_step`."""
         vals = np.asarray(vals, dtype=np.float64)
         n = vals.shape[1] if n is None else n
         if n > self.capacity:
@@ -262,7 +270,8 @@ class MPMSolver:
             v = torch.as_tensor(np.ascontiguousarray(vals[:, :n], dtype=np.float32), device=self.device)
             tmp = torch.zeros((self.nwords, n), dtype=torch.int64, device=self.device)
             _lib.check(_lib.lib().aq_pack(self._carr, self.ncomp, self.nwords, _ptr(v), None,
-                                          self._desc.seed, 0xFFFFFFFF, _ptr(tmp), n, _stream()))
+                                          self._desc.seed, int(dither_step) & 0xFFFFFFFF, _ptr(tmp), n,
+                                          _stream()))
             self.state[:, :n] = tmp
         else:
             self.state[:, :n] = torch.as_tensor(vals[:, :n], dtype=self.state.dtype, device=self.device)
@@ -402,6 +411,22 @@ class MPMSolver:
         out[d] = g[node, 3]
         return out
 
+    # ---- on-disk formats (mpm_solver.py:1761-1895) -------------------------
+    def dump_states(self, folder, frame):
+        """data_{frame:06d}.ply: packed point cloud of the positions (3D)."""
+        from .io import dump_states
+        return dump_states(self, folder, frame)
+
+    def checkpoint_states(self, folder):
+        """v.npy, C.npy, J.npy / F.npy of the decoded state (float32)."""
+        from .io import checkpoint_states
+        checkpoint_states(self, folder)
+
+    def load_states(self, folder, frame):
+        """The state from dump_states' PLY and checkpoint_states' npy files."""
+        from .io import load_states
+        load_states(self, folder, frame)
+
     # ---- loss and ranges -------------------------------------------------
     def compute_ek(self):
         out = torch.zeros(1, dtype=torch.float64, device=self.device)
@@ -419,6 +444,21 @@ class MPMSolver:
     def detect_bound_kernel(self, s=0):
         _lib.check(_lib.lib().aq_update_limits(self._h, _ptr(self.state), self.current_idx,
                                                _ptr(self._mxmn), _stream()))
+
+    def count_wraps(self, enable=True):
+        """Range-violation counter (quantized storage): while enabled, every
+        record encode adds, per component, the stores whose fixed-point code
+        wrapped (value outside the member's range).  Read with wrap_counts()."""
+        if enable:
+            self._wraps = torch.zeros(self.ncomp, dtype=torch.int64, device=self.device)
+            _lib.check(_lib.lib().aq_sim_count_wraps(self._h, _ptr(self._wraps)))
+        else:
+            _lib.check(_lib.lib().aq_sim_count_wraps(self._h, None))
+            self._wraps = None
+
+    def wrap_counts(self):
+        """Per-component wrapped stores since count_wraps() (numpy int64)."""
+        return self._wraps.cpu().numpy()
 
     def advance_detect(self, nsteps, sort_every=0):
         """nsteps substeps (as advance) with detect_bound_kernel after each."""

@@ -7,9 +7,11 @@ per-field sensitivity coefficients log10 of the mean error-contribution sum of
 the field's components from cheap statistics of the scene's initial state;
 it is trained by the library's own HIP kernels (aq_mlp_forward,
 aq_mlp_backward, aq_adam_step) on adjoint runs of the method itself.  The
-adjoint remains the judge and the fallback: search_scheme_surrogate can
-check a predicted allocation against the adjoint's sums and fall back to the
-adjoint allocation when the predicted one misses the error bound.
+adjoint remains the judge and the fallback: search_scheme_surrogate checks
+a predicted allocation with a dithered quantized run of the scene and falls
+back to the adjoint search when that run misses the error bound, and, given
+an adjoint search's result, reports whether the allocation meets the bound
+under the adjoint's sums.
 
 Scenes are the 2D elastic jelly of the adjoint configuration: fields x, v, C,
 F (12 components).
@@ -220,29 +222,60 @@ def make_dataset(nscenes, seed, n, n_grid, nsteps, checkpoint_every=16, log=None
     return np.array(X), np.array(G), np.array(R), np.array(P)
 
 
-def search_scheme_surrogate(model, st, E, dt, n_grid, nsteps, ref, tol_rel, judge=None):
+def search_scheme_surrogate(model, st, E, dt, n_grid, nsteps, tol_rel, ref=None, judge=None, check_runs=0,
+                            fallback=None, sort_every=16):
     """Bit allocation from the surrogate: features + MLP + range run +
-    LagrangianEB.  With judge (the result of search.search_scheme on the same
-    scene: its per-component sums, ranges and allocation) the allocation is
-    checked against the bound under the adjoint's sums and replaced by the
-    adjoint's allocation when it misses.  Returns dict(bits (sign included),
-    ranges, seconds, met, fell_back, pred)."""
-    from .search import detect_ranges, elastic_2d_ranges
+    LagrangianEB on the loss bound tol_rel * ref (ref: default the range
+    run's final loss).
+
+    check_runs > 0: the allocation is then checked by that many dithered
+    quantized runs of the scene (search.validate_scheme's run), passing when
+    every run's final loss is within 3 tol_rel ref of the range run's (the
+    reference's pass criterion, run_mpm_elastic_2d.py:205-207, with the
+    range run standing for the f64 run); when it
+    fails and `fallback` (a callable returning an adjoint search's result)
+    is given, its allocation replaces the surrogate's and its time counts.
+    judge (optional): an adjoint search's result on the same scene; reports
+    whether the allocation meets the bound under the adjoint's sums
+    (modelled error).
+
+    Returns dict(bits (sign included), ranges, ref, seconds (prediction,
+    range run and solve), check_seconds, check_passed, check_losses,
+    fallback_seconds, fell_back, met (under the judge's sums), pred)."""
+    from .search import detect_ranges, elastic_2d_ranges, quantized_losses, quantized_solver
     t0 = time.perf_counter()
     n = st.shape[1]
     pred = model.predict(scene_features(st, n_grid, dt, E))[0] + model.margin
     sim = make_solver(n, n_grid, E, dt)
     sim.set_state(st)
     ranges = elastic_2d_ranges(detect_ranges(sim, nsteps), 2)
+    L_range = sim.compute_ek()
+    if ref is None:
+        ref = L_range
     bits, _ = LagrangianEB(component_sums(pred) * ranges ** 2, ref=ref, tol_rel=tol_rel)
     torch.cuda.synchronize()
     seconds = time.perf_counter() - t0
     bits = bits + 1
-    met, fell_back = None, False
+    out = dict(bits=bits, ranges=ranges, ref=ref, seconds=seconds, check_seconds=0.0, check_passed=None,
+               check_losses=None, fallback_seconds=0.0, fell_back=False, met=None, pred=pred)
+    if check_runs > 0:
+        t1 = time.perf_counter()
+        q = quantized_solver(sim, bits, ranges, seed=12345)
+        torch.cuda.synchronize()
+        out["check_create_seconds"] = time.perf_counter() - t1
+        L = quantized_losses(q, st, nsteps, check_runs, sort_every=sort_every, first_run=1000)
+        torch.cuda.synchronize()
+        out["check_seconds"] = time.perf_counter() - t1
+        out["check_losses"] = L
+        out["check_passed"] = bool(np.all(np.abs(L - L_range) < 3.0 * tol_rel * abs(ref)))
+        if not out["check_passed"] and fallback is not None:
+            t2 = time.perf_counter()
+            res = fallback()
+            out["fallback_seconds"] = time.perf_counter() - t2
+            out["bits"] = np.asarray(res["bits"])
+            out["ranges"] = np.asarray(res["ranges"])
+            out["fell_back"] = True
     if judge is not None:
         G = np.asarray(judge["grads"]) * np.asarray(judge["ranges"]) ** 2
-        met = modelled_error(G, bits - 1) <= (ref * tol_rel) ** 2
-        if not met:
-            bits = np.asarray(judge["bits"])
-            fell_back = True
-    return dict(bits=bits, ranges=ranges, seconds=seconds, met=met, fell_back=fell_back, pred=pred)
+        out["met"] = bool(modelled_error(G, bits - 1) <= (ref * tol_rel) ** 2)
+    return out

@@ -11,10 +11,12 @@ layout at run time (--generic-kernel for the descriptor-driven one); the
 same scene with fp32 storage is timed beside it.  Printed: one JSON line,
 with the roofline entry of the fused kernel, the error of both runs against
 an FP64 run, the CPU baseline (the C oracle on the host's cores), the scheme
-search on the adjoint scene (adjoint path and learned surrogate; headline
-`scheme_search_s` is the surrogate's, plus the adjoint's when the adjoint
-rejects its allocation), the adjoint check against finite differences, and
-the batched scheme search.
+search on the adjoint scene (adjoint path and learned surrogate, each
+allocation validated by dithered quantized runs; headline `scheme_search_s`
+is the surrogate's checked by one quantized run, plus the adjoint search's
+time when that run misses the bound), the adjoint check against finite
+differences, and the batched scheme search.  Timing is steady state: the
+warm-up substeps run first, and the timed substeps continue from there.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -32,11 +34,27 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-substeps/s + scheme-search time, quantized vs fp32 MPM, 1 MI355X"
 HBM_PEAK = 8.0e12
-# HBM bytes per launch of the specialised fused kernel on this workload, from
-# rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a plain copy's readings
-# (profiles/k_tiled_fused_fluid3d_1M.md); a profiler run, not this script,
-# measures it.
-FUSED_TRAFFIC_BYTES = 102.8e6
+# HBM bytes per launch of the kernels on this workload are measured by
+# rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a plain copy's
+# readings) and summarised in this file, whose `bench-traffic:` line the
+# benchmark reads; a profiler run, not this script, measures them.
+TRAFFIC_PROFILE = os.path.join("profiles", "k_tiled_fused_fluid3d_1M.md")
+
+
+def profiled_traffic():
+    """{kernel: HBM bytes per launch} from the profile summary's
+    `bench-traffic: name=bytes ...` line (empty if absent)."""
+    out = {}
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_PROFILE)) as f:
+            for line in f:
+                if line.startswith("bench-traffic:"):
+                    for tok in line.split(":", 1)[1].split():
+                        k, v = tok.split("=")
+                        out[k] = float(v)
+    except OSError:
+        pass
+    return out
 
 
 def make_solver(storage, n, n_grid, seed=0):
@@ -48,6 +66,7 @@ def make_solver(storage, n, n_grid, seed=0):
     cfg.n_particles = n
     cfg.seed = 1234
     if storage == "quant":
+        cfg.compute_fp = "f32"
         names, t, g, sh = layouts.fluid_3d()
         return aq.MPMSolver(cfg, types=t, groups=g, shared=sh)
     cfg.compute_fp = storage
@@ -98,9 +117,19 @@ def field_errors(a, b):
     return out
 
 
+def host_threads():
+    """Host threads for the CPU baseline: OMP_NUM_THREADS when set (the GPU
+    box's CPU share), else the cores this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
 def cpu_baseline(n, n_grid, target_s=10.0):
     """The C oracle (the port of the reference substep) on the host's cores, on
-    the same quantized workload, for a bounded number of substeps."""
+    the same quantized workload, for a bounded number of substeps: with all
+    the threads of the host's share and with one thread."""
     from oracle import orc
     from autoquantizer_amd import layouts
     from autoquantizer_amd.mpm import plan_state
@@ -110,18 +139,28 @@ def cpu_baseline(n, n_grid, target_s=10.0):
                exp_bits=c["exp_bits"], exp_offset=c["exp_offset"], shared=c["shared"],
                dither=c["use_dithering"], scale=c["scale"], offset=c["offset"]) for c in comps]
     s = orc.mpm_params(3, orc.FLUID, n_grid, 2e-4, la=400.0)
-    threads = min(os.cpu_count() or 1, 16)
     words = orc.pack(oc, nw, initial_state(n).astype(np.float32), seed=1234, step=0)
-    t0 = time.perf_counter()
-    orc.run_quant(s, oc, nw, words, 1, seed=1234, step0=0, nthreads=threads)
-    t1 = time.perf_counter() - t0
-    k = int(max(1, min(20, target_s / max(t1, 1e-3))))
-    t0 = time.perf_counter()
-    orc.run_quant(s, oc, nw, words, k, seed=1234, step0=1, nthreads=threads)
-    tk = time.perf_counter() - t0
-    return {"value": n * k / tk, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
-            "sample": f"{n} particles, {k} quantized substeps after 1 untimed (128^3 grid), C oracle "
-                      f"with OpenMP, {tk:.1f} s"}
+
+    def rate(threads, budget):
+        t0 = time.perf_counter()
+        orc.run_quant(s, oc, nw, words, 1, seed=1234, step0=0, nthreads=threads)
+        t1 = time.perf_counter() - t0
+        k = int(max(1, min(20, budget / max(t1, 1e-3))))
+        t0 = time.perf_counter()
+        orc.run_quant(s, oc, nw, words, k, seed=1234, step0=1, nthreads=threads)
+        tk = time.perf_counter() - t0
+        return n * k / tk, k, tk
+
+    threads = host_threads()
+    v_all, k_all, t_all = rate(threads, target_s)
+    v_one, k_one, t_one = rate(1, target_s)
+    return {"value": v_all, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} particles, {k_all} quantized substeps after 1 untimed (128^3 grid), C oracle "
+                      f"with OpenMP on {threads} threads, {t_all:.1f} s",
+            "single_thread": {"value": v_one, "cores": 1,
+                              "sample": f"{k_one} substeps after 1 untimed, {t_one:.1f} s"},
+            "nproc": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0)),
+            "build": "gcc -O3 -ffp-contract=off -fopenmp (oracle/Makefile)"}
 
 
 def _adjoint_scene(n, n_grid, dt, seed=0):
@@ -146,59 +185,90 @@ def _adjoint_scene(n, n_grid, dt, seed=0):
     return make, init
 
 
-def scheme_search(n=65536, n_grid=256, nsteps=256, checkpoint_every=16, dt=2.5e-5, tol_rel=1e-3):
-    """Wall time of one scheme search on the adjoint scene (range run +
-    checkpointed adjoint run in fp32 with f64 sums + FP64 error-bound solve
-    at tol_rel of |x| = 1), and of the surrogate search on the same scene
-    when trained weights are present (features + MLP + range run + solve),
-    judged by the adjoint's sums.  dt = 2.5e-5: with E = 1e4 the explicit
-    update diverges at dt = 1e-4 and 5e-5 on this grid (wave speed 100 per
-    unit time, CFL number dt * 100 * n_grid)."""
+def scheme_search(n=65536, n_grid=256, nsteps=256, checkpoint_every=16, dt=2.5e-5, eps=1e-4, runs=20,
+                  check_runs=1):
+    """The scheme search on the adjoint scene, and the reference's experiment
+    on its result (run_mpm_elastic_2d.py:186-208, 227-242).
+
+    Adjoint search: range run + checkpointed fp32 adjoint (f64 sums) of the
+    energy loss (compute_ek, sum |v|^2) + FP64 LagrangianEB for the error
+    bound eps * loss.  Surrogate search: features + MLP + range run + solve,
+    then `check_runs` dithered quantized run(s) of the scene with its
+    allocation (the search falls back to the adjoint search when a run
+    misses the bound).  Each allocation is then clamped to [1, 32] bits
+    (set_bits_and_ranges), instantiated as the reference's quantized solver
+    (fixed-point components in one bit_pack, dithered) and run `runs` times;
+    the success fraction is the reference's |loss - f64| < 3 eps f64 against
+    an FP64 run of the scene.  dt = 2.5e-5: with E = 1e4 the explicit update
+    diverges at dt = 1e-4 and 5e-5 on this grid (wave speed 100 per unit
+    time, CFL number dt * 100 * n_grid)."""
     import torch
-    from autoquantizer_amd.search import search_scheme
+    from autoquantizer_amd.search import reference_run, search_scheme, validate_scheme
 
     make, init = _adjoint_scene(n, n_grid, dt)
-    search_scheme(make, init, 8, tol_rel=tol_rel, ref=1.0, checkpoint_every=4)  # warm-up (kernel loading)
+    plain, _, L64 = reference_run(make, init, nsteps)
+    search_scheme(make, init, 8, tol_rel=eps, checkpoint_every=4)  # warm-up (kernel loading)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = search_scheme(make, init, nsteps, tol_rel=tol_rel, ref=1.0, checkpoint_every=checkpoint_every)
+    res = search_scheme(make, init, nsteps, tol_rel=eps, checkpoint_every=checkpoint_every)
     torch.cuda.synchronize()
-    total = time.perf_counter() - t0
-    out = {"seconds": total, "phases": res["seconds"], "bits": [int(b) for b in res["bits"]],
-           "grads": [float(g) for g in res["grads"]],
+    t_adj = time.perf_counter() - t0
+    val = validate_scheme(plain, init, nsteps, res["bits"], res["ranges"], L64, eps, runs=runs)
+    out = {"adjoint": {"seconds": t_adj, "phases": res["seconds"], "bits": [int(b) for b in res["bits"]],
+                       "bits_clamped": [int(b) for b in val["bits"]], "record_bits": val["record_bits"],
+                       "grads": [float(g) for g in res["grads"]], "success": val["success"],
+                       "rel_error_median": float(np.median(val["rel_error"])),
+                       "rel_error_max": float(np.max(val["rel_error"]))},
+           "f64_loss": L64, "eps": eps, "runs": runs,
            "scene": f"2D elastic, {n} particles, {n_grid}^2, dt {dt}, {nsteps} substeps, checkpoints every "
-                    f"{checkpoint_every}, error bound {tol_rel} of |x|"}
+                    f"{checkpoint_every}; error bound eps * loss on the energy loss sum |v|^2; success: "
+                    f"|loss - f64| < 3 eps f64 over {runs} dithered quantized runs (f32 compute)"}
     wpath = os.path.join(ROOT, "autoquantizer_amd", "data", "surrogate_elastic2d.npz")
     if os.path.exists(wpath):
         from autoquantizer_amd.surrogate import Surrogate, search_scheme_surrogate
         model = Surrogate.load(wpath)
-        search_scheme_surrogate(model, init, 1e4, dt, n_grid, 8, ref=1.0, tol_rel=tol_rel)  # warm-up
-        r = search_scheme_surrogate(model, init, 1e4, dt, n_grid, nsteps, ref=1.0, tol_rel=tol_rel,
-                                    judge=res)
-        sur_bits = r["bits"] if not r["fell_back"] else None
-        out["surrogate"] = {"seconds": r["seconds"], "speedup": total / r["seconds"],
-                            "meets_bound_under_adjoint_sums": bool(r["met"]),
-                            "bits": [int(b) for b in (sur_bits if sur_bits is not None else r["bits"])],
-                            "extra_bits_per_particle": int(np.maximum(np.asarray(r["bits"]) - 1, 0).sum()
-                                                           - np.maximum(np.asarray(res["bits"]) - 1, 0).sum())}
+
+        def fallback():
+            return search_scheme(make, init, nsteps, tol_rel=eps, checkpoint_every=checkpoint_every)
+
+        search_scheme_surrogate(model, init, 1e4, dt, n_grid, 8, tol_rel=eps, check_runs=check_runs)  # warm-up
+        torch.cuda.synchronize()
+        r = search_scheme_surrogate(model, init, 1e4, dt, n_grid, nsteps, tol_rel=eps, judge=res,
+                                    check_runs=check_runs, fallback=fallback)
+        sval = validate_scheme(plain, init, nsteps, r["bits"], r["ranges"], L64, eps, runs=runs)
+        checked = r["seconds"] + r["check_seconds"] + r["fallback_seconds"]
+        out["surrogate"] = {
+            "seconds_unchecked": r["seconds"], "check_seconds": r["check_seconds"],
+            "check_create_seconds": r.get("check_create_seconds"),
+            "check_passed": r["check_passed"], "fell_back": r["fell_back"],
+            "fallback_seconds": r["fallback_seconds"], "seconds": checked,
+            "seconds_with_adjoint_judge": r["seconds"] + t_adj,
+            "speedup": t_adj / checked, "speedup_unchecked": t_adj / r["seconds"],
+            "meets_bound_under_adjoint_sums": r["met"],
+            "bits": [int(b) for b in r["bits"]], "bits_clamped": [int(b) for b in sval["bits"]],
+            "record_bits": sval["record_bits"], "success": sval["success"],
+            "rel_error_median": float(np.median(sval["rel_error"])), "rel_error_max": float(np.max(sval["rel_error"]))}
     return out
 
 
 def adjoint_check(n=65536, n_grid=256, nsteps=256, checkpoint_every=16, dt=2.5e-5):
     """The adjoint error-gradient run on the adjoint scene: noise per field
-    uniform of width 2^-bits times the field's range (x, v 16 bits; C, F 13
-    bits), d |x_T - x_T^FP64|^2 / d(noise scale of the field) from the fp32
-    adjoint (f64 sums), against central differences (h = 1e-3) of the f64
-    forward.  Each field is checked in a run with that field's noise alone:
-    with all fields noisy, x_T is moved ~1e7x more by the position noise than
-    by the C noise, and a difference quotient of the C scale would measure
-    rounding, not the derivative.  The f64 adjoint is reported beside the
-    fp32 one: the fp32 run's own rounding moves x_T by about as much as the
-    v and C noise do, so its loss (and the fp32 adjoint, exact for that
-    loss) differs from the f64 loss the differences are taken of there."""
+    uniform of width 2^-bits times the field's range (x, v 16 bits, C 4 bits,
+    F 13 bits), L = |x_T - x_T^FP64|^2, and dL/d(noise scale of the field)
+    from the fp32 adjoint kernels (f64 sums) run around the f64 forward,
+    against central differences (h = 1e-3) of that same f64 forward (sorted,
+    LDS-tiled segments, noise keyed by particle id).  Each field is checked
+    in a run with that field's noise alone: with all fields noisy, x_T is
+    moved far more by the position noise than by the C noise, and a
+    difference quotient of the C scale would measure rounding, not the
+    derivative.  C gets 4 bits because its noise at more bits moves x_T by
+    less than 1e-12, where f64 rounding of the forward limits the
+    differences.  The f64 adjoint is reported beside it, and the fp32
+    adjoint of the fp32 forward, whose own rounding moves x_T more than the
+    v and C noise do (so its loss is a different function there)."""
     import torch
-    from autoquantizer_amd.adjoint import (Noise, _Schedule, _advance_plain, field_gradients, final_position_error,
-                                           noise_scale_gradients)
+    from autoquantizer_amd.adjoint import _Schedule, Noise, _advance_plain, field_gradients, final_position_error, \
+        noise_scale_gradients
     from autoquantizer_amd.search import detect_ranges, elastic_2d_ranges
 
     make, init = _adjoint_scene(n, n_grid, dt)
@@ -211,16 +281,17 @@ def adjoint_check(n=65536, n_grid=256, nsteps=256, checkpoint_every=16, dt=2.5e-
     _advance_plain(s64, x, nsteps)
     xref = x[:2, :n].clone()
     xr = xref.cpu().numpy()
-    bits = np.array([16] * 4 + [13] * 8, dtype=np.float64)
+    bits = np.array([16] * 4 + [4] * 4 + [13] * 4, dtype=np.float64)
     sigma = ranges * np.exp2(-bits)
     fields = [("x", slice(0, 2)), ("v", slice(2, 4)), ("C", slice(4, 8)), ("F", slice(8, 12))]
     s32 = make("f32")
     s32.set_state(init)
-    noise_scale_gradients(s32, 8, sigma, xr, seed=1, checkpoint_every=4)  # warm-up
-    s32.set_state(init)
+    noise_scale_gradients(s64, 8, sigma, xr, seed=1, checkpoint_every=4, adjoint_solver=s32)  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    L, dsig = noise_scale_gradients(s32, nsteps, sigma, xr, seed=1, checkpoint_every=checkpoint_every)
+    L, dsig = noise_scale_gradients(s64, nsteps, sigma, xr, seed=1, checkpoint_every=checkpoint_every,
+                                    adjoint_solver=s32)
+    torch.cuda.synchronize()
     t_adj = time.perf_counter() - t0
     all_fields = field_gradients(dsig, sigma, fields)
 
@@ -233,21 +304,24 @@ def adjoint_check(n=65536, n_grid=256, nsteps=256, checkpoint_every=16, dt=2.5e-
     for name, sl in fields:
         sf = np.zeros(12)
         sf[sl] = sigma[sl]
-        s32.set_state(init)
-        Lf, df = noise_scale_gradients(s32, nsteps, sf, xr, seed=1, checkpoint_every=checkpoint_every)
-        ad = float(sf @ df)
-        s64.set_state(init)
+        L32, d32 = noise_scale_gradients(s64, nsteps, sf, xr, seed=1, checkpoint_every=checkpoint_every,
+                                         adjoint_solver=s32)
         L64, d64 = noise_scale_gradients(s64, nsteps, sf, xr, seed=1, checkpoint_every=checkpoint_every)
-        ad64 = float(sf @ d64)
+        Lp, dp = noise_scale_gradients(s32, nsteps, sf, xr, seed=1, checkpoint_every=checkpoint_every)
+        ad, ad64, adp = float(sf @ d32), float(sf @ d64), float(sf @ dp)
         fd = (loss(sf * (1 + h)) - loss(sf * (1 - h))) / (2 * h)
-        per_field[name] = {"loss_fp32": Lf, "loss_f64": L64, "adjoint_fp32": ad, "adjoint_f64": ad64,
-                           "finite_difference": fd,
+        per_field[name] = {"loss_f64": L64, "adjoint_fp32": ad, "adjoint_f64": ad64, "finite_difference": fd,
                            "relative_difference_fp32": abs(ad - fd) / max(abs(fd), 1e-300),
-                           "relative_difference_f64": abs(ad64 - fd) / max(abs(fd), 1e-300)}
-    return {"seconds": t_adj, "loss": L, "field_gradients_all_noise": all_fields, "per_field": per_field,
+                           "relative_difference_f64": abs(ad64 - fd) / max(abs(fd), 1e-300),
+                           "fp32_forward": {"loss": Lp, "adjoint": adp,
+                                            "relative_difference": abs(adp - fd) / max(abs(fd), 1e-300)}}
+    return {"seconds": t_adj, "loss": L, "bits": [int(b) for b in bits],
+            "field_gradients_all_noise": all_fields, "per_field": per_field,
+            "max_relative_difference_fp32": max(v["relative_difference_fp32"] for v in per_field.values()),
             "max_relative_difference_f64": max(v["relative_difference_f64"] for v in per_field.values()),
             "scene": f"2D elastic, {n} particles, {n_grid}^2, dt {dt}, {nsteps} substeps, checkpoints every "
-                     f"{checkpoint_every}, fp32 adjoint, f64 forward for the differences"}
+                     f"{checkpoint_every}; fp32 adjoint kernels (f64 sums) around the f64 forward, central "
+                     f"differences of the f64 forward"}
 
 
 def batched_search(nsims=32, nsteps=300, seed=0):
@@ -322,19 +396,24 @@ def main():
             t0 = time.perf_counter()
             sim.specialize()  # the fused kernel compiled for this record layout (cached)
             jit_s = time.perf_counter() - t0
+        # cold: K substeps from the unsorted initial state (the window opens with the cold sort of
+        # uniformly random particles)
         sim.set_state(init)
+        sim.step_count = 0
+        el_cold = timed_run(sim, K, args.sort_every)
+        # steady state (the headline): W warm-up substeps from the initial state, then K timed
+        # substeps continuing from there; every advance call sorts first, so the window opens with a
+        # sort of the nearly sorted state and sorts again every sort_every substeps
+        sim.set_state(init)
+        sim.step_count = 0
         if W:
             sim.advance(W, sort_every=args.sort_every)
-        sim.set_state(init)
-        sim.step_count = 0
         el = timed_run(sim, K, args.sort_every)
         final = sim.get_state()
-        # event-timed repeat from the same initial state, for the per-kernel roofline
-        sim.set_state(init)
-        sim.step_count = 0
+        # event-timed repeat continuing from there, for the per-kernel roofline
         ms, launches, act = kernel_times(sim, K, args.sort_every)
-        results[storage] = dict(elapsed=el, final=final, ms=ms, launches=launches, act=act,
-                                nwords=sim.nwords, g2p_words=None)
+        results[storage] = dict(elapsed=el, elapsed_cold=el_cold, final=final, ms=ms, launches=launches,
+                                act=act, nwords=sim.nwords, g2p_words=None)
         if storage == "quant":
             mask = 0
             for c in list(range(3)) + [15]:
@@ -349,15 +428,22 @@ def main():
     value = n * K / q["elapsed"]
     fp32_value = n * K / f["elapsed"]
 
-    # roofline of the dominant kernel: the fused G2P->P2G launch (HBM traffic
-    # per launch is measured separately with rocprofv3 FETCH_SIZE/WRITE_SIZE)
+    # roofline of the dominant kernel, the fused G2P->P2G launch: algorithmic bytes per substep
+    # B = N_p 2 S_p + N_act 32 B (each particle's record read once and written once, each active
+    # grid node's (m v, m) written once and velocity read once); HBM traffic per launch is measured
+    # separately with rocprofv3 FETCH_SIZE/WRITE_SIZE
     avg_s = q["ms"][0] / 1e3 / max(q["launches"][0], 1)
-    per_particle = q["g2p_words"] * 8 + q["nwords"] * 8  # words G2P reads + whole record written
-    per_node = 32  # grid velocity read once (16 B) + accumulated momentum/mass written once (16 B)
+    per_particle = 2 * q["nwords"] * 8
+    per_node = 32
     alg_bytes = n * per_particle + q["act"] * per_node
     achieved = alg_bytes / avg_s
+    # what the fused kernel itself must move: the record words G2P reads (x and J) + the record written
+    g2p_per_particle = q["g2p_words"] * 8 + q["nwords"] * 8
     f_avg = f["ms"][0] / 1e3 / max(f["launches"][0], 1)
-    f_bytes = n * (4 * 4 + 16 * 4) + f["act"] * per_node  # x, J read; 16 floats written
+    f_bytes = n * 2 * 16 * 4 + f["act"] * per_node
+    traffic = profiled_traffic()
+    default_workload = not args.generic_kernel and n == 1 << 20 and args.grid == 128
+    sorts_in_window = 1 + (K - 1) // args.sort_every if args.sort_every > 0 and K > 0 else 0
 
     out = {
         "metric": METRIC,
@@ -378,35 +464,38 @@ def main():
                    "fused_kernel": "generic" if args.generic_kernel else "specialised to the record layout",
                    "specialise_seconds": jit_s,
                    "bytes_per_particle": q["nwords"] * 8, "fp32_bytes_per_particle": 64},
+        "timing": "steady state: after the warm-up substeps, K substeps continuing from there; the window "
+                  f"holds {sorts_in_window} sort(s) of the nearly sorted state",
+        "cold": {"value": n * K / q["elapsed_cold"], "fp32_value": n * K / f["elapsed_cold"],
+                 "note": "K substeps from the unsorted initial state (the window opens with the cold sort)"},
         "fp32": {"value": fp32_value, "ms_per_step": f["elapsed"] * 1e3 / K,
-                 "fused_kernel_us": f_avg * 1e6, "fused_kernel_GBps": f_bytes / f_avg / 1e9},
+                 "fused_kernel_us": f_avg * 1e6, "fused_kernel_GBps": f_bytes / f_avg / 1e9,
+                 "traffic": traffic.get("k_tiled_f32") if default_workload else None},
         "speedup_vs_fp32": value / fp32_value,
         "kernel_ms": {"fused_g2p2g": q["ms"][0], "grid_update": q["ms"][1], "sort": q["ms"][2],
                       "first_p2g_last_g2p": q["ms"][3], "launches": q["launches"]},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK,
-                     "traffic": (FUSED_TRAFFIC_BYTES if not args.generic_kernel and n == 1 << 20 and args.grid == 128
-                                 else None),
-                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE (profiles/k_tiled_fused_fluid3d_1M.md)",
+                     "traffic": traffic.get("aq_jit_tiled") if default_workload else None,
+                     "traffic_source": f"rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE ({TRAFFIC_PROFILE})",
+                     "gather_traffic": traffic.get("k_gather") if default_workload else None,
                      "kernel": ("k_tiled<3,fluid,quant,8,float,G2P+P2G>" if args.generic_kernel else
                                 "aq_jit_tiled (k_tiled<3,fluid,quant,8,float,G2P+P2G> specialised to the record)")
                      + " (fused substep kernel)", "kernel_us": avg_s * 1e6,
                      "algorithmic_bytes": alg_bytes, "active_nodes": q["act"],
-                     "bytes_per_particle": per_particle, "bytes_per_node": per_node},
+                     "bytes_per_particle": per_particle, "bytes_per_node": per_node,
+                     "fused_kernel_bytes_per_particle": g2p_per_particle,
+                     "fused_kernel_frac": (n * g2p_per_particle + q["act"] * per_node) / avg_s / HBM_PEAK},
         "scheme_search_s": None,
     }
     if not args.no_search:
         try:
             ss = scheme_search()
-            # the search as shipped: the surrogate's allocation, with the adjoint search's time added
-            # when the adjoint (run here as the judge) rejects it
+            # the search as shipped: the surrogate's allocation checked by one dithered quantized run, with
+            # the adjoint search's time added when that run misses the bound
             sur = ss.get("surrogate")
-            if sur is None:
-                out["scheme_search_s"] = ss["seconds"]
-            else:
-                out["scheme_search_s"] = sur["seconds"] + (0.0 if sur["meets_bound_under_adjoint_sums"]
-                                                           else ss["seconds"])
-            out["scheme_search_adjoint_s"] = ss["seconds"]
+            out["scheme_search_s"] = sur["seconds"] if sur is not None else ss["adjoint"]["seconds"]
+            out["scheme_search_adjoint_s"] = ss["adjoint"]["seconds"]
             out["scheme_search"] = ss
         except Exception as e:  # reported, not hidden: the line still carries the throughput
             out["scheme_search"] = {"error": f"{type(e).__name__}: {e}"}
@@ -420,12 +509,19 @@ def main():
         except Exception as e:
             out["batched_search"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_fp64:
+        from autoquantizer_amd.layouts import fluid_3d_error_bounds
         sim = make_solver("f64", n, args.grid)
         sim.set_state(init)
+        if W:
+            sim.advance(W, sort_every=args.sort_every)
         sim.advance(K, sort_every=args.sort_every)
         ref = sim.get_state()
         del sim
-        out["error_vs_fp64"] = {"quant": field_errors(q["final"], ref), "fp32": field_errors(f["final"], ref)}
+        eq = field_errors(q["final"], ref)
+        bounds = fluid_3d_error_bounds(W + K, args.grid)
+        out["error_vs_fp64"] = {"quant": eq, "fp32": field_errors(f["final"], ref),
+                                "quant_rms_bound": bounds,
+                                "quant_within_bound": all(eq[k]["rms"] <= bounds[k] for k in bounds)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, args.grid)
     print(json.dumps(out), flush=True)

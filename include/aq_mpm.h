@@ -236,6 +236,15 @@ int aq_advance_limits(aq_sim *sim, void *state, int64_t n, int nsteps, uint32_t 
 /* compute_ek (mpm_solver.py:1043-1052) reduced in f64 into *out (device). */
 int aq_energy(aq_sim *sim, const void *state, int64_t n, double *out, void *stream);
 
+/* Range-violation counter (no counterpart in the reference, whose
+   out-of-range fixed-point values wrap silently, codegen_llvm_quant.cpp:
+   44-80): from now on every record encode of `sim` (G2P, fused G2P->P2G)
+   adds to counts[c] (device, ncomp entries, zeroed by the caller) the
+   stores of component c whose fixed-point or int code wrapped (the rounded
+   value outside the member's range, or NaN).  counts NULL stops counting.
+   Stores still wrap (bit-identical records either way). */
+int aq_sim_count_wraps(aq_sim *sim, unsigned long long *counts);
+
 /* Device pointer of the grid node buffer (float4 or double4 per node,
    blocked layout: (m v, m) after aq_p2g, (v, m) after the grid update;
    zero outside the blocks the last P2G touched) and its node count.  The
@@ -262,15 +271,19 @@ int aq_adj_substep(aq_sim *sim, const void *state, int64_t n, const void *adj_ne
    sigma[c] (the model behind the reference's error sums,
    mpm_solver.py:1232-1271 and solvers/analytical_solver.py:3-84, applied
    here as explicit noise so that d loss / d sigma can be taken and checked
-   by finite differences).  xi(step, c, p) is uniform in [-1/2, 1/2), the dithering noise of
-   (seed, step, component c, particle slot p) minus 1/2.
+   by finite differences).  xi(step, c, p) is uniform in [-1/2, 1/2), the
+   dithering noise of (seed, step, component c, particle id) minus 1/2; the
+   id is ids[slot] (device int32, the ids aq_sort permutes with the state)
+   or the slot when ids is NULL, so a sorted state draws the same noise per
+   particle as the unsorted one.
    aq_noise_add: state[c][p] += sigma[c] xi (sigma: host double[ncomp],
    finite, >= 0).  aq_noise_dot: out[c] += sum_p adj[c][p] xi (out: device
    double[ncomp]), the derivative of a loss w.r.t. sigma[c] for the noise
    added at `step` when adj = dL/d(state after the noise).  Plain storage. */
-int aq_noise_add(aq_sim *sim, void *state, int64_t n, const double *sigma, uint64_t seed, uint32_t step,
-                 void *stream);
-int aq_noise_dot(aq_sim *sim, const void *adj, int64_t n, uint64_t seed, uint32_t step, double *out, void *stream);
+int aq_noise_add(aq_sim *sim, void *state, int64_t n, const int32_t *ids, const double *sigma, uint64_t seed,
+                 uint32_t step, void *stream);
+int aq_noise_dot(aq_sim *sim, const void *adj, int64_t n, const int32_t *ids, uint64_t seed, uint32_t step,
+                 double *out, void *stream);
 
 /* Codec self-check (no counterpart in the reference): the fused kernels'
    encoder decodes the shared-exponent digits it writes with a shorter

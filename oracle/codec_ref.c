@@ -254,6 +254,15 @@ float orc_noise_u(uint64_t seed, uint32_t step, uint32_t slot, uint32_t comp) {
   return (float)(h >> 8) * (1.0f / 16777216.0f);
 }
 
+/* Dithering noise: one hash per component pair (2j, 2j+1) with the key of
+   j; component comp takes the low (even comp) or high (odd comp) 16 bits h16
+   of it: u = (h16 + 1/2) 2^-16. */
+float orc_dither_u(uint64_t seed, uint32_t step, uint32_t slot, uint32_t comp) {
+  uint32_t h = orc_hash32(slot ^ orc_noise_key(seed, step, comp >> 1));
+  uint32_t b = (comp & 1u) ? (h >> 16) : (h & 0xFFFFu);
+  return ((float)b + 0.5f) * (1.0f / 65536.0f);
+}
+
 void orc_noise_fill(uint64_t seed, uint32_t step, int nc, int64_t n, double *xi) {
   for (int c = 0; c < nc; c++)
     for (int64_t p = 0; p < n; p++)
@@ -363,7 +372,7 @@ void orc_pack_range_f32(const orc_comp *comps, int nc, int nwords, const float *
       float u = 0.0f;
       if (c->kind == ORC_FIXED && c->dither)
         u = noise ? noise[(int64_t)k * n + p]
-                  : orc_noise_u(seed, step, (uint32_t)p, (uint32_t)k);
+                  : orc_dither_u(seed, step, (uint32_t)p, (uint32_t)k);
       encode_one(c, rec, v[k], u);
     }
     for (int w = 0; w < nwords; w++) words[(int64_t)w * n + p] = rec[w];
@@ -410,7 +419,7 @@ void orc_pack_f64(const orc_comp *comps, int nc, int nwords, const double *vals,
         double u = 0.0;
         if (c->dither)
           u = noise ? noise[(int64_t)k * n + p]
-                    : (double)orc_noise_u(seed, step, (uint32_t)p, (uint32_t)k);
+                    : (double)orc_dither_u(seed, step, (uint32_t)p, (uint32_t)k);
         orc_put_bits(rec, c->bit_offset, c->nbits,
                      orc_fixed_encode_f64(vals[(int64_t)k * n + p], c->scale,
                                           c->offset, c->nbits, c->is_signed,

@@ -23,7 +23,7 @@
  *     spilling into neighbouring bits (TS/taichi/codegen/codegen_llvm_quant.cpp:282-284);
  *   - bit_pack_assign never writes the word past the record when the packed
  *     size is a multiple of 64 (codegen_llvm_quant.cpp:920-950);
- *   - dithering noise comes from an explicit array or from orc_noise_u(), a
+ *   - dithering noise comes from an explicit array or from orc_dither_u(), a
  *     counter-based hash, instead of the per-thread xorshift state
  *     (TS/taichi/runtime/llvm/runtime.cpp:1560-1589).
  *
@@ -80,16 +80,20 @@ void orc_put_bits(uint64_t *rec, int off, int nbits, uint32_t code);
 uint32_t orc_get_bits(const uint64_t *rec, int off, int nbits);
 int32_t orc_sext(uint32_t code, int nbits);
 
-/* ---- dithering noise: counter-based hash ------------------------------ */
+/* ---- noise: counter-based hash ----------------------------------------- */
 uint32_t orc_hash32(uint32_t x);
 uint32_t orc_noise_key(uint64_t seed, uint32_t step, uint32_t comp);
+/* uniform on a 2^-24 grid in [0, 1): hash of slot ^ key(comp) */
 float orc_noise_u(uint64_t seed, uint32_t step, uint32_t slot, uint32_t comp);
+/* dithering noise: (h16 + 1/2) 2^-16 from half (comp & 1) of the hash of
+   slot ^ key(comp >> 1), one hash per component pair */
+float orc_dither_u(uint64_t seed, uint32_t step, uint32_t slot, uint32_t comp);
 /* xi[c][p] = orc_noise_u(seed, step, p, c) - 1/2, the modelled rounding
    noise of the adjoint error-gradient run (uniform in [-1/2, 1/2)). */
 void orc_noise_fill(uint64_t seed, uint32_t step, int nc, int64_t n, double *xi);
 
 /* ---- record level: words are word planes, words[w * n + p] ----------- */
-/* noise: [nc][n] uniform [0,1) values, or NULL to use orc_noise_u(seed, step, p, c). */
+/* noise: [nc][n] uniform [0,1) values, or NULL to use orc_dither_u(seed, step, p, c). */
 void orc_pack_f32(const orc_comp *comps, int nc, int nwords, const float *vals,
                   const float *noise, uint64_t seed, uint32_t step,
                   uint64_t *words, int64_t n);

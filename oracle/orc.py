@@ -81,6 +81,8 @@ def lib():
         L.orc_noise_key.argtypes = [ctypes.c_uint64, u32, u32]
         L.orc_noise_u.restype = f32
         L.orc_noise_u.argtypes = [ctypes.c_uint64, u32, u32, u32]
+        L.orc_dither_u.restype = f32
+        L.orc_dither_u.argtypes = [ctypes.c_uint64, u32, u32, u32]
         L.orc_noise_fill.restype = None
         L.orc_noise_fill.argtypes = [ctypes.c_uint64, u32, ctypes.c_int, ctypes.c_int64, P]
         for name in ("orc_pack_f32", "orc_pack_f64"):
@@ -178,6 +180,11 @@ def shared_decode(e, d, nbits, signed, exp_bits):
 
 def noise_u(seed, step, slot, comp):
     return lib().orc_noise_u(seed, step, slot, comp)
+
+
+def dither_u(seed, step, slot, comp):
+    """The codec's dithering noise of component comp of record `slot`."""
+    return lib().orc_dither_u(seed, step, slot, comp)
 
 
 def noise_xi(seed, step, nc, n):

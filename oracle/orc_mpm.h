@@ -53,7 +53,7 @@ void orc_run_plain_f32(const orc_mpm *s, float *state, int64_t n, int nsteps,
 void orc_run_plain_f64(const orc_mpm *s, double *state, int64_t n, int nsteps,
                        double *grid_m, double *grid_v, int nthreads);
 /* whole substeps on quantized storage (f32 compute): decode, substep, encode.
-   Dithering noise of substep t is orc_noise_u(seed, step0 + t, p, c). */
+   Dithering noise of substep t is orc_dither_u(seed, step0 + t, p, c). */
 void orc_run_quant_f32(const orc_mpm *s, const orc_comp *comps, int nwords,
                        uint64_t *words, int64_t n, int nsteps, uint64_t seed,
                        uint32_t step0, float *grid_m, float *grid_v,

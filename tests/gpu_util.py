@@ -44,6 +44,8 @@ def state_layout(kind):
     from autoquantizer_amd.mpm import plan_state
     if kind == "elastic_2d":
         names, t, g, sh = layouts.elastic_2d()
+    elif kind == "hand_2d":
+        names, t, g, sh = layouts.hand_scheme_2d()
     else:
         names, t, g, sh = layouts.fluid_3d()
     comps, nw = plan_state(names, t, g, sh)

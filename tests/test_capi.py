@@ -2,6 +2,7 @@
 in include/aq_mpm.h, and the host-side layout planner follows the
 reference's placement rules (TS/taichi/struct/struct_llvm.cpp:71-116,
 TS/taichi/program/snode_expr_utils.cpp:44-89).  No GPU is touched."""
+import ctypes
 import os
 import re
 

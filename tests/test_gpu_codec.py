@@ -139,7 +139,7 @@ def random_layout(seed):
 
 def layouts_under_test():
     out = {"elastic_2d": state_layout("elastic_2d"), "fluid_3d": state_layout("fluid_3d"),
-           "fixed_scheme": fixed_scheme_layout()}
+           "fixed_scheme": fixed_scheme_layout(), "hand_2d": state_layout("hand_2d")}
     out.update(kat_layouts())
     for s in range(4):
         try:
@@ -154,7 +154,7 @@ def layouts():
     return layouts_under_test()
 
 
-@pytest.mark.parametrize("name", ["elastic_2d", "fluid_3d", "fixed_scheme", "assign", "split64",
+@pytest.mark.parametrize("name", ["elastic_2d", "fluid_3d", "fixed_scheme", "hand_2d", "assign", "split64",
                                   "shared5", "shared8", "random0", "random1", "random2", "random3"])
 def test_pack_parity(orc, layouts, name):
     if name not in layouts:
@@ -175,7 +175,34 @@ def test_pack_parity(orc, layouts, name):
                           orc.pack(to_oracle(comps), nw, vals, noise=noise))
 
 
-@pytest.mark.parametrize("name", ["elastic_2d", "fluid_3d", "fixed_scheme", "assign", "shared6",
+@pytest.mark.parametrize("name", ["fixed_scheme", "hand_2d", "assign", "random0", "random1", "random2", "random3"])
+def test_pack_saturation_nan_inf(orc, layouts, name):
+    """Values far outside every member's range, at the int32/uint32 limits,
+    infinities and NaN: the saturating float -> int conversion (NaN -> 0)
+    and everything after it agree bit for bit with the oracle."""
+    if name not in layouts:
+        pytest.skip("layout draw exceeded the record size")
+    comps, nw = layouts[name]
+    print(f"[pack special] layout={name}", flush=True)
+    special = np.array([np.inf, -np.inf, np.nan, -np.nan, 1e30, -1e30, 2.0 ** 31, -2.0 ** 31, 2.0 ** 32,
+                        2.0 ** 31 - 128, -2.0 ** 31 + 128, 2.0 ** 32 - 256, 0.0, -0.0, 1e-30, -1e-30,
+                        0.49999997, -0.49999997, 0.5, -0.5, 1.5, -1.5, 7e9, -7e9], dtype=np.float32)
+    rng = np.random.default_rng(17)
+    n = 4096
+    vals = random_values(comps, n, rng)
+    for k, c in enumerate(comps):
+        if c["kind"] in (0, 1):  # fixed and int members take the special values, scaled into code units
+            scale = c["scale"] if c["kind"] == 0 else 1.0
+            col = rng.integers(0, len(special), n)
+            pick = rng.random(n) < 0.5
+            vals[k, pick] = (special[col[pick]] * np.float32(scale) + np.float32(c["offset"])).astype(np.float32)
+            vals[k, :len(special)] = special
+    w_gpu = gpu_pack(comps, nw, vals, seed=5, step=9)
+    w_orc = orc.pack(to_oracle(comps), nw, vals, seed=5, step=9)
+    assert np.array_equal(w_gpu, w_orc)
+
+
+@pytest.mark.parametrize("name", ["elastic_2d", "fluid_3d", "fixed_scheme", "hand_2d", "assign", "shared6",
                                   "random0", "random1", "random2", "random3"])
 def test_unpack_parity_any_bits(orc, layouts, name):
     """Decode of arbitrary words (every code path incl. odd exponents) is bit-exact."""
@@ -221,7 +248,7 @@ def test_store_members_single(orc, layouts):
             oc = to_oracle(comps)
             for p in range(n):
                 rec = np.ascontiguousarray(exp[:, p])
-                u = orc.noise_u(3, 4, p, c)
+                u = orc.dither_u(3, 4, p, c)
                 orc.store_one(oc, rec, c, float(newv[p]), u)
                 exp[:, p] = rec
             if not np.array_equal(got, exp):

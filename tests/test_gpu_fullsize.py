@@ -34,6 +34,7 @@ def solver(storage="quant"):
     cfg.n_particles = N
     cfg.seed = 1234
     if storage == "quant":
+        cfg.compute_fp = "f32"
         _, t, g, sh = layouts.fluid_3d()
         return aq.MPMSolver(cfg, types=t, groups=g, shared=sh)
     cfg.compute_fp = storage
@@ -147,3 +148,35 @@ def test_dithered_encoding_is_unbiased():
     for c, step in ((0, 2.0 ** -20), (15, 0.5 / 2 ** 15)):
         assert abs(dec[c].mean() - vals[c, 0]) < 5 * (step / 2) / np.sqrt(N)
         assert len(np.unique(np.round((dec[c] - dec[c].min()) / step))) == 2
+
+
+def test_error_vs_fp64_within_stated_bound(tmp_path):
+    """The benchmark's own run: its scene from rest (v = 0, C = 0, J = 1)
+    for 20 + 500 substeps, the quantized record (specialised kernel) against
+    an FP64 run of the same scene; the RMS error of every field stays within
+    the scheme's stated bound (layouts.fluid_3d_error_bounds)."""
+    import torch
+    from autoquantizer_amd.layouts import fluid_3d_error_bounds
+    print("[fullsize] error vs fp64", flush=True)
+    rng = np.random.default_rng(0)
+    init = np.zeros((16, N))
+    init[:3] = rng.uniform(0.1, 0.5, (3, N))
+    init[15] = 1.0
+    runs = {}
+    for storage in ("quant", "f64"):
+        s = solver(storage)
+        if storage == "quant":
+            s.specialize(cache_dir=str(tmp_path))
+        s.set_state(init)
+        s.advance(20, sort_every=32)
+        s.advance(500, sort_every=32)
+        runs[storage] = s.get_state()
+        del s
+    bounds = fluid_3d_error_bounds(520, 128)
+    a, b = runs["quant"], runs["f64"]
+    assert np.all(np.isfinite(a))
+    for name, sl in (("x", slice(0, 3)), ("v", slice(3, 6)), ("C", slice(6, 15)), ("J", slice(15, 16))):
+        rms = float(np.sqrt(np.mean((a[sl] - b[sl]) ** 2)))
+        print(f"[fullsize] {name}: rms error {rms:.3e}, bound {bounds[name]:.3e}", flush=True)
+        assert rms <= bounds[name], (name, rms, bounds[name])
+    torch.cuda.synchronize()

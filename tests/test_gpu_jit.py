@@ -34,6 +34,7 @@ def solver(kind, n):
     cfg.p_vol = (1 / cfg.n_grid * 0.5) ** d
     cfg.n_particles = n
     cfg.seed = 3
+    cfg.compute_fp = "f32"
     sim = aq.MPMSolver(cfg, types=t, groups=g, shared=sh)
     vals = np.zeros((sim.ncomp, n))
     lo, hi = (0.08, 0.92) if kind.endswith("_sparse") else (0.2, 0.6)

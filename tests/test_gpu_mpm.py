@@ -40,9 +40,9 @@ def scene(orc, kind, n, seed=0, storage="quant"):
     from autoquantizer_amd import layouts
     cfg = aq.MPMConfig()
     rng = np.random.default_rng(seed)
-    if kind == "elastic_2d":
+    if kind in ("elastic_2d", "hand_2d"):
         cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.E, cfg.nu, cfg.sticky = 2, False, 128, 1e-4, 1e4, 0.2, True
-        names, t, g, sh = layouts.elastic_2d()
+        names, t, g, sh = layouts.elastic_2d() if kind == "elastic_2d" else layouts.hand_scheme_2d()
         s = orc.mpm_params(2, orc.ELASTIC, 128, 1e-4, E=1e4, nu=0.2, boundary=orc.STICKY, gravity=9.8)
         lo, hi = 0.3, 0.5
     else:
@@ -64,7 +64,7 @@ def scene(orc, kind, n, seed=0, storage="quant"):
     vals[:d] = rng.uniform(lo, hi, (d, n))
     vals[d:2 * d] = rng.uniform(-1, 1, (d, n))
     vals[2 * d:2 * d + d * d] = rng.uniform(-0.5, 0.5, (d * d, n))
-    if kind == "elastic_2d":
+    if kind in ("elastic_2d", "hand_2d"):
         vals[-4:] = np.eye(2).reshape(-1, 1) + rng.uniform(-0.01, 0.01, (4, n))
     else:
         vals[-1] = rng.uniform(0.99, 1.01, n)
@@ -78,17 +78,24 @@ def oracle_comps(sim):
 
 def step_sizes(sim, dec):
     """Quantization step of every component at every particle: fixed-point
-    scale, or one digit of the shared-exponent group (the group's largest
-    member's power of two over 2^(digit bits - 1))."""
+    scale, or one digit of the (shared- or own-) exponent group (the group's
+    largest member's power of two over 2^(digit bits - 1)); for unquantized
+    f32 members 2^-20 of the component's largest value.  A float member's
+    step is at least 2^-20 of its group's largest value over all particles:
+    near zero its digit step falls below the f32 rounding of the physics
+    (v is a sum of grid velocities of the field's magnitude)."""
     steps = np.zeros_like(dec)
     for k, c in enumerate(sim.comps):
         if c["kind"] == 0:
             steps[k] = c["scale"]
+        elif c["kind"] == 3:  # unquantized f32: operand order differences, relative to the largest value
+            steps[k] = max(float(np.abs(dec[k]).max()), 1e-30) * 2.0 ** -20
         else:
             grp = [j for j, cc in enumerate(sim.comps) if cc["kind"] == 2 and cc["exp_offset"] == c["exp_offset"]]
             mx = np.abs(dec[grp]).max(axis=0)
             db = c["num_bits"] - c["is_signed"]
             steps[k] = np.exp2(np.floor(np.log2(np.maximum(mx, 1e-30)))) * 2.0 ** (1 - db)
+            steps[k] = np.maximum(steps[k], max(float(mx.max()), 1e-30) * 2.0 ** -20)
     return steps
 
 
@@ -105,6 +112,33 @@ def test_p2g_grid(orc, kind, tiled):
     gm, gv = orc.grids(s)
     orc.p2g(s, st, gm, gv)
     d = s.dim
+    assert g[d] == approx(gm, rel=2e-5, abs=1e-6 * s.p_mass)
+    mom_scale = np.abs(gv).max()
+    assert np.abs(g[:d] - gv).max() <= 2e-5 * mom_scale
+
+
+@pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d"])
+def test_p2g_grid_edge_particles(orc, kind):
+    """Particles at and beyond the domain edge: the stencil base is clamped
+    to [0, n_grid - 3], fx leaves [0.5, 1.5] and a weight turns negative;
+    the sorted (tiled) P2G scatters them through the float-atomic path and
+    agrees with the oracle's P2G."""
+    print(f"[p2g edge] {kind}", flush=True)
+    sim, s = scene(orc, kind, 4000, seed=2, storage="f32")
+    st = sim.get_state(original_order=False)
+    d = s.dim
+    dx = 1.0 / s.n_grid
+    edge = [0.0, 0.05 * dx, 0.3 * dx, 1.0 - 2.6 * dx, 1.0 - 0.5 * dx, 1.0]
+    for j, xe in enumerate(edge * 3):
+        st[j % d, j] = xe
+        st[(j + 1) % d, j] = 0.4
+    sim.set_state(st)
+    sim.build_pid()
+    st = sim.get_state(original_order=False).astype(np.float32)
+    sim.p2g_bls()
+    g = sim.grid_dense()
+    gm, gv = orc.grids(s)
+    orc.p2g(s, st, gm, gv)
     assert g[d] == approx(gm, rel=2e-5, abs=1e-6 * s.p_mass)
     mom_scale = np.abs(gv).max()
     assert np.abs(g[:d] - gv).max() <= 2e-5 * mom_scale
@@ -149,7 +183,7 @@ def test_g2p_on_gpu_grid(orc, kind, tiled):
     assert_within_steps(orc, sim, words_gpu, words_orc, frac_equal=0.97)
 
 
-@pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d"])
+@pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d", "hand_2d"])
 @pytest.mark.parametrize("tiled", [False, True])
 def test_g2p_codec_bit_exact(orc, kind, tiled):
     """The codec fused into G2P is bit-exact: the quantized kernel's words
@@ -189,6 +223,7 @@ def test_g2p_codec_bit_exact_any_field_order(orc):
         cfg.p_vol = (1 / 64 * 0.5) ** 3
         cfg.n_particles = 3000
         cfg.seed = 11
+        cfg.compute_fp = "f32"
         if storage == "quant":
             sims.append(aq.MPMSolver(cfg, types=t, groups=[("pack", 0, order)], shared=sh))
         else:
@@ -233,7 +268,7 @@ def test_grid_op_exact(orc, kind):
     assert np.array_equal(g1[:d][:, touched].astype(np.float32), gv[:, touched])
 
 
-@pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d"])
+@pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d", "hand_2d"])
 def test_substeps_reseeded(orc, kind):
     """Each GPU substep (sort + tiled P2G + grid + tiled G2P) from state S_t
     against one oracle substep from the same words S_t."""
@@ -254,8 +289,9 @@ def test_substeps_reseeded(orc, kind):
         for k, c in enumerate(sim.comps):
             lim = 1.0 if c["kind"] == 0 else 2.0
             assert rel[k].max() <= lim + 1e-9, (t, k, rel[k].max())
-        # nearly all records are identical
-        assert np.mean((wg != wo).any(axis=0)) < 0.05
+        # nearly all records are identical (unquantized f32 members keep the operand-order differences)
+        if all(c["kind"] != 3 for c in sim.comps):
+            assert np.mean((wg != wo).any(axis=0)) < 0.05
         worst = max(worst, float(rel.max()))
     print("max difference in quantization steps:", worst)
 
@@ -384,3 +420,66 @@ def test_range_run_in_one_call(orc, storage):
     la, lb = a.get_limits(), b.get_limits()
     assert np.all(np.isfinite(la))
     np.testing.assert_allclose(la, lb, rtol=1e-3)
+
+
+def test_wrap_counter(orc):
+    """Range-violation counter (aq_sim_count_wraps): a fluid record whose v
+    range (0.25) is below the particles' speeds.  After one G2P the count per
+    component equals the number of stores whose rounded value left the
+    member's range, recomputed from the fp32-storage kernel's G2P output on
+    the same grid with the same dithering noise; the records are still the
+    oracle's encode of that output (counting changes nothing stored)."""
+    _torch()
+    import autoquantizer_amd as aq
+    from autoquantizer_amd import layouts
+    n = 3000
+    names, t, g, sh = layouts.fluid_3d(v_range=0.25)
+    sims = []
+    for storage in ("quant", "f32"):
+        cfg = aq.MPMConfig()
+        cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.la = 3, True, 64, 2e-4, 400.0
+        cfg.p_vol = (1 / 64 * 0.5) ** 3
+        cfg.n_particles = n
+        cfg.seed = 21
+        cfg.compute_fp = "f32"
+        sims.append(aq.MPMSolver(cfg, types=t, groups=g, shared=sh) if storage == "quant" else aq.MPMSolver(cfg))
+    sim_q, sim_f = sims
+    rng = np.random.default_rng(31)
+    vals = np.zeros((16, n))
+    vals[:3] = rng.uniform(0.2, 0.6, (3, n))
+    vals[3:6] = rng.uniform(-0.24, 0.24, (3, n))
+    vals[6:15] = rng.uniform(-30, 30, (9, n))  # a strong affine field: grid speeds well above 0.25
+    vals[15] = rng.uniform(0.99, 1.01, n)
+    sim_q.set_state(vals)
+    sim_f.set_state(sim_q.get_state(original_order=True))
+    sim_q.build_pid()
+    sim_f.build_pid()
+    sim_q.p2g_bls()
+    sim_q.grid_op()
+    sim_f.load_grid(sim_q.grid_velocity())
+    step = sim_q.step_count
+    sim_q.count_wraps()
+    sim_q.g2p_bls()
+    sim_f.g2p_bls()
+    got = sim_q.wrap_counts()
+    out = sim_f.get_state(original_order=True).astype(np.float32)
+    words_orc = orc.pack(oracle_comps(sim_q), sim_q.nwords, out, seed=21, step=step)
+    assert np.array_equal(sim_q.get_words(original_order=True), words_orc)
+    want = np.zeros(16, dtype=np.int64)
+    f32 = np.float32
+    for c, cc in enumerate(sim_q.comps):
+        if cc["kind"] != 0:
+            continue
+        u = np.array([orc.dither_u(21, step, p, c) for p in range(n)], dtype=f32) if cc["use_dithering"] else 0
+        y = (out[c] - f32(cc["offset"])) * f32(1.0 / cc["scale"])
+        if cc["use_dithering"]:
+            y = (y + u) - f32(0.5)
+        y = y + np.copysign(f32(0.5), y)
+        q = np.trunc(y.astype(np.float64))
+        nb = cc["num_bits"]
+        lo, hi = (-2.0 ** (nb - 1), 2.0 ** (nb - 1) - 1) if cc["is_signed"] else (0.0, 2.0 ** nb - 1)
+        want[c] = int(np.sum(np.isnan(y) | (q < lo) | (q > hi)))
+    print("wrapped stores per component:", got.tolist(), flush=True)
+    assert want[3:6].sum() > 0  # the scene does overflow v
+    np.testing.assert_array_equal(got, want)
+    sim_q.count_wraps(False)

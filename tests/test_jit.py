@@ -21,6 +21,7 @@ def fluid_cfg():
     cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.la = 3, True, 128, 2e-4, 400.0
     cfg.p_vol = (1 / 128 * 0.5) ** 3
     cfg.n_particles = 1 << 20
+    cfg.compute_fp = "f32"
     return cfg
 
 
@@ -29,6 +30,7 @@ def elastic_cfg():
     cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.E, cfg.nu, cfg.sticky = 2, False, 128, 1e-4, 1e4, 0.2, True
     cfg.p_vol = (1 / 128 * 0.5) ** 2
     cfg.n_particles = 8192
+    cfg.compute_fp = "f32"
     return cfg
 
 

@@ -399,3 +399,25 @@ def test_dithering_ratio(orc):
     # 10^6 samples: the up-fraction has std 4e-4, the ratio std 0.01
     assert abs(up / n - 0.8) < 0.002
     assert abs(up / down - 4) < 0.1
+
+
+def test_dither_noise_grid_uniformity_and_pairs(orc):
+    """The codec's dithering noise (one hash per component pair, 16 bits per
+    component): every value is (h + 1/2) 2^-16 with h in 0..65535; over many
+    slots each component's values are uniform (mean 1/2, variance 1/12,
+    every 1/16 bin near 1/16) and the two components of a pair are
+    uncorrelated, as are neighbouring slots."""
+    n = 200000
+    u = np.array([[orc.dither_u(99, 7, p, c) for p in range(n)] for c in (4, 5)], dtype=np.float64)
+    h = u * 65536.0 - 0.5
+    assert np.all(h == np.round(h)) and h.min() >= 0 and h.max() <= 65535
+    for row in u:
+        assert abs(row.mean() - 0.5) < 3e-3
+        assert abs(row.var() - 1.0 / 12.0) < 2e-3
+        bins = np.bincount(np.minimum((row * 16).astype(int), 15), minlength=16) / n
+        assert np.all(np.abs(bins - 1 / 16) < 4e-3)
+    assert abs(np.corrcoef(u[0], u[1])[0, 1]) < 1e-2
+    assert abs(np.corrcoef(u[0, :-1], u[0, 1:])[0, 1]) < 1e-2
+    # other pairs and steps draw other values
+    assert orc.dither_u(99, 7, 11, 6) != orc.dither_u(99, 7, 11, 4)
+    assert orc.dither_u(99, 8, 11, 4) != orc.dither_u(99, 7, 11, 4)

@@ -182,3 +182,43 @@ def test_quant_run_matches_plain_then_quantize(orc):
     ref = dec.copy()
     orc.run_plain(s, ref, 1)
     assert np.array_equal(w2, orc.pack(comps, nw, ref))
+
+
+@pytest.mark.parametrize("case", ["rotation", "stretch", "rotated_stretch", "compression"])
+def test_fixed_corotated_stress_closed_form(orc, case):
+    """The oracle's 2D fixed-corotated stress (mpm_solver.py:1012-1031 with
+    polar_decompose2d, TS/python/taichi/lang/linalg_impl.py:9-26), read back
+    through its P2G: one particle at rest (v = 0, C = 0) scatters momentum
+    w_i A (x_i - x_p) with A = -dt p_vol 4 inv_dx^2 sigma, and for quadratic
+    B-splines sum_i w_i (x_i - x_p)(x_i - x_p)^T = dx^2/4 I, so
+    sigma = (4 / dx^2) sum_i mom_i (x_i - x_p)^T / (-dt p_vol 4 inv_dx^2).
+    Closed forms for F = R(theta) diag(s, t): R(theta) is F's rotation, so
+    sigma = 2 mu R diag((s-1) s, (t-1) t) R^T + la J (J - 1) I, J = s t;
+    a pure rotation is stress free."""
+    E, nu, dt, ng = 1e4, 0.2, 1e-4, 64
+    mu, la = E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+    s = orc.mpm_params(2, orc.ELASTIC, ng, dt, gravity=0.0, E=E, nu=nu)
+    theta, sx, ty = {"rotation": (0.7, 1.0, 1.0), "stretch": (0.0, 1.2, 1.0),
+                     "rotated_stretch": (-0.4, 1.15, 0.9), "compression": (1.1, 0.8, 0.85)}[case]
+    c, sn = np.cos(theta), np.sin(theta)
+    R = np.array([[c, -sn], [sn, c]])
+    F = R @ np.diag([sx, ty])
+    st = np.zeros((12, 1), dtype=np.float64)
+    xp = np.array([0.5 + 0.37 / ng, 0.5 + 0.61 / ng])
+    st[:2, 0] = xp
+    st[8:, 0] = F.reshape(-1)
+    gm, gv = orc.grids(s, f64=True)
+    orc.p2g(s, st, gm, gv)
+    dx = 1.0 / ng
+    nodes = np.nonzero(gm)[0]
+    assert len(nodes) == 9
+    A = np.zeros((2, 2))
+    for nd in nodes:
+        d = np.array([nd // ng, nd % ng]) * dx - xp
+        A += np.outer(gv[:, nd], d)
+    A *= 4.0 / dx ** 2
+    sigma = A / (-dt * s.p_vol * 4 / dx ** 2)
+    J = sx * ty
+    want = 2 * mu * R @ np.diag([(sx - 1) * sx, (ty - 1) * ty]) @ R.T + la * J * (J - 1) * np.eye(2)
+    scale = max(np.abs(want).max(), mu)
+    assert np.abs(sigma - want).max() <= 1e-9 * scale, (sigma, want)
