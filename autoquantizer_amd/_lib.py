@@ -88,6 +88,7 @@ class AQError(RuntimeError):
 
 
 AQ_MLP_IN, AQ_MLP_HIDDEN, AQ_MLP_OUT, AQ_MLP_PARAMS = 24, 64, 4, 10180  # include/aq_mpm.h
+AQ_MLP_NOALLOC_MAX_SAMPLES = 2048
 
 EXPORTS = [
     "aq_last_error", "aq_version", "aq_layout_plan", "aq_pack", "aq_unpack",
@@ -95,7 +96,7 @@ EXPORTS = [
     "aq_substep", "aq_p2g", "aq_grid_op", "aq_g2p", "aq_advance", "aq_sort",
     "aq_update_limits", "aq_advance_limits", "aq_energy", "aq_sim_grid", "aq_advance_timed",
     "aq_adj_workspace_bytes", "aq_adj_substep", "aq_noise_add", "aq_noise_dot", "aq_codec_selftest", "aq_mlp_forward",
-    "aq_mlp_backward", "aq_adam_step", "aq_sim_specialize", "aq_jit_build",
+    "aq_mlp_backward", "aq_adam_step", "aq_sim_specialize", "aq_jit_build", "aq_mlp_backward_noalloc",
     "aq_sim_count_wraps",
 ]
 
@@ -147,6 +148,7 @@ def lib():
     f32 = ctypes.c_float
     L.aq_mlp_backward.argtypes = [P, P, P, P, P, i32, f32, P, P, P, P]
     L.aq_sim_count_wraps.argtypes = [P, P]
+    L.aq_mlp_backward_noalloc.argtypes = [P, P, P, P, P, i32, f32, P, P, P]
     L.aq_adam_step.argtypes = [P, P, P, P, i32, i32, f32, f32, f32, f32, P]
     for name in EXPORTS:
         if name not in ("aq_last_error", "aq_version", "aq_sim_workspace_bytes", "aq_adj_workspace_bytes"):

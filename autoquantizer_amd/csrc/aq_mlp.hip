@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "aq_mpm.h"
@@ -169,6 +170,56 @@ extern "C" int aq_mlp_backward(const float *params, const float *x, const float 
                      weight_decay, grad);
   MLP_TRY(hipGetLastError());
   return AQ_OK;
+}
+
+// ---- allocation-free backward -----------------------------------------------
+// The deltas of aq_mlp_backward_noalloc live in this file-scope device array
+// (one copy per GPU, held until the process ends) instead of a caller
+// workspace.  Host code never passes the array's name to a kernel: its
+// device address on each GPU comes from hipGetSymbolAddress, taken once per
+// device with that device current.
+static __device__ __align__(256) float scratch[(int64_t)AQ_MLP_NOALLOC_MAX_SAMPLES * (3 * H + OUT)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call per device).
+int scratch_ptr(float **out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return aq_fail(AQ_ERR_HIP, hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return aq_fail(AQ_ERR_INVALID, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return aq_fail(AQ_ERR_HIP, (std::string("scratch address: ") + hipGetErrorString(e)).c_str());
+    if (!p) return aq_fail(AQ_ERR_HIP, "scratch address is null");
+    g_scratch_ptr[dev] = (float *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return AQ_OK;
+}
+}  // namespace
+
+extern "C" int aq_mlp_backward_noalloc(const float *params, const float *x, const float *act, const float *y,
+                                       const float *target, int32_t nsamples, float weight_decay, float *grad,
+                                       double *loss, void *stream) {
+  // every check that needs no device comes before the scratch lookup
+  if (nsamples < 1) return aq_fail(AQ_ERR_INVALID, "no samples");
+  if (nsamples > AQ_MLP_NOALLOC_MAX_SAMPLES)
+    return aq_fail(AQ_ERR_CAPACITY, "more samples than the static delta buffer holds (AQ_MLP_NOALLOC_MAX_SAMPLES)");
+  if (!(weight_decay >= 0.0f) || !std::isfinite(weight_decay))
+    return aq_fail(AQ_ERR_INVALID, "weight decay must be finite and >= 0");
+  if (!params || !x || !act || !y || !target || !grad) return aq_fail(AQ_ERR_INVALID, "null buffer");
+  float *delta = nullptr;
+  const int rc = scratch_ptr(&delta);
+  if (rc) return rc;
+  // every delta entry the gradient kernel reads is written by this call first
+  return aq_mlp_backward(params, x, act, y, target, nsamples, weight_decay, delta, grad, loss, stream);
 }
 
 extern "C" int aq_adam_step(float *params, const float *grad, float *m, float *v, int32_t n, int32_t t, float lr,

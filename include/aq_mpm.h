@@ -313,6 +313,18 @@ int aq_mlp_forward(const float *params, const float *x, int32_t nsamples, float 
 int aq_mlp_backward(const float *params, const float *x, const float *act, const float *y,
                     const float *target, int32_t nsamples, float weight_decay, float *delta, float *grad,
                     double *loss, void *stream);
+/* aq_mlp_backward without the delta scratch argument, for callers that cannot
+   allocate device memory (code under HIP graph capture, serving loops,
+   harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+   deltas live in a static device array of AQ_MLP_NOALLOC_MAX_SAMPLES
+   samples (1.6 MB per GPU, held until the process ends); nsamples above
+   that returns AQ_ERR_CAPACITY.  Allocates nothing.  All calls on one GPU
+   share the array: calls that overlap in time on one GPU (other streams or
+   threads) race.  Results equal aq_mlp_backward's. */
+#define AQ_MLP_NOALLOC_MAX_SAMPLES 2048
+int aq_mlp_backward_noalloc(const float *params, const float *x, const float *act, const float *y,
+                            const float *target, int32_t nsamples, float weight_decay, float *grad,
+                            double *loss, void *stream);
 /* One Adam step (bias-corrected, step t >= 1) on n parameters. */
 int aq_adam_step(float *params, const float *grad, float *m, float *v, int32_t n, int32_t t, float lr,
                  float beta1, float beta2, float eps, void *stream);

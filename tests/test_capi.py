@@ -129,7 +129,22 @@ def test_header_constants_match_binding():
     from autoquantizer_amd import _lib
     txt = open(os.path.join(ROOT, "include", "aq_mpm.h")).read()
     defs = dict(re.findall(r"#define (AQ_[A-Z_0-9]+) \(?(-?\d+)\)?", txt))
-    for name in ("AQ_MAX_COMPS", "AQ_MAX_WORDS", "AQ_MLP_IN", "AQ_MLP_HIDDEN", "AQ_MLP_OUT", "AQ_MLP_PARAMS"):
+    for name in ("AQ_MAX_COMPS", "AQ_MAX_WORDS", "AQ_MLP_IN", "AQ_MLP_HIDDEN", "AQ_MLP_OUT", "AQ_MLP_PARAMS",
+                 "AQ_MLP_NOALLOC_MAX_SAMPLES"):
         assert int(defs[name]) == getattr(_lib, name), name
     h, i, o = _lib.AQ_MLP_HIDDEN, _lib.AQ_MLP_IN, _lib.AQ_MLP_OUT
     assert _lib.AQ_MLP_PARAMS == h * i + h + 2 * (h * h + h) + o * h + o
+
+
+def test_noalloc_backward_rejects_before_device_work():
+    """aq_mlp_backward_noalloc checks its sample count against the static
+    buffer (and its other arguments) before it touches a device."""
+    from autoquantizer_amd import _lib
+    L = _lib.lib()
+    p = ctypes.c_void_p(0x1000)  # never dereferenced: every call below fails its checks first
+    n = _lib.AQ_MLP_NOALLOC_MAX_SAMPLES + 1
+    assert L.aq_mlp_backward_noalloc(p, p, p, p, p, n, 0.0, p, None, None) == -3
+    assert b"static delta buffer" in L.aq_last_error()
+    assert L.aq_mlp_backward_noalloc(p, p, p, p, p, 0, 0.0, p, None, None) == -1
+    assert L.aq_mlp_backward_noalloc(p, p, p, p, p, 4, -1.0, p, None, None) == -1
+    assert L.aq_mlp_backward_noalloc(p, p, p, p, p, 4, 0.0, None, None, None) == -1
