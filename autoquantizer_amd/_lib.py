@@ -80,7 +80,16 @@ class SimDesc(ctypes.Structure):
                 ("p_mass", ctypes.c_double), ("gravity", ctypes.c_double),
                 ("la", ctypes.c_double), ("mu_0", ctypes.c_double),
                 ("la_0", ctypes.c_double), ("seed", ctypes.c_uint64),
-                ("comps", Component * AQ_MAX_COMPS)]
+                ("comps", Component * AQ_MAX_COMPS),
+                ("axis_bounds", ctypes.c_int32), ("lb", ctypes.c_int32 * 3), ("ub", ctypes.c_int32 * 3),
+                ("bound_clip", ctypes.c_int32)]
+
+
+class Collider(ctypes.Structure):
+    _fields_ = [("sdf", ctypes.c_void_p), ("normal", ctypes.c_void_p),
+                ("size", ctypes.c_int32 * 3), ("offset", ctypes.c_int32 * 3), ("center", ctypes.c_int32 * 3),
+                ("lag", ctypes.c_int32), ("velocity_from_rotated", ctypes.c_int32), ("first_step", ctypes.c_uint32),
+                ("omega", ctypes.c_double), ("height_rate", ctypes.c_double)]
 
 
 class AQError(RuntimeError):
@@ -97,7 +106,8 @@ EXPORTS = [
     "aq_update_limits", "aq_advance_limits", "aq_energy", "aq_sim_grid", "aq_advance_timed",
     "aq_adj_workspace_bytes", "aq_adj_substep", "aq_noise_add", "aq_noise_dot", "aq_codec_selftest", "aq_mlp_forward",
     "aq_mlp_backward", "aq_adam_step", "aq_sim_specialize", "aq_jit_build", "aq_mlp_backward_noalloc",
-    "aq_sim_count_wraps",
+    "aq_sim_count_wraps", "aq_group_workspace_bytes", "aq_group_create", "aq_group_advance", "aq_group_destroy",
+    "aq_sim_set_collider",
 ]
 
 _lib = None
@@ -127,7 +137,8 @@ def lib():
     L.aq_sim_destroy.argtypes = [P]
     L.aq_substep.argtypes = [P, P, i64, P, u32, P]
     L.aq_p2g.argtypes = [P, P, i64, P]
-    L.aq_grid_op.argtypes = [P, P]
+    L.aq_grid_op.argtypes = [P, u32, P]
+    L.aq_sim_set_collider.argtypes = [P, P]
     L.aq_g2p.argtypes = [P, P, i64, P, u32, P]
     L.aq_advance.argtypes = [P, P, i64, i32, u32, i32, P, P]
     L.aq_sort.argtypes = [P, P, i64, P, P]
@@ -150,8 +161,14 @@ def lib():
     L.aq_sim_count_wraps.argtypes = [P, P]
     L.aq_mlp_backward_noalloc.argtypes = [P, P, P, P, P, i32, f32, P, P, P]
     L.aq_adam_step.argtypes = [P, P, P, P, i32, i32, f32, f32, f32, f32, P]
+    L.aq_group_workspace_bytes.restype = i64
+    L.aq_group_workspace_bytes.argtypes = [i32]
+    L.aq_group_create.argtypes = [P, P, P, P, i32, P, i64, ctypes.POINTER(P)]
+    L.aq_group_advance.argtypes = [P, i32, u32, i32, i32, P]
+    L.aq_group_destroy.argtypes = [P]
     for name in EXPORTS:
-        if name not in ("aq_last_error", "aq_version", "aq_sim_workspace_bytes", "aq_adj_workspace_bytes"):
+        if name not in ("aq_last_error", "aq_version", "aq_sim_workspace_bytes", "aq_adj_workspace_bytes",
+                        "aq_group_workspace_bytes"):
             getattr(L, name).restype = i32
     _lib = L
     return L

@@ -13,13 +13,14 @@ that times the candidates on the simulation itself.  Every quantized run is
 judged by the final-position error against an unquantized FP64 run of the
 same scene.
 """
+import ctypes
 import time
 from dataclasses import dataclass, field
 
 import numpy as np
 import torch
 
-from . import MPMConfig, MPMSolver
+from . import MPMConfig, MPMSolver, _lib
 from .mpm import component_names
 from .quant import quant
 
@@ -123,6 +124,54 @@ def spec_state(spec):
     return st
 
 
+class SimGroup:
+    """Quantized simulations advanced together through one aq_group: every
+    substep's fused G2P->P2G is one launch per class of simulations
+    (dimension, material, record-word bucket) whose workgroups walk the
+    items of all of them, and the gather + grid update one launch per
+    dimension.  States, particle counts and ids are bound at creation; all
+    simulations must be at the same step."""
+
+    def __init__(self, sims):
+        self.sims = list(sims)
+        n = len(self.sims)
+        L = _lib.lib()
+        nbytes = L.aq_group_workspace_bytes(n)
+        if nbytes < 0:
+            _lib.check(nbytes)
+        dev = self.sims[0].state.device
+        self.workspace = torch.empty(int(nbytes) + 256, dtype=torch.uint8, device=dev)
+        base = self.workspace.data_ptr()
+        ws = ctypes.c_void_p((base + 255) // 256 * 256)
+        V = ctypes.c_void_p * n
+        handles = V(*[sim._h.value for sim in self.sims])
+        states = V(*[sim.state.data_ptr() for sim in self.sims])
+        ids = V(*[sim.ids.data_ptr() for sim in self.sims])
+        ns = (ctypes.c_int64 * n)(*[sim.current_idx for sim in self.sims])
+        h = ctypes.c_void_p()
+        _lib.check(L.aq_group_create(handles, states, ns, ids, n, ws, int(nbytes), ctypes.byref(h)))
+        self._h = h
+
+    def advance(self, nsteps, sort_every, workgroups=0):
+        """nsteps substeps of every simulation, sorted every `sort_every`;
+        `workgroups`: grid of the fused launches (0: from the particle count)."""
+        step0 = self.sims[0].step_count
+        if any(sim.step_count != step0 for sim in self.sims):
+            raise ValueError("the simulations of a group must be at the same step")
+        _lib.check(_lib.lib().aq_group_advance(self._h, int(nsteps), int(step0), int(sort_every),
+                                               int(workgroups), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        for sim in self.sims:
+            sim.step_count += int(nsteps)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().aq_group_destroy(h)
+            except Exception:
+                pass
+
+
 class Batch:
     """The quantized simulations of a batch (and, on request, their FP64
     references), set to their initial states."""
@@ -202,6 +251,74 @@ class Batch:
         torch.cuda.synchronize()
         self.graph, self.lanes = g, lanes
         return g
+
+    def group(self):
+        """The batch's simulations as one SimGroup (created once)."""
+        if getattr(self, "_group", None) is None:
+            self._group = SimGroup(self.sims)
+        return self._group
+
+    def autotune_group(self, sorts=(8, 16, 32), workgroups=(0, 1024, 2048, 4096), steps=32):
+        """Launch options of the grouped path: the sort interval and the
+        fused launches' grid with the least time for `steps` substeps of the
+        whole batch from the initial states (HIP events, best of two)."""
+        grp = self.group()
+        best = None
+        for k in sorts:
+            for wg in workgroups:
+                ms = []
+                for _ in range(2):
+                    self.reset()
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record()
+                    grp.advance(steps, k, wg)
+                    e1.record()
+                    e1.synchronize()
+                    ms.append(e0.elapsed_time(e1))
+                if best is None or min(ms) < best[2]:
+                    best = (k, wg, min(ms))
+        self.group_sort_every, self.group_workgroups = best[0], best[1]
+        self.reset()
+        return {"sort_every": best[0], "workgroups": best[1], "ms_per_substep": best[2] / steps}
+
+    def run_group(self, nsteps, sort_every=None, workgroups=None):
+        """All simulations through the grouped launches; returns wall seconds."""
+        grp = self.group()
+        k = sort_every or getattr(self, "group_sort_every", 16)
+        wg = workgroups if workgroups is not None else getattr(self, "group_workgroups", 0)
+        self.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        grp.advance(nsteps, k, wg)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    def capture_group(self, nsteps):
+        """The grouped run captured into one HIP graph (its launches replayed
+        without host work)."""
+        grp = self.group()
+        k = getattr(self, "group_sort_every", 16)
+        wg = getattr(self, "group_workgroups", 0)
+        self.reset()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            grp.advance(nsteps, k, wg)
+        torch.cuda.synchronize()
+        self.group_graph, self.group_graph_steps = g, int(nsteps)
+        return g
+
+    def run_group_graph(self):
+        """One replay of the grouped graph from the initial states; wall seconds."""
+        self.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        self.group_graph.replay()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        for sim in self.sims:
+            sim.step_count += self.group_graph_steps
+        return el
 
     def run_graph(self):
         """One launch of the captured graph from the initial states; returns

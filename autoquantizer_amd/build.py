@@ -13,7 +13,8 @@ from concurrent.futures import ThreadPoolExecutor
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "libaqmpm.so")
-SOURCES = [os.path.join(PKG, "csrc", "aq_mpm.hip"), os.path.join(PKG, "csrc", "aq_mlp.hip")]
+SOURCES = [os.path.join(PKG, "csrc", f) for f in ("aq_mpm.hip", "aq_particles.hip", "aq_fused.hip", "aq_group.hip",
+                                                  "aq_adjoint.hip", "aq_mlp.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 OBJ_DIR = os.path.join(PKG, "_build")
 

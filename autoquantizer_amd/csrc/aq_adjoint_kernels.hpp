@@ -216,17 +216,16 @@ __global__ void __launch_bounds__(256) k_adj_grid(ParamsT<T> P, int32_t nblocks,
   T v[3] = {inv_m * mv[0], inv_m * mv[1], inv_m * mv[2]};
   v[1] = v[1] - P.g_dt;
   T gb[3] = {gbar[node * 4 + 0], gbar[node * 4 + 1], gbar[node * 4 + 2]};
-  const int lb = P.bound, ub = P.n - P.bound;
   if (P.boundary == AQ_STICKY) {
     bool wall = false;
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) wall = wall || I[d] < lb || I[d] > ub;
+    for (int d = 0; d < DIM; ++d) wall = wall || I[d] < P.lb[d] || I[d] > P.ub[d];
     if (wall) gb[0] = gb[1] = gb[2] = (T)0;
   } else {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-      if (I[d] < lb && v[d] < (T)0) gb[d] = (T)0;
-      if (I[d] > ub && v[d] > (T)0) gb[d] = (T)0;
+      if (I[d] < P.lb[d] && v[d] < (T)0) gb[d] = (T)0;
+      if (I[d] > P.ub[d] && v[d] > (T)0) gb[d] = (T)0;
     }
   }
   T mb = (T)0;
@@ -274,6 +273,35 @@ __device__ __forceinline__ void adj_corotated(const ParamsT<T> &P, const T (&F)[
   Fb[3] += Jb * F[0];
   Fb[1] -= Jb * F[2];
   Fb[2] -= Jb * F[1];
+}
+
+// 3D elastic stress adjoint (sigma = mu_0 (F F^T - I) + la_0 log(J) on
+// every entry): F_bar += mu_0 (sigma_bar + sigma_bar^T) F
+//                      + la_0 (sum_ij sigma_bar_ij) F^-T,  F^-T = cof(F) / J.
+template <class T>
+__device__ __forceinline__ void adj_elastic3(const ParamsT<T> &P, const T (&F)[9], const T (&sb)[9], T (&Fb)[9]) {
+  const T mu = P.two_mu * (T)0.5;
+  T ssum = (T)0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) ssum += sb[i];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      T acc = (T)0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) acc += (sb[a * 3 + j] + sb[j * 3 + a]) * F[j * 3 + b];
+      Fb[a * 3 + b] += mu * acc;
+    }
+  // cofactors: cof_ab = (-1)^(a+b) minor_ab; F^-T = cof / det
+  const T c00 = F[4] * F[8] - F[5] * F[7], c01 = F[5] * F[6] - F[3] * F[8], c02 = F[3] * F[7] - F[4] * F[6];
+  const T c10 = F[2] * F[7] - F[1] * F[8], c11 = F[0] * F[8] - F[2] * F[6], c12 = F[1] * F[6] - F[0] * F[7];
+  const T c20 = F[1] * F[5] - F[2] * F[4], c21 = F[2] * F[3] - F[0] * F[5], c22 = F[0] * F[4] - F[1] * F[3];
+  const T J = F[0] * c00 + F[1] * c01 + F[2] * c02;
+  const T k = P.la_0 * ssum / J;
+  const T cof[9] = {c00, c01, c02, c10, c11, c12, c20, c21, c22};
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Fb[i] += k * cof[i];
 }
 
 // P2G adjoint: completes adj_out and adds sum_p adj_out[c][p]^2 to gsum[c].
@@ -356,6 +384,17 @@ __global__ void __launch_bounds__(256) k_adj_p2g(const T *__restrict__ st, T *__
 #pragma unroll
       for (int a = 0; a < DIM; ++a) tr += affb[a * DIM + a];
       jfb[0] += P.la * (P.stress_k * tr);
+    } else if constexpr (DIM == 3) {
+      T F[9], sb[9], Fb[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        F[i] = v[CI::CJ + i];
+        sb[i] = P.stress_k * affb[i];
+        Fb[i] = (T)0;
+      }
+      adj_elastic3<T>(P, F, sb, Fb);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) jfb[i] += Fb[i];
     } else {
       const T F[4] = {v[CI::CJ + 0], v[CI::CJ + 1], v[CI::CJ + 2], v[CI::CJ + 3]};
       T sb[4], Fb[4] = {(T)0, (T)0, (T)0, (T)0};

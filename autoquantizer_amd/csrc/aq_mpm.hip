@@ -8,39 +8,25 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
 
-#include "aq_mpm.h"
-#include "aq_adjoint_kernels.hpp"
-#include "aq_mpm_kernels.hpp"
-#include "aq_tiled.hpp"
-
-using namespace aq;
+#include "aq_sim.hpp"
+#include "aq_gather.hpp"
 
 // ---------------------------------------------------------------------------
 // errors
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
 
-static int fail(int code, const std::string &msg) {
+extern "C" const char *aq_last_error(void) { return g_err.c_str(); }
+// the error of every translation unit of the library
+__attribute__((visibility("hidden"))) int aq_fail(int code, const char *msg) {
   g_err = msg;
   return code;
 }
-
-#define HIP_TRY(expr)                                                              \
-  do {                                                                             \
-    hipError_t e_ = (expr);                                                        \
-    if (e_ != hipSuccess)                                                          \
-      return fail(AQ_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));  \
-  } while (0)
-
-#define LAUNCH_CHECK() HIP_TRY(hipGetLastError())
-
-extern "C" const char *aq_last_error(void) { return g_err.c_str(); }
-// for the other translation units of the library (aq_mlp.hip)
-__attribute__((visibility("hidden"))) int aq_fail(int code, const char *msg) { return fail(code, msg); }
 extern "C" const char *aq_version(void) { return "aqmpm 0.1 gfx950"; }
 
 // ---------------------------------------------------------------------------
@@ -105,35 +91,6 @@ static int build_layout(const aq_component *comps, int nc, int nw, LayoutK *out)
   }
   return AQ_OK;
 }
-
-struct KeysK {
-  uint32_t k[AQ_MAX_COMPS];
-};
-
-// Keys of one (seed, step): key_j = h(j ^ key_base), key_base = h(step ^
-// h(seed_lo ^ h(seed_hi))).  Dithering: component c uses key_{c >> 1} (one
-// hash per component pair, dither_u); modelled noise: component c uses
-// key_c (noise_u).
-static uint32_t host_hash32(uint32_t x) {
-  x ^= x >> 16;
-  x *= 0x7feb352du;
-  x ^= x >> 15;
-  x *= 0x846ca68bu;
-  x ^= x >> 16;
-  return x;
-}
-static uint32_t key_base(uint64_t seed, uint32_t step) {
-  const uint32_t k = host_hash32((uint32_t)seed ^ host_hash32((uint32_t)(seed >> 32)));
-  return host_hash32(step ^ k);
-}
-static KeysK make_keys(uint64_t seed, uint32_t step) {
-  KeysK K;
-  const uint32_t k = key_base(seed, step);
-  for (int c = 0; c < AQ_MAX_COMPS; ++c) K.k[c] = host_hash32((uint32_t)c ^ k);
-  return K;
-}
-
-static int nw32_bucket(int nw) { return nw <= 2 ? 4 : nw <= 4 ? 8 : nw <= 8 ? 16 : 32; }
 
 // ---------------------------------------------------------------------------
 // codec kernels
@@ -407,228 +364,92 @@ extern "C" int aq_layout_plan(const aq_container *cs, int ncont, const aq_place 
 // ---------------------------------------------------------------------------
 // MPM kernels
 // ---------------------------------------------------------------------------
-// Untiled kernels (one particle per thread, grid in global memory).  P2G
-// scatters with float atomics into the accumulation grid grid_f and flags
-// the touched blocks; the gather kernel turns grid_f into grid_v.
-template <int DIM, int MAT, int ST, int NW32, class T>
-__global__ void __launch_bounds__(256) k_p2g(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
-                                             vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags) {
-  __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
-  const int64_t p = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  constexpr int NC = Comps<DIM, MAT>::NC;
-  T v[NC];
-  load_state<DIM, ST, NW32, NC, T>(s, cod, p, 0xFFFFFFFFu, rows + threadIdx.x * RowsOf<ST, NW32>::RS, v);
-  p2g_particle<DIM, MAT, T>(v, P, grid_f, fflags);
-}
-
-template <int DIM, int MAT, int ST, int NW32, class T>
-__global__ void __launch_bounds__(256) k_g2p(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
-                                             const vec4_t<T> *__restrict__ grid_v) {
-  __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
-  const int64_t p = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  constexpr int NC = Comps<DIM, MAT>::NC;
-  uint32_t *row = rows + threadIdx.x * RowsOf<ST, NW32>::RS;
-  T v[NC], out[NC];
-  load_state<DIM, ST, NW32, NC, T, true>(s, cod, p, s.g2p_mask, row, v);
-  g2p_particle<DIM, MAT, T>(v, P, grid_v, out);
-  store_state<ST, NW32, NC, T>(s, cod, p, out, row);
-}
-
-// G2P of substep t (reading grid_v) fused with P2G of substep t+1
-// (scattering into grid_f): the state is read once and written once.
-template <int DIM, int MAT, int ST, int NW32, class T>
-__global__ void __launch_bounds__(256) k_g2p2g(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
-                                               const vec4_t<T> *__restrict__ grid_v,
-                                               vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags) {
-  __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
-  const int64_t p = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  constexpr int NC = Comps<DIM, MAT>::NC;
-  uint32_t *row = rows + threadIdx.x * RowsOf<ST, NW32>::RS;
-  T v[NC], out[NC];
-  load_state<DIM, ST, NW32, NC, T, true>(s, cod, p, s.g2p_mask, row, v);
-  g2p_particle<DIM, MAT, T>(v, P, grid_v, out);
-  store_state<ST, NW32, NC, T>(s, cod, p, out, row);  // out: what storage now holds
-  p2g_particle<DIM, MAT, T>(out, P, grid_f, fflags);
-}
-
-// Grid update (mpm_solver.py:633-665) of node I with accumulated (m v, m).
-template <int DIM, class T>
-__device__ __forceinline__ vec4_t<T> grid_op_node(const ParamsT<T> &P, const int (&I)[3], vec4_t<T> nd) {
-  const T inv_m = (T)1 / (nd.w + P.eps_m);
-  T v[3] = {inv_m * nd.x, inv_m * nd.y, inv_m * nd.z};
-  v[1] = v[1] - P.g_dt;
-  const int lb = P.bound, ub = P.n - P.bound;
-  if (P.boundary == AQ_STICKY) {
-    bool wall = false;
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) wall = wall || I[d] < lb || I[d] > ub;
-    if (wall) v[0] = v[1] = v[2] = (T)0;
-  } else {
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-      if (I[d] < lb && v[d] < (T)0) v[d] = (T)0;
-      if (I[d] > ub && v[d] > (T)0) v[d] = (T)0;
-    }
-  }
-  nd.x = v[0];
-  nd.y = v[1];
-  nd.z = DIM == 3 ? v[2] : (T)0;
-  return nd;
-}
-
-// Local coordinates of lane `lane` in a 64-node block.
-template <int DIM> __device__ __forceinline__ void lane_coords(int lane, int (&l)[3]) {
-  if constexpr (DIM == 3) {
-    l[0] = lane >> 4;
-    l[1] = (lane >> 2) & 3;
-    l[2] = lane & 3;
-  } else {
-    l[0] = lane >> 3;
-    l[1] = lane & 7;
-    l[2] = 0;
-  }
-}
-
-// Gather of P2G results into grid_v, one 64-node block per wavefront.  A
-// node is covered by the tiles of 2^DIM neighbouring blocks (per axis its own
-// block and the one on the side of the block edge it is nearest); each lane
-// reads the item ranges of its 2^DIM blocks and the first two items' tile
-// values all at once (independent loads), then any further items, and sums
-// in a fixed order (neighbour, then item).  A node in a tile's rim is read
-// only when the item's rim flags say it was written (otherwise it is zero);
-// grid_f is added for blocks flagged by untiled scatters (and zeroed after
-// use).  OP applies the grid update; without it grid_v holds (m v, m).
-// Blocks with no contribution are marked inactive in `act` (and zeroed if
-// they were active).
 template <int DIM, class T, bool OP>
-__global__ void __launch_bounds__(256) k_gather(ParamsT<T> P, int32_t nblocks, const int32_t *__restrict__ item_off,
+__global__ void __launch_bounds__(256) k_gather(ParamsT<T> P, ColliderK K, int32_t nblocks,
+                                                const int32_t *__restrict__ item_off,
                                                 const uint32_t *__restrict__ near,
                                                 const vec4_t<T> *__restrict__ tiles, vec4_t<T> *__restrict__ grid_f,
                                                 uint32_t *__restrict__ fflags, vec4_t<T> *__restrict__ grid_v,
                                                 uint32_t *__restrict__ act) {
-  using TL = Tile<DIM>;
-  constexpr int E = TL::E, TN = TL::TN, ST = TL::STRIDE, NN = 1 << DIM;
   const int32_t b = __builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * 4u + (threadIdx.x >> 6)));
-  const int lane = threadIdx.x & 63;
   if (b >= nblocks) return;
-  const bool fall = fflags[b] != 0u;
-  // a block with no items in its neighbourhood and no untiled scatter has
-  // nothing to gather: skip the item ranges of its neighbours
-  if (item_off && !near[b] && !fall) {
-    if (act[b]) {
-      vec4_t<T> z;
-      z.x = z.y = z.z = z.w = (T)0;
-      grid_v[(int64_t)b * 64 + lane] = z;
+  gather_block<DIM, T, OP>(P, K, b, item_off, near, tiles, grid_f, fflags, grid_v, act);
+}
+
+// Gather over the block list of the last sort (tiled P2G): gcnt[0] blocks
+// of glist (near a particle or active at the sort, and those first reached
+// since), then the gcnt[1] blocks of flist (flagged by untiled scatters
+// since the last gather) that glist does not hold.  A flagged block first
+// reached since the sort is appended to glist (at gcnt[3], the append
+// cursor), so that later gathers visit it and zero its grid once nothing
+// reaches it; k_glist_commit, launched after the gather, commits the
+// appended entries and empties flist.  (The last workgroup to finish doing
+// that instead costs 2048 serialised atomics on one address: +35 us.)
+template <int DIM, class T, bool OP>
+__global__ void __launch_bounds__(256) k_gather_list(ParamsT<T> P, ColliderK K, const int32_t *__restrict__ item_off,
+                                                     const uint32_t *__restrict__ near,
+                                                     const vec4_t<T> *__restrict__ tiles,
+                                                     vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags,
+                                                     vec4_t<T> *__restrict__ grid_v, uint32_t *__restrict__ act,
+                                                     int32_t *__restrict__ glist, const int32_t *__restrict__ flist,
+                                                     uint32_t *__restrict__ gcnt, uint32_t *__restrict__ listed) {
+  const uint32_t gc = gcnt[0], fc = gcnt[1];
+  const uint32_t nw = gridDim.x * 4u;
+  const int lane = threadIdx.x & 63;
+  for (uint32_t i = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * 4u + (threadIdx.x >> 6)));
+       i < gc + fc; i += nw) {
+    int32_t b;
+    if (i < gc) {
+      b = glist[i];
+    } else {
+      b = flist[i - gc];
+      if (listed[b]) continue;  // glist holds it
       __builtin_amdgcn_wave_barrier();
-      if (lane == 0) act[b] = 0u;
+      if (lane == 0) {
+        listed[b] = 1u;
+        glist[atomicAdd(&gcnt[3], 1u)] = b;
+      }
     }
-    return;
+    b = __builtin_amdgcn_readfirstlane(b);
+    gather_block<DIM, T, OP>(P, K, b, item_off, near, tiles, grid_f, fflags, grid_v, act);
   }
-  const int nb = P.nb;
-  int bc[3], loc[3];
-  block_coords<DIM>(b, nb, bc);
-  lane_coords<DIM>(lane, loc);
-  int32_t r0[NN], r1[NN], q[NN];
-  uint32_t need[NN];
-#pragma unroll
-  for (int k = 0; k < NN; ++k) {
-    int d[3] = {0, 0, 0}, t[3] = {0, 0, 0};
-    bool ok = item_off != nullptr;
-#pragma unroll
-    for (int a = 0; a < DIM; ++a) {
-      d[a] = ((k >> a) & 1) ? (loc[a] == E - 1 ? 1 : -1) : 0;
-      const int c = bc[a] + d[a];
-      t[a] = loc[a] + 1 - d[a] * E;
-      ok = ok && c >= 0 && c < nb && t[a] < TL::TE;  // 2D: only 3 nodes reach past the edge
-    }
-    q[k] = ok ? tile_pos<DIM>(t[0], t[1], t[2]) : 0;
-    need[k] = TL::need(t[0], t[1], t[2]);
-    r0[k] = r1[k] = 0;
-    if (ok) {
-      const int32_t nbid = DIM == 3 ? ((bc[0] + d[0]) * nb + (bc[1] + d[1])) * nb + (bc[2] + d[2])
-                                    : (bc[0] + d[0]) * nb + (bc[1] + d[1]);
-      r0[k] = item_off[nbid];
-      r1[k] = item_off[nbid + 1];
-    }
+}
+
+// After a list gather: the blocks it appended join glist; flist is empty.
+__global__ void k_glist_commit(uint32_t *__restrict__ gcnt) {
+  if (threadIdx.x == 0) {
+    gcnt[0] = gcnt[3];
+    gcnt[1] = 0u;
   }
-  bool mine = false;
-#pragma unroll
-  for (int k = 0; k < NN; ++k) mine = mine || r1[k] > r0[k];
-  const bool any = fall || __ballot(mine) != 0ull;
-  const int64_t node = (int64_t)b * 64 + lane;
-  if (!any) {
-    if (act[b]) {
-      vec4_t<T> z;
-      z.x = z.y = z.z = z.w = (T)0;
-      grid_v[node] = z;
-      __builtin_amdgcn_wave_barrier();
-      if (lane == 0) act[b] = 0u;
-    }
-    return;
+}
+
+// The gather list of a sort: every block with particles within one block
+// (near) or a grid left from before (act), once (listed[b] marks them);
+// slots from gcnt[0], with gcnt[3] counting alike (both zero before).
+__global__ void __launch_bounds__(256) k_glist_build(const uint32_t *__restrict__ near,
+                                                     const uint32_t *__restrict__ act, uint32_t *__restrict__ listed,
+                                                     int32_t *__restrict__ glist, uint32_t *__restrict__ gcnt,
+                                                     int32_t m) {
+  const int32_t b = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  const bool in = b < m && (near[b] != 0u || act[b] != 0u);
+  if (b < m) listed[b] = in ? 1u : 0u;
+  const unsigned long long mask = __ballot(in);
+  if (mask == 0ull) return;  // uniform over the wavefront
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)mask) - 1;
+  uint32_t base = 0u;
+  if (lane == leader) {
+    base = atomicAdd(&gcnt[0], (uint32_t)__popcll(mask));
+    atomicAdd(&gcnt[3], (uint32_t)__popcll(mask));
   }
-  // rim faces of item `it` reached by its particles (stored after its nodes)
-  auto rim = [&](int32_t it) { return *(const uint32_t *)(tiles + (int64_t)it * ST + TN); };
-  vec4_t<T> c0[NN], c1[NN];
-#pragma unroll
-  for (int k = 0; k < NN; ++k) {
-    c0[k].x = c0[k].y = c0[k].z = c0[k].w = (T)0;
-    c1[k] = c0[k];
-    if (r1[k] > r0[k] && (need[k] == 0u || (rim(r0[k]) & need[k]) == need[k]))
-      c0[k] = tiles[(int64_t)r0[k] * ST + q[k]];
-    if (r1[k] > r0[k] + 1 && (need[k] == 0u || (rim(r0[k] + 1) & need[k]) == need[k]))
-      c1[k] = tiles[(int64_t)(r0[k] + 1) * ST + q[k]];
-  }
-  vec4_t<T> sum;
-  sum.x = sum.y = sum.z = sum.w = (T)0;
-#pragma unroll
-  for (int k = 0; k < NN; ++k) {
-    sum.x += c0[k].x;
-    sum.y += c0[k].y;
-    sum.z += c0[k].z;
-    sum.w += c0[k].w;
-    sum.x += c1[k].x;
-    sum.y += c1[k].y;
-    sum.z += c1[k].z;
-    sum.w += c1[k].w;
-    for (int32_t it = r0[k] + 2; it < r1[k]; ++it) {
-      if (need[k] != 0u && (rim(it) & need[k]) != need[k]) continue;
-      const vec4_t<T> c = tiles[(int64_t)it * ST + q[k]];
-      sum.x += c.x;
-      sum.y += c.y;
-      sum.z += c.z;
-      sum.w += c.w;
-    }
-  }
-  if (fall) {
-    const vec4_t<T> c = grid_f[node];
-    sum.x += c.x;
-    sum.y += c.y;
-    sum.z += c.z;
-    sum.w += c.w;
-    vec4_t<T> z;
-    z.x = z.y = z.z = z.w = (T)0;
-    grid_f[node] = z;
-  }
-  if constexpr (OP) {
-    int I[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) I[a] = bc[a] * E + loc[a];
-    sum = grid_op_node<DIM, T>(P, I, sum);
-  }
-  grid_v[node] = sum;
-  __builtin_amdgcn_wave_barrier();
-  if (lane == 0) {
-    act[b] = 1u;
-    if (fall) fflags[b] = 0u;
-  }
+  base = __shfl(base, leader);
+  if (in) glist[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = b;
 }
 
 // Grid update in place on the active blocks of grid_v (after a gather without OP).
 template <int DIM, class T>
-__global__ void __launch_bounds__(256) k_grid_update(ParamsT<T> P, int32_t nblocks, vec4_t<T> *__restrict__ grid_v,
+__global__ void __launch_bounds__(256) k_grid_update(ParamsT<T> P, ColliderK K, int32_t nblocks,
+                                                     vec4_t<T> *__restrict__ grid_v,
                                                      const uint32_t *__restrict__ act) {
   const int32_t b = (int32_t)(blockIdx.x * 4u + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
@@ -639,7 +460,7 @@ __global__ void __launch_bounds__(256) k_grid_update(ParamsT<T> P, int32_t nbloc
 #pragma unroll
   for (int a = 0; a < 3; ++a) I[a] = bc[a] * BlockShape<DIM>::E + loc[a];
   const int64_t node = (int64_t)b * 64 + lane;
-  grid_v[node] = grid_op_node<DIM, T>(P, I, grid_v[node]);
+  grid_v[node] = grid_op_node<DIM, T>(P, K, I, grid_v[node]);
 }
 
 // Work items from the block counts and offsets of a sort (item_off from the scan).
@@ -981,41 +802,6 @@ __global__ void __launch_bounds__(256) k_energy(StateK s, CodecK cod, ParamsT<T>
 // ---------------------------------------------------------------------------
 // simulation handle
 // ---------------------------------------------------------------------------
-// Grids: grid_v holds the gathered P2G result ((m v, m) after aq_p2g,
-// velocities after the grid update) on the blocks flagged in `act`, zeros
-// elsewhere; grid_f is the float-atomic accumulation grid of untiled
-// scatters (zero outside the blocks flagged in fflags); `tiles` holds one
-// P2G tile per work item.
-struct aq_sim {
-  aq_sim_desc d;
-  LayoutK lay;
-  CompK *lay_dev;  // lay.c in the workspace, read by the MPM kernels
-  ParamsT<float> pf;
-  ParamsT<double> pd;
-  int64_t nodes;
-  int32_t nblocks;
-  int32_t node_bytes;
-  void *grid_v, *grid_f, *tiles;
-  uint32_t *fflags, *act;
-  uint32_t *skey, *srank, *scount, *ccount, *soff, *spart;
-  void *tmp_state;
-  int32_t *tmp_ids;
-  int4 *items;
-  int32_t *item_off;  // [nblocks + 1]; item_off[nblocks] = item count
-  uint32_t *near;     // [nblocks]: particles within one block (k_near, at each sort)
-  float *lim_part;    // [kLimitBlocks][2 ncomp]: partials of k_limits
-  int32_t max_items;
-  bool have_items;    // items describe the current order of `items_state`
-  const void *items_state;
-  int64_t items_n;
-  bool dither_any;
-  bool stream_order;  // the encoder's field order has non-decreasing offsets
-  int64_t ws_bytes;
-  hipModule_t jit_mod = nullptr;      // fused kernel specialised to the layout (aq_sim_specialize)
-  hipFunction_t jit_fn = nullptr;
-  unsigned long long *wraps = nullptr;  // range-violation counters (aq_sim_count_wraps)
-};
-
 // Whether the record's fields, in the encoder's order (components in order,
 // a shared group's exponent before its first member, an own exponent before
 // its digits), lie at non-decreasing bit offsets: then records are written
@@ -1036,14 +822,12 @@ static bool stream_ordered(const LayoutK &lay) {
   return true;
 }
 
-static int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
 static int check_desc(const aq_sim_desc *d) {
   if (!d) return fail(AQ_ERR_INVALID, "null descriptor");
   if (d->dim != 2 && d->dim != 3) return fail(AQ_ERR_INVALID, "dim must be 2 or 3");
   if (d->material != AQ_FLUID && d->material != AQ_ELASTIC) return fail(AQ_ERR_INVALID, "bad material");
-  if (d->material == AQ_ELASTIC && d->dim != 2)
-    return fail(AQ_ERR_INVALID, "fixed-corotated elasticity is implemented in 2D");
+
   const int e = d->dim == 3 ? 4 : 8;
   if (d->n_grid < 8 || d->n_grid % e) return fail(AQ_ERR_INVALID, "n_grid must be a multiple of the block edge");
   if (d->dim == 3 && d->n_grid > 1024) return fail(AQ_ERR_INVALID, "n_grid too large");
@@ -1054,6 +838,12 @@ static int check_desc(const aq_sim_desc *d) {
   if (d->capacity < 0 || d->capacity > 0x7FFFFFFFll) return fail(AQ_ERR_INVALID, "capacity out of range");
   if (d->boundary != AQ_SEPARATING && d->boundary != AQ_STICKY) return fail(AQ_ERR_INVALID, "bad boundary");
   if (d->bound < 0 || 2 * d->bound >= d->n_grid) return fail(AQ_ERR_INVALID, "bad wall thickness");
+  if (d->axis_bounds != 0 && d->axis_bounds != 1) return fail(AQ_ERR_INVALID, "axis_bounds must be 0 or 1");
+  if (d->axis_bounds)
+    for (int a = 0; a < d->dim; ++a)
+      if (d->lb[a] < 0 || d->lb[a] >= d->ub[a] || d->ub[a] > d->n_grid)
+        return fail(AQ_ERR_INVALID, "per-axis walls need 0 <= lb < ub <= n_grid");
+  if (d->bound_clip != 0 && d->bound_clip != 1) return fail(AQ_ERR_INVALID, "bound_clip must be 0 or 1");
   return AQ_OK;
 }
 
@@ -1073,7 +863,7 @@ static int32_t max_items_of(const aq_sim_desc *d) {
 static int64_t tile_slots(const aq_sim_desc *d) { return d->dim == 3 ? Tile<3>::STRIDE : Tile<2>::STRIDE; }
 
 enum {
-  WS_GRID_V, WS_GRID_F, WS_FLAGS, WS_KEY, WS_RANK, WS_COUNT, WS_CCOUNT, WS_OFF, WS_PART, WS_TMP_STATE, WS_TMP_IDS,
+  WS_GRID_V, WS_GRID_F, WS_FLAGS, WS_GLIST, WS_KEY, WS_RANK, WS_COUNT, WS_CCOUNT, WS_OFF, WS_PART, WS_TMP_STATE, WS_TMP_IDS,
   WS_ITEMS, WS_ITEM_OFF, WS_NEAR, WS_TILES, WS_LIMITS, WS_LAYOUT, WS_N
 };
 
@@ -1089,6 +879,7 @@ static void ws_plan(const aq_sim_desc *d, int64_t *offs /*[WS_N]*/, int64_t *tot
   sz[WS_GRID_V] = nodes * node_bytes;
   sz[WS_GRID_F] = nodes * node_bytes;
   sz[WS_FLAGS] = nblocks * 4 * 2;  // fflags, act
+  sz[WS_GLIST] = 64 + nblocks * 4 * 3;  // counters; listed flags, block list, flagged-block list
   sz[WS_KEY] = cap * 4;
   sz[WS_RANK] = cap * 4;
   sz[WS_COUNT] = (nblocks + 1) * 4;
@@ -1136,8 +927,14 @@ static ParamsT<T> make_params(const aq_sim_desc *d) {
   P.eps_m = (T)1e-10;
   P.n = d->n_grid;
   P.nb = d->n_grid / (d->dim == 3 ? 4 : 8);
-  P.bound = d->bound;
   P.boundary = d->boundary;
+  P.clip = d->bound_clip;
+  for (int a = 0; a < 3; ++a) {
+    P.lb[a] = d->axis_bounds && a < d->dim ? d->lb[a] : d->bound;
+    P.ub[a] = d->axis_bounds && a < d->dim ? d->ub[a] : d->n_grid - d->bound;
+    P.xlo[a] = (T)(P.lb[a] * d->dx);  // self.lb[k] * self.dx, a Python double folded into the kernel
+    P.xhi[a] = (T)(P.ub[a] * d->dx);
+  }
   return P;
 }
 
@@ -1176,6 +973,10 @@ extern "C" int aq_sim_create(const aq_sim_desc *d, void *ws, int64_t ws_bytes, a
   s->grid_f = base + offs[WS_GRID_F];
   s->fflags = (uint32_t *)(base + offs[WS_FLAGS]);
   s->act = s->fflags + s->nblocks;
+  s->gcnt = (uint32_t *)(base + offs[WS_GLIST]);
+  s->listed = s->gcnt + 16;
+  s->glist = (int32_t *)(s->listed + s->nblocks);
+  s->flist = s->glist + s->nblocks;
   s->skey = (uint32_t *)(base + offs[WS_KEY]);
   s->srank = (uint32_t *)(base + offs[WS_RANK]);
   s->scount = (uint32_t *)(base + offs[WS_COUNT]);
@@ -1213,7 +1014,7 @@ extern "C" int aq_sim_destroy(aq_sim *s) {
   return AQ_OK;
 }
 
-static StateK make_state(const aq_sim *s, const void *state, const int32_t *ids) {
+StateK make_state(const aq_sim *s, const void *state, const int32_t *ids) {
   StateK k;
   k.ptr = (void *)state;
   k.cap = s->d.capacity;
@@ -1239,7 +1040,7 @@ static StateK make_state(const aq_sim *s, const void *state, const int32_t *ids)
   return k;
 }
 
-static CodecK make_codec(const aq_sim *s, uint32_t step) {
+CodecK make_codec(const aq_sim *s, uint32_t step) {
   CodecK c;
   c.c = s->lay_dev;
   c.nc = s->lay.nc;
@@ -1339,9 +1140,9 @@ static int jit_source(const aq_sim &v, std::string &src) {
                 "extern \"C\" __global__ void __launch_bounds__(%d) __attribute__((amdgpu_waves_per_eu(%d)))\n"
                 "aq_jit_tiled(aq::StateK s, aq::CodecK cod, aq::ParamsT<float> P, int64_t n, const int4 *items,\n"
                 "             const int32_t *nitems, const float4 *grid_v, float4 *tiles, float4 *grid_f,\n"
-                "             uint32_t *fflags) {\n"
+                "             uint32_t *fflags, int32_t *flist, uint32_t *fcnt) {\n"
                 "  aq::tiled_body<%d, %d, AQ_STORE_QUANT, %d, float, true, true>(s, cod, P, n, items, nitems, grid_v,\n"
-                "                                                               tiles, grid_f, fflags);\n}\n",
+                "                                                               tiles, grid_f, fflags, flist, fcnt);\n}\n",
                 kItem, kJitWaves, v.d.dim, v.d.material, nw32_bucket(v.d.nwords));
   src += buf;
   return AQ_OK;
@@ -1466,171 +1267,76 @@ extern "C" int aq_sim_specialize(aq_sim *s, const char *src_dir, const char *inc
   return AQ_OK;
 }
 
-// Dispatch over (dim, material, storage, record bucket): calls f(Tag<...>{}).
-template <int D_, int M_, int S_, int W_, class T_> struct Tag {
-  static constexpr int DIM = D_, MAT = M_, ST = S_, NW32 = W_;
-  using T = T_;
-};
-
-template <int D_, int M_, class F>
-static void dispatch_quant(int bucket, F &&f) {
-  if (bucket == 4) f(Tag<D_, M_, AQ_STORE_QUANT, 4, float>{});
-  else if (bucket == 8) f(Tag<D_, M_, AQ_STORE_QUANT, 8, float>{});
-  else if (bucket == 16) f(Tag<D_, M_, AQ_STORE_QUANT, 16, float>{});
-  else f(Tag<D_, M_, AQ_STORE_QUANT, 32, float>{});
-}
-
-template <class F>
-static void dispatch(const aq_sim *s, F &&f) {
-  const aq_sim_desc &d = s->d;
-  if (d.storage == AQ_STORE_QUANT) {
-    const int b = nw32_bucket(d.nwords);
-    if (d.dim == 3) dispatch_quant<3, AQ_FLUID>(b, f);
-    else if (d.material == AQ_FLUID) dispatch_quant<2, AQ_FLUID>(b, f);
-    else dispatch_quant<2, AQ_ELASTIC>(b, f);
-  } else if (d.storage == AQ_STORE_F32) {
-    if (d.dim == 3) f(Tag<3, AQ_FLUID, AQ_STORE_F32, 4, float>{});
-    else if (d.material == AQ_FLUID) f(Tag<2, AQ_FLUID, AQ_STORE_F32, 4, float>{});
-    else f(Tag<2, AQ_ELASTIC, AQ_STORE_F32, 4, float>{});
-  } else {
-    if (d.dim == 3) f(Tag<3, AQ_FLUID, AQ_STORE_F64, 4, double>{});
-    else if (d.material == AQ_FLUID) f(Tag<2, AQ_FLUID, AQ_STORE_F64, 4, double>{});
-    else f(Tag<2, AQ_ELASTIC, AQ_STORE_F64, 4, double>{});
+// The collider at substep `step` (aq_collider's schedule).
+static ColliderK collider_at(const aq_sim *s, uint32_t step) {
+  ColliderK K{};
+  if (!s->has_collider) return K;
+  const aq_collider &c = s->col;
+  const int64_t sub = (int64_t)step - (int64_t)c.first_step;  // the collider's substep
+  int64_t k = sub - c.lag;
+  if (k < 0) k = 0;
+  const double theta = c.omega * s->d.dt * (double)k;
+  const double height = c.height_rate * s->d.dt * (double)k;
+  K.sdf = c.sdf;
+  K.normal = c.normal;
+  for (int a = 0; a < 3; ++a) {
+    K.size[a] = c.size[a];
+    K.offset[a] = c.offset[a];
+    K.center[a] = c.center[a];
   }
+  K.on = 1;
+  K.yshift = (int32_t)(height / s->d.dx + 0.5);
+  K.from_rot = c.velocity_from_rotated;
+  K.cos_t = std::cos(theta);
+  K.sin_t = std::sin(theta);
+  K.omega = sub >= 1 ? c.omega : 0.0;
+  return K;
 }
 
-// Dispatch over (dim, compute type) for the grid kernels.
-template <class F>
-static void dispatch_grid(const aq_sim *s, F &&f) {
-  if (s->d.storage == AQ_STORE_F64) {
-    if (s->d.dim == 3) f(Tag<3, 0, 0, 0, double>{});
-    else f(Tag<2, 0, 0, 0, double>{});
-  } else {
-    if (s->d.dim == 3) f(Tag<3, 0, 0, 0, float>{});
-    else f(Tag<2, 0, 0, 0, float>{});
-  }
+// gather grid in list mode: enough wavefronts for the blocks of a compact
+// scene, a loop over the list beyond that
+static unsigned lblocks(const aq_sim *s) {
+  const unsigned g = gblocks(s);
+  return g < 2048u ? g : 2048u;
 }
 
-#define TAG_TYPES(tag)                      \
-  using TG = decltype(tag);                 \
-  constexpr int DIM = TG::DIM;              \
-  constexpr int MAT = TG::MAT;              \
-  constexpr int ST = TG::ST;                \
-  constexpr int NW32 = TG::NW32;            \
-  using T = typename TG::T;
-
-template <class T> static const ParamsT<T> &params_of(const aq_sim *s);
-template <> const ParamsT<float> &params_of<float>(const aq_sim *s) { return s->pf; }
-template <> const ParamsT<double> &params_of<double>(const aq_sim *s) { return s->pd; }
-
-static unsigned pblocks(int64_t n) { return (unsigned)((n + 255) / 256); }
-static unsigned gblocks(const aq_sim *s) { return (unsigned)((s->nblocks + 3) / 4); }
-// persistent grid of the tiled kernels: a multiple of 8 (one share per XCD)
-static unsigned tblocks(const aq_sim *s, int64_t n) {
-  int64_t want = (n + kItem - 1) / kItem + 64;
-  if (want > 4096) want = 4096;
-  return (unsigned)((want + 7) / 8 * 8);
-}
-
-static int check_n(const aq_sim *s, const void *state, int64_t n) {
-  if (!s) return fail(AQ_ERR_INVALID, "null handle");
-  if (n < 0 || n > s->d.capacity) return fail(AQ_ERR_CAPACITY, "particle count exceeds capacity");
-  if (n && !state) return fail(AQ_ERR_INVALID, "null state");
-  return AQ_OK;
-}
-
-static bool items_valid(const aq_sim *s, const void *state, int64_t n) {
-  return s->have_items && s->items_state == state && s->items_n == n;
-}
-
-// Gather of the P2G result into grid_v (OP: with the grid update).
-static int launch_gather(aq_sim *s, bool tiled, bool op, hipStream_t st) {
+int launch_gather(aq_sim *s, bool tiled, bool op, uint32_t step, hipStream_t st) {
   const int32_t *ioff = tiled ? s->item_off : nullptr;
+  const ColliderK K = collider_at(s, step);
+  if (tiled) {
+    dispatch_grid(s, [&](auto tag_) {
+      TAG_TYPES(tag_)
+      (void)MAT;
+      (void)ST;
+      (void)NW32;
+      if (op)
+        hipLaunchKernelGGL((k_gather_list<DIM, T, true>), dim3(lblocks(s)), dim3(256), 0, st, params_of<T>(s), K,
+                           ioff, s->near, (const vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags,
+                           (vec4_t<T> *)s->grid_v, s->act, s->glist, (const int32_t *)s->flist, s->gcnt,
+                           s->listed);
+      else
+        hipLaunchKernelGGL((k_gather_list<DIM, T, false>), dim3(lblocks(s)), dim3(256), 0, st, params_of<T>(s), K,
+                           ioff, s->near, (const vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags,
+                           (vec4_t<T> *)s->grid_v, s->act, s->glist, (const int32_t *)s->flist, s->gcnt,
+                           s->listed);
+    });
+    hipLaunchKernelGGL(k_glist_commit, dim3(1), dim3(64), 0, st, s->gcnt);
+    LAUNCH_CHECK();
+    return AQ_OK;
+  }
   dispatch_grid(s, [&](auto tag_) {
     TAG_TYPES(tag_)
     (void)MAT;
     (void)ST;
     (void)NW32;
     if (op)
-      hipLaunchKernelGGL((k_gather<DIM, T, true>), dim3(gblocks(s)), dim3(256), 0, st, params_of<T>(s), s->nblocks,
+      hipLaunchKernelGGL((k_gather<DIM, T, true>), dim3(gblocks(s)), dim3(256), 0, st, params_of<T>(s), K, s->nblocks,
                          ioff, s->near, (const vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags,
                          (vec4_t<T> *)s->grid_v, s->act);
     else
-      hipLaunchKernelGGL((k_gather<DIM, T, false>), dim3(gblocks(s)), dim3(256), 0, st, params_of<T>(s), s->nblocks,
+      hipLaunchKernelGGL((k_gather<DIM, T, false>), dim3(gblocks(s)), dim3(256), 0, st, params_of<T>(s), K, s->nblocks,
                          ioff, s->near, (const vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags,
                          (vec4_t<T> *)s->grid_v, s->act);
-  });
-  LAUNCH_CHECK();
-  return AQ_OK;
-}
-
-// P2G of `state` (tiled when the last sort describes it); *tiled tells which.
-static int launch_p2g(aq_sim *s, const void *state, int64_t n, bool *tiled, hipStream_t st) {
-  *tiled = n > 0 && items_valid(s, state, n);
-  if (n == 0) return AQ_OK;
-  StateK sk = make_state(s, state, nullptr);
-  CodecK cod = make_codec(s, 0u);
-  dispatch(s, [&](auto tag_) {
-    TAG_TYPES(tag_)
-    if (*tiled)
-      hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, false, true>), dim3(tblocks(s, n)), dim3(kItem), 0, st, sk,
-                         cod, params_of<T>(s), n, s->items, s->item_off + s->nblocks, (const vec4_t<T> *)nullptr,
-                         (vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags);
-    else
-      hipLaunchKernelGGL((k_p2g<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, cod,
-                         params_of<T>(s), n, (vec4_t<T> *)s->grid_f, s->fflags);
-  });
-  LAUNCH_CHECK();
-  return AQ_OK;
-}
-
-static int launch_g2p(aq_sim *s, void *state, int64_t n, const int32_t *ids, uint32_t step, hipStream_t st) {
-  if (n == 0) return AQ_OK;
-  StateK sk = make_state(s, state, ids);
-  CodecK cod = make_codec(s, step);
-  const bool tiled = items_valid(s, state, n);
-  dispatch(s, [&](auto tag_) {
-    TAG_TYPES(tag_)
-    if (tiled)
-      hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, true, false>), dim3(tblocks(s, n)), dim3(kItem), 0, st, sk,
-                         cod, params_of<T>(s), n, s->items, s->item_off + s->nblocks,
-                         (const vec4_t<T> *)s->grid_v, (vec4_t<T> *)nullptr, (vec4_t<T> *)nullptr,
-                         (uint32_t *)nullptr);
-    else
-      hipLaunchKernelGGL((k_g2p<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, cod,
-                         params_of<T>(s), n, (const vec4_t<T> *)s->grid_v);
-  });
-  LAUNCH_CHECK();
-  return AQ_OK;
-}
-
-// G2P of substep `step` fused with the next P2G; *tiled tells which P2G ran.
-static int launch_g2p2g(aq_sim *s, void *state, int64_t n, const int32_t *ids, uint32_t step, bool *tiled,
-                        hipStream_t st) {
-  *tiled = n > 0 && items_valid(s, state, n);
-  if (n == 0) return AQ_OK;
-  StateK sk = make_state(s, state, ids);
-  CodecK cod = make_codec(s, step);
-  if (*tiled && s->jit_fn) {
-    ParamsT<float> P = s->pf;
-    const int4 *items = s->items;
-    const int32_t *nitems = s->item_off + s->nblocks;
-    const void *gv = s->grid_v;
-    void *tl = s->tiles, *gf = s->grid_f;
-    uint32_t *ff = s->fflags;
-    void *args[] = {&sk, &cod, &P, &n, &items, &nitems, &gv, &tl, &gf, &ff};
-    HIP_TRY(hipModuleLaunchKernel(s->jit_fn, tblocks(s, n), 1, 1, kItem, 1, 1, 0, st, args, nullptr));
-    return AQ_OK;
-  }
-  dispatch(s, [&](auto tag_) {
-    TAG_TYPES(tag_)
-    if (*tiled)
-      hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, true, true>), dim3(tblocks(s, n)), dim3(kItem), 0, st, sk,
-                         cod, params_of<T>(s), n, s->items, s->item_off + s->nblocks,
-                         (const vec4_t<T> *)s->grid_v, (vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags);
-    else
-      hipLaunchKernelGGL((k_g2p2g<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, cod,
-                         params_of<T>(s), n, (const vec4_t<T> *)s->grid_v, (vec4_t<T> *)s->grid_f, s->fflags);
   });
   LAUNCH_CHECK();
   return AQ_OK;
@@ -1642,18 +1348,19 @@ extern "C" int aq_p2g(aq_sim *s, const void *state, int64_t n, void *stream) {
   hipStream_t st = (hipStream_t)stream;
   bool tiled;
   if ((rc = launch_p2g(s, state, n, &tiled, st))) return rc;
-  return launch_gather(s, tiled, false, st);
+  return launch_gather(s, tiled, false, 0u, st);
 }
 
-extern "C" int aq_grid_op(aq_sim *s, void *stream) {
+extern "C" int aq_grid_op(aq_sim *s, uint32_t step, void *stream) {
   if (!s) return fail(AQ_ERR_INVALID, "null handle");
   hipStream_t st = (hipStream_t)stream;
+  const ColliderK K = collider_at(s, step);
   dispatch_grid(s, [&](auto tag_) {
     TAG_TYPES(tag_)
     (void)MAT;
     (void)ST;
     (void)NW32;
-    hipLaunchKernelGGL((k_grid_update<DIM, T>), dim3(gblocks(s)), dim3(256), 0, st, params_of<T>(s), s->nblocks,
+    hipLaunchKernelGGL((k_grid_update<DIM, T>), dim3(gblocks(s)), dim3(256), 0, st, params_of<T>(s), K, s->nblocks,
                        (vec4_t<T> *)s->grid_v, (const uint32_t *)s->act);
   });
   LAUNCH_CHECK();
@@ -1720,6 +1427,10 @@ extern "C" int aq_sort(aq_sim *s, void *state, int64_t n, int32_t *ids, void *st
     hipLaunchKernelGGL(k_near<3>, dim3((m + 255) / 256), dim3(256), 0, st, s->scount, (int32_t)s->pf.nb, s->near, m);
   else
     hipLaunchKernelGGL(k_near<2>, dim3((m + 255) / 256), dim3(256), 0, st, s->scount, (int32_t)s->pf.nb, s->near, m);
+  // the gather list of the new order (counters cleared: glist, flist, cursor)
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, s->gcnt, 0u, (int64_t)16);
+  hipLaunchKernelGGL(k_glist_build, dim3((m + 255) / 256), dim3(256), 0, st, (const uint32_t *)s->near,
+                     (const uint32_t *)s->act, s->listed, s->glist, s->gcnt, m);
   LAUNCH_CHECK();
   s->have_items = true;
   s->items_state = state;
@@ -1731,7 +1442,7 @@ extern "C" int aq_substep(aq_sim *s, void *state, int64_t n, int32_t *ids, uint3
   int rc;
   if ((rc = aq_sort(s, state, n, ids, stream))) return rc;
   if ((rc = aq_p2g(s, state, n, stream))) return rc;
-  if ((rc = aq_grid_op(s, stream))) return rc;
+  if ((rc = aq_grid_op(s, step, stream))) return rc;
   return aq_g2p(s, state, n, ids, step, stream);
 }
 
@@ -1814,7 +1525,7 @@ static int advance_impl(aq_sim *s, void *state, int64_t n, int nsteps, uint32_t 
   bool tiled;
   TIMED(tm, Timer::ENDS, rc = launch_p2g(s, state, n, &tiled, st));
   if (rc) return rc;
-  TIMED(tm, Timer::GRID, rc = launch_gather(s, tiled, true, st));
+  TIMED(tm, Timer::GRID, rc = launch_gather(s, tiled, true, step0, st));
   if (rc) return rc;
   for (int t = 0; t + 1 < nsteps; ++t) {
     if (sort_every > 0 && t > 0 && t % sort_every == 0) {
@@ -1824,7 +1535,7 @@ static int advance_impl(aq_sim *s, void *state, int64_t n, int nsteps, uint32_t 
     TIMED(tm, Timer::FUSED, rc = launch_g2p2g(s, state, n, ids, step0 + (uint32_t)t, &tiled, st));
     if (rc) return rc;
     if (mxmn && (rc = launch_limits(s, state, n, mxmn, st))) return rc;
-    TIMED(tm, Timer::GRID, rc = launch_gather(s, tiled, true, st));
+    TIMED(tm, Timer::GRID, rc = launch_gather(s, tiled, true, step0 + (uint32_t)t + 1u, st));
     if (rc) return rc;
   }
   TIMED(tm, Timer::ENDS, rc = launch_g2p(s, state, n, ids, step0 + (uint32_t)(nsteps - 1), st));
@@ -1988,6 +1699,8 @@ extern "C" int aq_noise_dot(aq_sim *s, const void *adj, int64_t n, const int32_t
   int rc = check_n(s, adj, n);
   if (rc) return rc;
   if (s->d.storage == AQ_STORE_QUANT) return fail(AQ_ERR_INVALID, "the adjoint runs on plain (f32/f64) storage");
+  if (s->has_collider || s->d.bound_clip)
+    return fail(AQ_ERR_INVALID, "the adjoint of the SDF collider and of the bound clip is not implemented");
   if (!out) return fail(AQ_ERR_INVALID, "null output");
   if (n == 0) return AQ_OK;
   const KeysK keys = make_keys(seed, step);
@@ -2002,6 +1715,25 @@ extern "C" int aq_noise_dot(aq_sim *s, const void *adj, int64_t n, const int32_t
   return AQ_OK;
 }
 
+extern "C" int aq_sim_set_collider(aq_sim *s, const aq_collider *c) {
+  if (!s) return fail(AQ_ERR_INVALID, "null handle");
+  if (!c) {
+    s->has_collider = false;
+    return AQ_OK;
+  }
+  if (s->d.dim != 3) return fail(AQ_ERR_INVALID, "the SDF collider is 3D");
+  if (!c->sdf || !c->normal) return fail(AQ_ERR_INVALID, "null collider volume");
+  for (int a = 0; a < 3; ++a)
+    if (c->size[a] < 1 || c->size[a] > 4096) return fail(AQ_ERR_INVALID, "collider volume size out of range");
+  if (c->lag < 0 || c->lag > 1) return fail(AQ_ERR_INVALID, "collider lag must be 0 or 1");
+  if (c->velocity_from_rotated != 0 && c->velocity_from_rotated != 1)
+    return fail(AQ_ERR_INVALID, "velocity_from_rotated must be 0 or 1");
+  if (!std::isfinite(c->omega) || !std::isfinite(c->height_rate)) return fail(AQ_ERR_INVALID, "non-finite collider motion");
+  s->col = *c;
+  s->has_collider = true;
+  return AQ_OK;
+}
+
 extern "C" int aq_sim_count_wraps(aq_sim *s, unsigned long long *counts) {
   if (!s) return fail(AQ_ERR_INVALID, "null handle");
   if (counts && s->d.storage != AQ_STORE_QUANT) return fail(AQ_ERR_INVALID, "only quantized records wrap");
@@ -2013,61 +1745,6 @@ extern "C" int aq_sim_grid(aq_sim *s, void **grid, int64_t *nodes) {
   if (!s || !grid || !nodes) return fail(AQ_ERR_INVALID, "null argument");
   *grid = s->grid_v;
   *nodes = s->nodes;
-  return AQ_OK;
-}
-
-// ---------------------------------------------------------------------------
-// adjoint of the plain substep
-// ---------------------------------------------------------------------------
-template <class T>
-__global__ void __launch_bounds__(256) k_zero_active(int32_t nblocks, const uint32_t *__restrict__ act,
-                                                     vec4_t<T> *__restrict__ g) {
-  const int32_t b = (int32_t)(blockIdx.x * 4u + (threadIdx.x >> 6));
-  if (b >= nblocks || !act[b]) return;
-  vec4_t<T> z;
-  z.x = z.y = z.z = z.w = (T)0;
-  g[(int64_t)b * 64 + (threadIdx.x & 63)] = z;
-}
-
-extern "C" int64_t aq_adj_workspace_bytes(const aq_sim *s) {
-  if (!s) return -1;
-  return 2 * s->nodes * s->node_bytes;
-}
-
-extern "C" int aq_adj_substep(aq_sim *s, const void *state, int64_t n, const void *adj_next, void *adj_out,
-                              double *gsum, void *ws, int64_t ws_bytes, void *stream) {
-  int rc = check_n(s, state, n);
-  if (rc) return rc;
-  if (s->d.storage == AQ_STORE_QUANT) return fail(AQ_ERR_INVALID, "the adjoint runs on plain (f32/f64) storage");
-  if (n && (!adj_next || !adj_out)) return fail(AQ_ERR_INVALID, "null adjoint buffer");
-  if (!ws || ws_bytes < aq_adj_workspace_bytes(s)) return fail(AQ_ERR_CAPACITY, "adjoint workspace too small");
-  if (n == 0) return AQ_OK;
-  hipStream_t st = (hipStream_t)stream;
-  void *pre = ws;
-  void *gbar = (char *)ws + s->nodes * s->node_bytes;
-  // forward recompute of s_t's grid: P2G (tiled when the last sort describes
-  // `state`), gather, copy of (m v, m), grid update
-  bool tiled;
-  if ((rc = launch_p2g(s, state, n, &tiled, st))) return rc;
-  if ((rc = launch_gather(s, tiled, false, st))) return rc;
-  HIP_TRY(hipMemcpyAsync(pre, s->grid_v, (size_t)(s->nodes * s->node_bytes), hipMemcpyDeviceToDevice, st));
-  if ((rc = aq_grid_op(s, stream))) return rc;
-  const int64_t cap = s->d.capacity;
-  dispatch(s, [&](auto tag_) {
-    TAG_TYPES(tag_)
-    (void)ST;
-    (void)NW32;
-    const ParamsT<T> &P = params_of<T>(s);
-    hipLaunchKernelGGL((k_zero_active<T>), dim3(gblocks(s)), dim3(256), 0, st, s->nblocks, (const uint32_t *)s->act,
-                       (vec4_t<T> *)gbar);
-    hipLaunchKernelGGL((k_adj_g2p<DIM, MAT, T>), dim3(pblocks(n)), dim3(256), 0, st, (const T *)state,
-                       (const T *)adj_next, (T *)adj_out, cap, n, P, (const vec4_t<T> *)s->grid_v, (T *)gbar);
-    hipLaunchKernelGGL((k_adj_grid<DIM, T>), dim3(gblocks(s)), dim3(256), 0, st, P, s->nblocks,
-                       (const uint32_t *)s->act, (const vec4_t<T> *)pre, (T *)gbar);
-    hipLaunchKernelGGL((k_adj_p2g<DIM, MAT, T>), dim3(pblocks(n)), dim3(256), 0, st, (const T *)state, (T *)adj_out,
-                       cap, n, P, (const T *)gbar, gsum);
-  });
-  LAUNCH_CHECK();
   return AQ_OK;
 }
 

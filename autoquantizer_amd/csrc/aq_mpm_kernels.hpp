@@ -30,7 +30,9 @@ template <class T> using vec4_t = typename Vec4<T>::type;
 
 template <class T> struct ParamsT {
   T dx, inv_dx, dt, p_mass, stress_k, la, two_mu, la_0, g_dt, c4, half_pm, g_pm, eps_m;
-  int32_t n, nb, bound, boundary;  // cells per axis, blocks per axis, wall, kind
+  T xlo[3], xhi[3];                 // bound clip: lb dx, ub dx per axis
+  int32_t n, nb, boundary, clip;    // cells per axis, blocks per axis, wall kind, bound clip on
+  int32_t lb[3], ub[3];             // walls per axis (cells)
 };
 
 template <int DIM> struct BlockShape;
@@ -207,6 +209,9 @@ template <> __device__ __forceinline__ double tmin<double>(double a, double b) {
 template <class T> __device__ __forceinline__ T tsqrt(T a);
 template <> __device__ __forceinline__ float tsqrt<float>(float a) { return sqrtf(a); }
 template <> __device__ __forceinline__ double tsqrt<double>(double a) { return sqrt(a); }
+template <class T> __device__ __forceinline__ T tlog(T a);
+template <> __device__ __forceinline__ float tlog<float>(float a) { return logf(a); }
+template <> __device__ __forceinline__ double tlog<double>(double a) { return log(a); }
 
 // base = floor(x * inv_dx - 0.5) clamped to [0, n-3] (NaN -> 0); fx; weights.
 template <class T>
@@ -267,8 +272,26 @@ __device__ __forceinline__ void affine_of(const T (&v)[NC], const ParamsT<T> &P,
     const T sig = P.la * (v[CI::CJ] - (T)1.0);
 #pragma unroll
     for (int i = 0; i < DIM; ++i) cauchy[i * DIM + i] = sig;
+  } else if constexpr (DIM == 3) {
+    // mu_0 (F F^T - I) + la_0 log(J), the scalar added to every entry as
+    // the reference's matrix + scalar broadcast does (mpm_solver.py:1027-1029);
+    // J by Matrix.determinant's cofactor expansion (TS/python/taichi/lang/matrix.py:625-628)
+    const T *F = &v[CI::CJ];
+    const T J = F[0] * (F[4] * F[8] - F[7] * F[5]) - F[3] * (F[1] * F[8] - F[7] * F[2]) +
+                F[6] * (F[1] * F[5] - F[4] * F[2]);
+    const T mu = P.two_mu * (T)0.5;
+    const T lg = P.la_0 * tlog<T>(J);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        T fft = F[i * 3 + 0] * F[j * 3 + 0];
+        fft = fft + F[i * 3 + 1] * F[j * 3 + 1];
+        fft = fft + F[i * 3 + 2] * F[j * 3 + 2];
+        cauchy[i * 3 + j] = mu * (fft - (i == j ? (T)1 : (T)0)) + lg;
+      }
   } else {
-    static_assert(DIM == 2, "fixed-corotated elasticity is 2D");
+    // 2D fixed-corotated
     const T F00 = v[CI::CJ + 0], F01 = v[CI::CJ + 1], F10 = v[CI::CJ + 2], F11 = v[CI::CJ + 3];
     const T J = F00 * F11 - F01 * F10;
     const T x = F00 + F11, y = F10 - F01;
@@ -396,6 +419,7 @@ __device__ __forceinline__ void g2p_apply(const T (&v)[NC], const ParamsT<T> &P,
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
     out[CI::CX + a] = v[CI::CX + a] + P.dt * nv[a];
+    if (P.clip) out[CI::CX + a] = tmax<T>(tmin<T>(out[CI::CX + a], P.xhi[a]), P.xlo[a]);
     out[CI::CV + a] = nv[a];
   }
 #pragma unroll
@@ -520,10 +544,19 @@ __device__ __forceinline__ void p2g_emit(const P2GTerms<DIM, T> &R, const Stenci
   }
 }
 
+// Where an untiled scatter reports its blocks: the per-block flags and,
+// when `list` is set, a list that gets each block once when its flag goes
+// from 0 to 1 (`count`: the list's length).
+struct FlagSink {
+  uint32_t *flags;
+  int32_t *list;
+  uint32_t *count;
+};
+
 // Flag every grid block that the stencil of `base` touches (the base block
 // and, per axis, the next one when the stencil crosses a block edge).
 template <int DIM>
-__device__ __forceinline__ void mark_stencil_blocks(const int (&base)[3], int nb, uint32_t *flags) {
+__device__ __forceinline__ void mark_stencil_blocks(const int (&base)[3], int nb, const FlagSink &fs) {
   constexpr int LG = BlockShape<DIM>::LOG;
   int lo[3], hi[3];
 #pragma unroll
@@ -533,14 +566,20 @@ __device__ __forceinline__ void mark_stencil_blocks(const int (&base)[3], int nb
   }
   for (int i = lo[0]; i <= hi[0]; ++i)
     for (int j = lo[1]; j <= hi[1]; ++j)
-      for (int k = lo[2]; k <= hi[2]; ++k)
-        flags[DIM == 3 ? (i * nb + j) * nb + k : i * nb + j] = 1u;
+      for (int k = lo[2]; k <= hi[2]; ++k) {
+        const int32_t b = DIM == 3 ? (i * nb + j) * nb + k : i * nb + j;
+        if (fs.list) {
+          if (atomicExch(&fs.flags[b], 1u) == 0u) fs.list[atomicAdd(fs.count, 1u)] = b;
+        } else {
+          fs.flags[b] = 1u;
+        }
+      }
 }
 
 // P2G of one particle with global atomics.
 template <int DIM, int MAT, class T, int NC>
 __device__ __forceinline__ void p2g_particle(const T (&v)[NC], const ParamsT<T> &P,
-                                             vec4_t<T> *__restrict__ grid, uint32_t *__restrict__ flags) {
+                                             vec4_t<T> *__restrict__ grid, const FlagSink &flags) {
   const Stencil<DIM, T> S = stencil_of<DIM, MAT, T>(v, P);
   int32_t o0[3], o1[3], o2[3] = {0, 0, 0};
   axis_offsets<DIM>(S.base[0], 0, P.nb, o0);

@@ -12,10 +12,8 @@ namespace aq {
 // at the last aq_sort: int4 (block, first particle, count, 0).  The item's
 // tile covers the block, the 2-node stencil reach and a 1-cell drift margin
 // on each side: TE = E + 4 nodes per axis from block_origin - 1 (LDS layout
-// in Tile; in the tile buffer, the core nodes grouped by the grid block they
-// belong to, then the rim nodes, make_tile_order).  The rim (the drift
-// margin) is written only when some particle of the item reached it: the
-// item's rim flags (one bit per face) follow its nodes in the buffer.
+// in Tile; in the tile buffer, TE^DIM nodes grouped by the grid block they
+// belong to, Tile::store_pos).
 //
 // G2P reads grid velocities staged in LDS.  P2G accumulates in LDS in fixed
 // point (integer LDS atomics, exact and so independent of the order): per
@@ -38,15 +36,7 @@ template <int DIM> struct Tile {
   static constexpr int E = BlockShape<DIM>::E, LG = BlockShape<DIM>::LOG;
   static constexpr int TE = E + 4;                                // nodes per axis
   static constexpr int TN = DIM == 3 ? TE * TE * TE : TE * TE;    // nodes
-  // Per axis the tile covers the last node of the previous block (t = 0),
-  // the E nodes of its own block (1..E), the first 3 of the next (E+1..E+3).
-  // A particle sorted into the block has its stencil base in the block, so
-  // its stencil covers t = 1..E+2: the core.  Only a particle that drifted a
-  // cell since the sort reaches t = 0 or t = E+3: the rim.
-  static constexpr int NCORE = DIM == 3 ? (E + 2) * (E + 2) * (E + 2) : (E + 2) * (E + 2);
-  // Nodes per item slot of the tile buffer: the TN tile nodes, then the
-  // item's rim flags (in node TN), padded to 128 bytes.
-  static constexpr int STRIDE = TN + 8;
+  static constexpr int STRIDE = TN;                               // nodes per item in the tile buffer
   // LDS slot of tile node (x, y, z): x SX + y SY + z.  SX = 112 = 48 mod 64,
   // SY = 12 (3D) / SX = 72 = 8 mod 64 (2D): the nodes of any 4x4x4 (8x8)
   // window get 64 distinct banks, and stencil offsets are immediate offsets.
@@ -67,84 +57,56 @@ template <int DIM> struct Tile {
       t[2] = 0;
     }
   }
-  // Rim faces of tile node t: bit 2a when t_a = 0, bit 2a+1 when t_a = E+3.
-  // The node can be non-zero only if the item's particles reached every
-  // one of these faces.
-  __host__ __device__ static constexpr uint32_t need(int x, int y, int z) {
+  // Position of tile node (x, y, z) in the tile buffer.  Per axis the tile
+  // covers the last node of the previous block (t = 0), the E nodes of its
+  // own block (1..E) and the first 3 of the next (E+1..E+3); the buffer
+  // groups the nodes by the block they belong to (3^DIM groups, in group
+  // order) and stores each group row-major, which is the gathering
+  // wavefront's lane order: a wavefront reads each covering tile as one
+  // contiguous run.
+  __host__ __device__ static constexpr int store_pos(int x, int y, int z) {
+    int g[3] = {0, 0, 0}, u[3] = {0, 0, 0}, sz[3] = {0, 0, 0};
     const int t[3] = {x, y, z};
-    uint32_t m = 0u;
-    for (int a = 0; a < DIM; ++a) {
-      if (t[a] == 0) m |= 1u << (2 * a);
-      if (t[a] == E + 3) m |= 2u << (2 * a);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      g[a] = t[a] == 0 ? 0 : (t[a] <= E ? 1 : 2);
+      u[a] = t[a] == 0 ? 0 : (t[a] <= E ? t[a] - 1 : t[a] - E - 1);
+      sz[a] = g[a] == 0 ? 1 : (g[a] == 1 ? E : 3);
     }
-    return m;
-  }
-  // Position of core node (x, y, z) in its item's slot of the tile buffer.
-  // Core nodes come first (positions 0..NCORE-1): per axis the core splits
-  // into the E nodes of the own block and the 2 of the next, and the core
-  // is stored as these 2^DIM groups (in group order), each row-major, which
-  // is the gathering wavefront's lane order: a wavefront reads each covering
-  // tile's core as one contiguous run.  Rim nodes follow (make_tile_order).
-  __host__ __device__ static constexpr int core_pos(int x, int y, int z) {
-    const int t[3] = {x, y, z};
-    int g[3] = {0, 0, 0}, u[3] = {0, 0, 0}, sz[3] = {1, 1, 1};
-    for (int a = 0; a < DIM; ++a) {
-      g[a] = t[a] <= E ? 0 : 1;
-      u[a] = t[a] <= E ? t[a] - 1 : t[a] - E - 1;
-      sz[a] = g[a] == 0 ? E : 2;
-    }
-    const int cum[2] = {0, E};  // core nodes per axis before group g
-    constexpr int C = E + 2;
+    // nodes of the groups before (g0, g1, g2): the slab below g0, the rows
+    // of slab g0 below g1, the groups of row (g0, g1) below g2
+    const int cum[3] = {0, 1, E + 1};  // nodes per axis before group g
     if constexpr (DIM == 3) {
-      const int before = cum[g[0]] * C * C + sz[0] * cum[g[1]] * C + sz[0] * sz[1] * cum[g[2]];
+      const int before = cum[g[0]] * TE * TE + sz[0] * cum[g[1]] * TE + sz[0] * sz[1] * cum[g[2]];
       return before + (u[0] * sz[1] + u[1]) * sz[2] + u[2];
     } else {
-      const int before = cum[g[0]] * C + sz[0] * cum[g[1]];
+      const int before = cum[g[0]] * TE + sz[0] * cum[g[1]];
       return before + u[0] * sz[1] + u[1];
     }
   }
 };
 
-// Per position of an item's slot in the tile buffer (core nodes at
-// Tile::core_pos, then the rim nodes in row-major order): the LDS slot of its node in bits 0..9 and its rim faces
-// (Tile::need) in bits 10..15; and per tile node (x TE + y) TE + z its
-// position.  Built at compile time: the write-out walks the slot in order,
-// so each wavefront stores a contiguous run, and the gather looks the
-// positions up.
+// LDS slot of the node at each position of the tile buffer (the inverse of
+// Tile::store_pos), built at compile time: the write-out walks the buffer
+// in order, so each wavefront stores a contiguous run.
 template <int DIM> struct TileOrder {
-  uint16_t at[Tile<DIM>::TN];
-  uint16_t pos[Tile<DIM>::TN];
+  uint16_t slot[Tile<DIM>::TN];
 };
 template <int DIM> constexpr TileOrder<DIM> make_tile_order() {
   using TL = Tile<DIM>;
-  static_assert(TL::TS <= 1024, "LDS slots fit 10 bits");
   TileOrder<DIM> o{};
-  int rim = TL::NCORE;  // rim nodes in row-major order after the core
   for (int x = 0; x < TL::TE; ++x)
     for (int y = 0; y < TL::TE; ++y)
-      for (int z = 0; z < (DIM == 3 ? TL::TE : 1); ++z) {
-        const uint32_t need = TL::need(x, y, z);
-        const int q = need ? rim++ : TL::core_pos(x, y, z);
-        o.at[q] = (uint16_t)(TL::slot(x, y, z) | (need << 10));
-        o.pos[DIM == 3 ? (x * TL::TE + y) * TL::TE + z : x * TL::TE + y] = (uint16_t)q;
-      }
+      for (int z = 0; z < (DIM == 3 ? TL::TE : 1); ++z) o.slot[TL::store_pos(x, y, z)] = (uint16_t)TL::slot(x, y, z);
   return o;
 }
 __constant__ constexpr TileOrder<3> kTileOrder3 = make_tile_order<3>();
 __constant__ constexpr TileOrder<2> kTileOrder2 = make_tile_order<2>();
-// LDS slot and rim faces of the node at position `pos` of a slot.
 template <int DIM> __device__ __forceinline__ int tile_slot_at(int pos) {
   if constexpr (DIM == 3)
-    return kTileOrder3.at[pos];
+    return kTileOrder3.slot[pos];
   else
-    return kTileOrder2.at[pos];
-}
-// Position of tile node (x, y, z) in its slot.
-template <int DIM> __device__ __forceinline__ int tile_pos(int x, int y, int z) {
-  if constexpr (DIM == 3)
-    return kTileOrder3.pos[(x * Tile<3>::TE + y) * Tile<3>::TE + z];
-  else
-    return kTileOrder2.pos[x * Tile<2>::TE + y];
+    return kTileOrder2.slot[pos];
 }
 
 template <int DIM>
@@ -266,17 +228,70 @@ __device__ __forceinline__ void tile_copy(const ParamsT<T> &P, const int (&o)[3]
   }
 }
 
-// A workgroup walks its items; per item three barriers: the tile staged,
-// the bounds of the fixed-point scales, the scatter complete.  (Loading the
-// next item during the current item's scatter saves a barrier but holds
-// ~20 more VGPRs: 2 % faster for the specialised quantized kernel with the
-// particles of a block in random order, 1 % slower with the layered order;
-// the fp32-storage kernel lost a wave per SIMD.)
-template <int DIM, int MAT, int ST, int NW32, class T, bool DO_G2P, bool DO_P2G>
-__device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
-                                           const int4 *__restrict__ items, const int32_t *__restrict__ nitems,
-                                           const vec4_t<T> *__restrict__ grid_v, vec4_t<T> *__restrict__ tiles,
-                                           vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags) {
+// Everything the tiled kernels need of one simulation.
+template <class T> struct TiledArgs {
+  StateK s;
+  CodecK cod;
+  ParamsT<T> P;
+  int64_t n;
+  const int4 *items;
+  const vec4_t<T> *grid_v;
+  vec4_t<T> *tiles;
+  vec4_t<T> *grid_f;
+  uint32_t *fflags;
+  int32_t *flist;  // blocks newly flagged by untiled scatters (null: flags only)
+  uint32_t *fcnt;
+};
+
+// Work of one simulation: its `count` items.
+template <class T> struct OneSim {
+  TiledArgs<T> a;
+  int count;
+  __device__ __forceinline__ int total() const { return count; }
+  __device__ __forceinline__ const TiledArgs<T> &args(int it, int &li) const {
+    li = it;
+    return a;
+  }
+  __device__ __forceinline__ uint32_t key_base(uint32_t kb) const { return kb; }
+};
+
+// Work of a group of simulations: the items of all of them, numbered
+// simulation after simulation (prefix[s] = items before simulation s,
+// prefix[nsims] = all).  An item's simulation is found by binary search
+// over the prefix; item numbers are uniform over a workgroup, so the search
+// and the descriptor loads are scalar.  The descriptors' codec key is the
+// seed's key; the dithering key of substep `step` is hash32(step ^ it), as
+// for a single simulation.
+template <class T> struct GroupSims {
+  const TiledArgs<T> *__restrict__ descs;
+  const int32_t *__restrict__ prefix;
+  int32_t nsims;
+  uint32_t step;
+  __device__ __forceinline__ uint32_t key_base(uint32_t seed_key) const { return hash32(step ^ seed_key); }
+  __device__ __forceinline__ int total() const { return prefix[nsims]; }
+  __device__ __forceinline__ const TiledArgs<T> &args(int it, int &li) const {
+    int lo = 0, hi = nsims;  // prefix[lo] <= it < prefix[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (prefix[mid] <= it)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    li = it - prefix[lo];
+    return descs[lo];
+  }
+};
+
+// A workgroup walks its items (of one simulation, or of a group of them,
+// `Src`); per item three barriers: the tile staged, the bounds of the
+// fixed-point scales, the scatter complete.  (Loading the next item during
+// the current item's scatter saves a barrier but holds ~20 more VGPRs: 2 %
+// faster for the specialised quantized kernel with the particles of a block
+// in random order, 1 % slower with the layered order; the fp32-storage
+// kernel lost a wave per SIMD.)
+template <int DIM, int MAT, int ST, int NW32, class T, bool DO_G2P, bool DO_P2G, class Src>
+__device__ __forceinline__ void tiled_run(const Src &src) {
   using TL = Tile<DIM>;
   using ACC = AccOf<T>;
   using U = typename ACC::U;
@@ -286,20 +301,11 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
   __shared__ unsigned long long ts[DO_P2G ? NP : 1][DO_P2G ? TS : 1];
   constexpr int NWAVES = kItem / 64;
   __shared__ T bpart[NWAVES][4];
-  // rim faces reached by the item's particles, two buffers by item parity
-  // (one is cleared while the other collects)
-  __shared__ uint32_t rimfl[2];
   __shared__ uint32_t rows[kItem * RowsOf<ST, NW32>::RS];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-#ifdef AQ_JIT_LAYOUT
-  constexpr uint32_t g2p_mask = jit::kG2PMask;
-#else
-  const uint32_t g2p_mask = s.g2p_mask;
-#endif
   constexpr bool G2P_ONLY_LOAD = DO_G2P;  // G2P needs only the words of x and J/F
-  const uint32_t load_mask = DO_G2P ? g2p_mask : 0xFFFFFFFFu;
   uint32_t *row = rows + tid * RowsOf<ST, NW32>::RS;
-  const int ni = *nitems;
+  const int ni = src.total();
   // each XCD (blocks dealt round-robin to 8 XCDs) walks a contiguous item range
   const int xcd = (int)(blockIdx.x & 7u);
   const int per = (int)(gridDim.x >> 3);
@@ -316,10 +322,25 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
     for (int q = tid; q < TS; q += kItem)
 #pragma unroll
       for (int a = 0; a < NP; ++a) ts[a][q] = 0ull;  // cleared before the first barrier
-    if (tid < 2) rimfl[tid] = 0u;
   }
-  for (uint32_t par = 0u; it < hi; it += per, par ^= 1u) {
-    const int4 item = items[it];
+  for (; it < hi; it += per) {
+    int li;  // the item's number in its simulation
+    const TiledArgs<T> &A = src.args(it, li);
+    const StateK &s = A.s;
+    CodecK cod = A.cod;
+    cod.key_base = src.key_base(cod.key_base);
+    const ParamsT<T> &P = A.P;
+    const int64_t n = A.n;
+    const vec4_t<T> *__restrict__ grid_v = A.grid_v;
+    vec4_t<T> *__restrict__ grid_f = A.grid_f;
+    uint32_t *__restrict__ fflags = A.fflags;
+#ifdef AQ_JIT_LAYOUT
+    constexpr uint32_t g2p_mask = jit::kG2PMask;
+#else
+    const uint32_t g2p_mask = s.g2p_mask;
+#endif
+    const uint32_t load_mask = DO_G2P ? g2p_mask : 0xFFFFFFFFu;
+    const int4 item = A.items[li];
     int o[3];
     tile_origin<DIM, T>(item, P.nb, o);
     // the particle's loads go out before the tile's, so their latencies overlap
@@ -404,12 +425,6 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
             for (int b = 0; b < DIM; ++b) R.A[a][b] = tldexp<T>(R.A[a][b], k[a]);
           }
           R.pm = tldexp<T>(R.pm, k[3]);
-          // faces of the tile's rim this stencil reaches (it covers rel..rel+2)
-          uint32_t fl = 0u;
-#pragma unroll
-          for (int a = 0; a < DIM; ++a)
-            fl |= (rel[a] == 0 ? 1u : 0u) << (2 * a) | (rel[a] == TE - 3 ? 2u : 0u) << (2 * a);
-          if (fl) atomicOr(&rimfl[par], fl);
           p2g_emit<DIM, T>(R, S, [&](int i, int j, int kk, const T (&mom)[3], T m) {
             const int q = TL::slot(rel[0] + i, rel[1] + j, rel[2] + kk);
             if constexpr (NP == 2) {
@@ -422,23 +437,15 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
             }
           });
         } else {
-          p2g_particle<DIM, MAT, T>(out, P, grid_f, fflags);
+          p2g_particle<DIM, MAT, T>(out, P, grid_f, FlagSink{fflags, A.flist, A.fcnt});
         }
       }
     }
     if constexpr (DO_P2G) {
       __syncthreads();  // tile sums complete
-      const uint32_t fl = rimfl[par];
-      if (tid == 0) rimfl[par ^ 1u] = 0u;  // read by every thread before the previous barrier
-      vec4_t<T> *dst = tiles + (int64_t)it * TL::STRIDE;
-      if (tid == 0) *(uint32_t *)(dst + TN) = fl;
-      // the core always; rim nodes only when the item reached their faces
-      // (untouched rim nodes are zero in LDS and need no clearing)
-      const int lim = fl ? TN : TL::NCORE;
-      for (int nd = tid; nd < lim; nd += kItem) {  // nd: position in the buffer
-        const uint32_t sn = wsl[nd];
-        const int q = (int)(sn & 1023u);
-        const uint32_t need = sn >> 10;
+      vec4_t<T> *dst = A.tiles + (int64_t)li * TN;
+      for (int nd = tid; nd < TN; nd += kItem) {  // nd: position in the buffer
+        const int q = wsl[nd];
         vec4_t<T> val;
         if constexpr (NP == 2) {
           int32_t mx, m, my, mz;
@@ -454,7 +461,7 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
           val.z = DIM == 3 ? ACC::unfix(ts[2][q], k[2]) : (T)0;
           val.w = ACC::unfix(ts[3][q], k[3]);
         }
-        if ((fl & need) == need) dst[nd] = val;
+        dst[nd] = val;
 #pragma unroll
         for (int a = 0; a < NP; ++a) ts[a][q] = 0ull;
       }
@@ -462,12 +469,34 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
   }
 }
 
+// One simulation's items (the count is the last sort's, read on the device).
+template <int DIM, int MAT, int ST, int NW32, class T, bool DO_G2P, bool DO_P2G>
+__device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, int64_t n,
+                                           const int4 *__restrict__ items, const int32_t *__restrict__ nitems,
+                                           const vec4_t<T> *__restrict__ grid_v, vec4_t<T> *__restrict__ tiles,
+                                           vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags,
+                                           int32_t *flist, uint32_t *fcnt) {
+  const OneSim<T> src{{s, cod, P, n, items, grid_v, tiles, grid_f, fflags, flist, fcnt}, *nitems};
+  tiled_run<DIM, MAT, ST, NW32, T, DO_G2P, DO_P2G>(src);
+}
+
 template <int DIM, int MAT, int ST, int NW32, class T, bool DO_G2P, bool DO_P2G>
 __global__ void __launch_bounds__(kItem) __attribute__((amdgpu_waves_per_eu(TiledWaves<ST>::W)))
 k_tiled(StateK s, CodecK cod, ParamsT<T> P, int64_t n, const int4 *__restrict__ items,
         const int32_t *__restrict__ nitems, const vec4_t<T> *__restrict__ grid_v, vec4_t<T> *__restrict__ tiles,
-        vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags) {
-  tiled_body<DIM, MAT, ST, NW32, T, DO_G2P, DO_P2G>(s, cod, P, n, items, nitems, grid_v, tiles, grid_f, fflags);
+        vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags, int32_t *flist, uint32_t *fcnt) {
+  tiled_body<DIM, MAT, ST, NW32, T, DO_G2P, DO_P2G>(s, cod, P, n, items, nitems, grid_v, tiles, grid_f, fflags,
+                                                    flist, fcnt);
+}
+
+// The fused G2P -> P2G substep of a group of quantized simulations (one
+// dimension and material, any record layouts) in one launch.
+template <int DIM, int MAT, int NW32>
+__global__ void __launch_bounds__(kItem) __attribute__((amdgpu_waves_per_eu(TiledWaves<AQ_STORE_QUANT>::W)))
+k_tiled_group(const TiledArgs<float> *__restrict__ descs, const int32_t *__restrict__ prefix, int32_t nsims,
+              uint32_t step) {
+  const GroupSims<float> src{descs, prefix, nsims, step};
+  tiled_run<DIM, MAT, AQ_STORE_QUANT, NW32, float, true, true>(src);
 }
 
 }  // namespace aq

@@ -13,6 +13,7 @@ state checkpoints, plus exact record checkpoints.
   data_{frame:06d}.ply (positions) and v.npy [N, d], C.npy [N, d, d],
   J.npy [N] or F.npy [N, d, d] (float32), and read them back, re-encoding
   into the solver's records as the reference's push_* kernels store them.
+- load_mesh: triangle meshes for the voxelizer (mesh_io.py:14-33).
 - save_records / load_records: the solver's raw records (word planes),
   particle ids, step counter and a layout fingerprint in one npz, restored
   bit for bit (no counterpart in the reference).
@@ -73,6 +74,57 @@ def read_ply_vertices(fn):
             raise ValueError(f"{fn}: only binary_little_endian vertex data is read")
         data = np.frombuffer(f.read(np.dtype(props).itemsize * n), dtype=np.dtype(props), count=n)
     return data
+
+
+def load_mesh(fn, scale=1, offset=(0, 0, 0)):
+    """Triangles [T, 9] (float32; vertex a, b, c) of a binary little-endian
+    PLY mesh, each vertex coordinate scaled and offset per axis
+    (mesh_io.py:14-33).  Reads a vertex element of scalar properties and a
+    face element whose only property is a list of vertex indices (uchar
+    count, then 3 integer indices per face: triangles only)."""
+    if isinstance(scale, (int, float)):
+        scale = (scale, scale, scale)
+    with open(fn, "rb") as f:
+        if f.readline().strip() != b"ply":
+            raise ValueError(f"{fn}: not a PLY file")
+        elems, fmt = [], None
+        while True:
+            line = f.readline()
+            if not line:
+                raise ValueError(f"{fn}: no end_header")
+            tok = line.decode("ascii").split()
+            if not tok or tok[0] in ("comment", "obj_info"):
+                continue
+            if tok[0] == "format":
+                fmt = tok[1]
+            elif tok[0] == "element":
+                elems.append([tok[1], int(tok[2]), []])
+            elif tok[0] == "property":
+                elems[-1][2].append(tok[1:])
+            elif tok[0] == "end_header":
+                break
+        if fmt != "binary_little_endian":
+            raise ValueError(f"{fn}: only binary_little_endian meshes are read")
+        verts = faces = None
+        for name, count, props in elems:
+            if name == "vertex":
+                dt = np.dtype([(pr[1], _PLY_TYPES[pr[0]]) for pr in props])
+                verts = np.frombuffer(f.read(dt.itemsize * count), dtype=dt, count=count)
+            elif name == "face":
+                if len(props) != 1 or props[0][0] != "list":
+                    raise ValueError(f"{fn}: faces must hold one vertex-index list")
+                cnt_t, idx_t = _PLY_TYPES[props[0][1]], _PLY_TYPES[props[0][2]]
+                dt = np.dtype([("n", cnt_t), ("i", idx_t, (3,))])
+                faces = np.frombuffer(f.read(dt.itemsize * count), dtype=dt, count=count)
+                if np.any(faces["n"] != 3):
+                    raise ValueError(f"{fn}: only triangles are read")
+            else:
+                raise ValueError(f"{fn}: element {name} is not read")
+    if verts is None or faces is None:
+        raise ValueError(f"{fn}: vertex and face elements needed")
+    xyz = np.stack([verts["x"], verts["y"], verts["z"]], axis=1).astype(np.float32)
+    xyz = xyz * np.asarray(scale, dtype=np.float32) + np.asarray(offset, dtype=np.float32)
+    return xyz[faces["i"].astype(np.int64)].reshape(-1, 9)
 
 
 def load_particles_only(fn):

@@ -87,11 +87,11 @@ def fluid_3d_error_bounds(nsteps, n_grid, v_range=8.0):
     return {"x": qx * r, "v": qv, "C": n_grid * qv / 4.0, "J": qj * r}
 
 
-def fluid_3d(dither=True, v_range=8.0):
+def fluid_3d(dither=True, v_range=8.0, x_range=1.0):
     names = component_names(3, True)
     t = {}
     for a in range(3):
-        t[f"x{a}"] = quant.fixed(21, signed=True, range=1.0, use_dithering=dither)
+        t[f"x{a}"] = quant.fixed(21, signed=True, range=x_range, use_dithering=dither)
         t[f"v{a}"] = quant.fixed(16, signed=True, range=v_range, use_dithering=dither)
         for b in range(3):
             t[f"C{a}{b}"] = quant.float(exp=8, frac=10)

@@ -167,9 +167,11 @@ class MPMSolver:
         self.fluid = bool(cfg.use_fluid)
         self.n_particles = int(cfg.n_particles)
         self.capacity = int(capacity or self.n_particles)
-        self.n_grid = int(cfg.n_grid)
-        self.dx = 1 / self.n_grid
-        self.inv_dx = float(self.n_grid)
+        # dx = 1 / n_grid; the grid spans `size` (MPMConfig.size) in every
+        # axis, n_grid * size cells
+        self.dx = 1 / int(cfg.n_grid)
+        self.inv_dx = float(int(cfg.n_grid))
+        self.n_grid = int(round(int(cfg.n_grid) * max(float(getattr(cfg, "size", 1.0)), 1.0)))
         self.dt = cfg.dt
         self.p_vol = cfg.p_vol
         self.p_mass = cfg.p_vol * cfg.p_rho
@@ -199,6 +201,11 @@ class MPMSolver:
             self.storage = _lib.AQ_STORE_F64 if f64 else _lib.AQ_STORE_F32
         bound = cfg.bound if cfg.bound is not None else [3] * self.dim
         self.bound = int(bound[0])
+        # per-axis walls (mpm_solver.py:97-105): lb = bound, ub = cells - bound unless given
+        lb = cfg.lb if getattr(cfg, "lb", None) is not None else [int(b) for b in bound]
+        ub = cfg.ub if getattr(cfg, "ub", None) is not None else [self.n_grid - int(b) for b in bound]
+        self.lb, self.ub = [int(v) for v in lb][:self.dim], [int(v) for v in ub][:self.dim]
+        self.bound_clip = bool(getattr(cfg, "use_bound_clip", False))
 
     # ---- handle -----------------------------------------------------------
     def _build_desc(self, cfg):
@@ -215,6 +222,11 @@ class MPMSolver:
         d.p_mass, d.gravity = self.p_mass, cfg.g
         d.la, d.mu_0, d.la_0 = cfg.la, self.mu_0, self.la_0
         d.seed = int(getattr(cfg, "seed", 0))
+        if self.lb != [self.bound] * self.dim or self.ub != [self.n_grid - self.bound] * self.dim:
+            d.axis_bounds = 1
+            for a in range(self.dim):
+                d.lb[a], d.ub[a] = self.lb[a], self.ub[a]
+        d.bound_clip = 1 if self.bound_clip else 0
         for k, c in enumerate(self.comps):
             for key, val in c.items():
                 setattr(d.comps[k], key, val)
@@ -261,7 +273,8 @@ class MPMSolver:
         """vals: [ncomp, n] float array (canonical component order).  With
         quantized storage the values are encoded with the dithering noise of
         step `dither_step`."""
-        vals = np.asarray(vals, dtype=np.float64)
+        if not (isinstance(vals, np.ndarray) and vals.dtype == np.float32 and self.storage != _lib.AQ_STORE_F64):
+            vals = np.asarray(vals, dtype=np.float64)  # float32 input goes to the f32 paths unconverted
         n = vals.shape[1] if n is None else n
         if n > self.capacity:
             raise ValueError("more particles than capacity")
@@ -342,6 +355,113 @@ class MPMSolver:
         allv = np.concatenate(self._pending, axis=1)
         self.set_state(allv)
 
+    def emit(self, vals, dither_step=None):
+        """Append particles ([ncomp, k] values, canonical component order) to
+        the running state, as the reference's emitters write new particles
+        after current_idx (fill_in_ball_with_vel / emit_water,
+        mpm_solver.py:1598-1632; emit_letters_one_frame, :131-134).  Particles
+        already there keep their records and order; the new ones get ids
+        n .. n + k - 1 and, with quantized storage, are encoded with the
+        dithering noise of `dither_step` (default: the current step with the
+        top bit set, apart from the substeps' own noise)."""
+        vals = np.asarray(vals, dtype=np.float64)
+        k = vals.shape[1]
+        n0 = self.current_idx
+        if n0 + k > self.capacity:
+            raise ValueError("more particles than capacity")
+        if self.storage == _lib.AQ_STORE_QUANT:
+            step = (self.step_count | 0x80000000) if dither_step is None else int(dither_step)
+            v = torch.as_tensor(np.ascontiguousarray(vals, dtype=np.float32), device=self.device)
+            tmp = torch.zeros((self.nwords, k), dtype=torch.int64, device=self.device)
+            _lib.check(_lib.lib().aq_pack(self._carr, self.ncomp, self.nwords, _ptr(v), None,
+                                          self._desc.seed, step & 0xFFFFFFFF, _ptr(tmp), k, _stream()))
+            self.state[:, n0:n0 + k] = tmp
+        else:
+            self.state[:, n0:n0 + k] = torch.as_tensor(vals, dtype=self.state.dtype, device=self.device)
+        self.ids[n0:n0 + k] = torch.arange(n0, n0 + k, dtype=torch.int32, device=self.device)
+        self.current_idx = n0 + k
+        torch.cuda.current_stream().synchronize()
+
+    def add_mesh(self, triangles, sample_density=None, velocity=None, translation=None, seed=0):
+        """add_mesh (mpm_solver.py:1356-1380): the mesh voxelized on the
+        solver's voxelizer (Voxelizer(res = (n_grid,)^3, dx, super_sample 2),
+        :115-116), sample_density + 1 particles seeded per inside voxel in
+        solver cells (seed_from_voxels, :1414-1441), appended with
+        `velocity`, C = 0 and F = I (J = 1).  Returns the particle count."""
+        from .voxelizer import Voxelizer, seed_from_voxels
+        if self.dim != 3:
+            raise ValueError("add_mesh is 3D")
+        if sample_density is None:
+            sample_density = 2 ** self.dim
+        if getattr(self, "_voxelizer", None) is None:
+            self._voxelizer = Voxelizer((int(self.param.n_grid),) * 3, self.dx, super_sample=2)
+        vox = self._voxelizer.voxelize(triangles)
+        x, v = seed_from_voxels(vox, sample_density, self.dx, translation if translation is not None else (0, 0, 0),
+                                velocity if velocity is not None else (0, 0, 0), np.random.default_rng(seed))
+        d = self.dim
+        vals = np.zeros((self.ncomp, x.shape[0]))
+        vals[:d] = x.T
+        vals[d:2 * d] = v.T
+        if self.fluid:
+            vals[-1] = 1.0
+        else:
+            vals[-d * d:] = np.eye(d).reshape(-1, 1)
+        self.emit(vals)
+        return x.shape[0]
+
+    def emit_water(self, center, radius, velocity, num, seed=0):
+        """emit_water (mpm_solver.py:1627-1630): num particles at center + r
+        (cos t, sin t, 0) * radius with t and r uniform in [0, 2 pi) and
+        [0, 1) (random_point_in_unit_sphere_in_xy, :1606-1616), velocity
+        `velocity`, J = 1 (fluid) or F = I; seeded numpy draws."""
+        d = self.dim
+        rng = np.random.default_rng(seed)
+        t = rng.random(num) * np.pi * 2.0
+        r = rng.random(num)
+        vals = np.zeros((self.ncomp, num))
+        off = np.zeros((d, num))
+        off[0], off[1] = np.cos(t) * r, np.sin(t) * r
+        vals[:d] = np.asarray(center, dtype=np.float64).reshape(d, 1) + off * radius
+        vals[d:2 * d] = np.asarray(velocity, dtype=np.float64).reshape(d, 1)
+        if self.fluid:
+            vals[-1] = 1.0
+        else:
+            vals[-d * d:] = np.eye(d).reshape(-1, 1)
+        self.emit(vals)
+
+    def set_collider(self, sdf, normal, offset=(0, 0, 0), center=None, omega=5.0, height_rate=0.15, lag=1,
+                     velocity_from_rotated=False, first_step=None):
+        """The SDF collider of the 3D fluid scenes (init_sdf_collider /
+        init_sdf_value, mpm_solver.py:284-319; sdf_collision(_no_arg),
+        rot_scene, step_collider(_difftaichi), :1444-1559): sdf [sx, sy, sz]
+        and its gradient normal [sx, sy, sz, 3] at node offset `offset`,
+        rotating about the vertical line through `center` (default
+        get_center_idx: (n_grid, n_grid // 2, n_grid) of MPMConfig.n_grid)
+        at omega, rising at height_rate; lag 1 / rate 0.15 is the range and
+        AD runs' schedule, lag 0 / rate 0.1 the forward runs'.  The
+        collider's substep 0 is the current step unless first_step is given.
+        None removes it."""
+        if sdf is None:
+            self._collider = None
+            _lib.check(_lib.lib().aq_sim_set_collider(self._h, None))
+            return
+        sdf = np.ascontiguousarray(sdf, dtype=np.float32)
+        normal = np.ascontiguousarray(normal, dtype=np.float32).reshape(sdf.shape + (3,))
+        ng = int(self.param.n_grid)
+        if center is None:
+            center = (ng * 2 // 2, ng // 2, ng * 2 // 2)
+        sdf_t = torch.as_tensor(sdf, device=self.device)
+        nrm_t = torch.as_tensor(normal, device=self.device)
+        c = _lib.Collider()
+        c.sdf, c.normal = sdf_t.data_ptr(), nrm_t.data_ptr()
+        for a in range(3):
+            c.size[a], c.offset[a], c.center[a] = sdf.shape[a], int(offset[a]), int(center[a])
+        c.lag, c.velocity_from_rotated = int(lag), 1 if velocity_from_rotated else 0
+        c.first_step = int(self.step_count if first_step is None else first_step) & 0xFFFFFFFF
+        c.omega, c.height_rate = float(omega), float(height_rate)
+        _lib.check(_lib.lib().aq_sim_set_collider(self._h, ctypes.byref(c)))
+        self._collider = (sdf_t, nrm_t, c)  # the device arrays stay alive while set
+
     # ---- the substep (mpm_solver.py:624-631) ------------------------------
     def substep(self, s=0, t=None):
         _lib.check(_lib.lib().aq_substep(self._h, _ptr(self.state), self.current_idx, _ptr(self.ids),
@@ -356,7 +476,7 @@ class MPMSolver:
         _lib.check(_lib.lib().aq_p2g(self._h, _ptr(self.state), self.current_idx, _stream()))
 
     def grid_op(self):
-        _lib.check(_lib.lib().aq_grid_op(self._h, _stream()))
+        _lib.check(_lib.lib().aq_grid_op(self._h, self.step_count, _stream()))
 
     def g2p_bls(self, s=0, t=None):
         _lib.check(_lib.lib().aq_g2p(self._h, _ptr(self.state), self.current_idx, _ptr(self.ids),

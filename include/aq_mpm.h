@@ -153,6 +153,14 @@ typedef struct aq_sim_desc {
   /* AQ_STORE_QUANT: components in the order of g2p_bls's bit_pack_assign
      (mpm_solver.py:700-744): x[d], v[d], C[d*d] row major, J or F[d*d]. */
   aq_component comps[AQ_MAX_COMPS];
+  /* Per-axis walls (MPMConfig.lb / ub, mpm_solver.py:97-105): with
+     axis_bounds = 1, axis a has walls at lb[a] and ub[a] (cells,
+     0 <= lb[a] < ub[a] <= n_grid) instead of bound and n_grid - bound. */
+  int32_t axis_bounds;
+  int32_t lb[3], ub[3];
+  /* use_bound_clip (mpm_solver.py:696-698): G2P clamps every new position
+     component to [lb dx, ub dx]. */
+  int32_t bound_clip;
 } aq_sim_desc;
 
 typedef struct aq_sim aq_sim;
@@ -193,7 +201,7 @@ int aq_jit_build(const aq_sim_desc *desc, const char *src_dir, const char *inclu
    turns it into velocities in place. */
 int aq_substep(aq_sim *sim, void *state, int64_t n, int32_t *ids, uint32_t step, void *stream);
 int aq_p2g(aq_sim *sim, const void *state, int64_t n, void *stream);
-int aq_grid_op(aq_sim *sim, void *stream);
+int aq_grid_op(aq_sim *sim, uint32_t step, void *stream);
 int aq_g2p(aq_sim *sim, void *state, int64_t n, const int32_t *ids, uint32_t step, void *stream);
 
 /* nsteps substeps with G2P of substep t fused with P2G of substep t+1
@@ -244,6 +252,68 @@ int aq_energy(aq_sim *sim, const void *state, int64_t n, double *out, void *stre
    value outside the member's range, or NaN).  counts NULL stops counting.
    Stores still wrap (bit-identical records either way). */
 int aq_sim_count_wraps(aq_sim *sim, unsigned long long *counts);
+
+/* ---- SDF collider of the 3D fluid scenes ----------------------------------
+   Replaces init_sdf_collider / init_sdf_value (mpm_solver.py:284-319) and
+   sdf_collision / sdf_collision_no_arg / rot_scene / step_collider /
+   step_collider_difftaichi (:1444-1559).  A signed-distance volume (a
+   rotating propeller in the reference) is placed in the grid; after the
+   grid update of substep s (the `step` of aq_substep / aq_grid_op, step0 + t
+   in aq_advance) every node I is mapped into the volume by rot_scene: the
+   offset of I from the rotation centre c (x, z) is rotated by -theta, the
+   height subtracted, the indices truncated and clamped to the walls.  Where
+   the volume's value is below -1e-5 the node is inside: with normal n (the
+   stored gradient rotated by theta) and collider velocity omega e_y x
+   ((I - c) dx, 0 in y) (or of the rotated index when
+   velocity_from_rotated), the velocity relative to the collider loses its
+   component into the collider: v = v_c + v_rel - min(v_rel . n, 0) n.
+   With s counted from first_step, the collider stands still during
+   substep 0 (collider_omega is set by its first step) and has taken
+   k = max(s - lag, 0) steps at substep s:
+   theta = omega dt k, height = height_rate dt k (lag 1, rate 0.15: the
+   range and AD runs' step_collider_difftaichi; lag 0, rate 0.1: the
+   forward runs' step_collider).  3D only; the arrays are device memory the
+   caller keeps alive while the collider is set; NULL removes it. */
+typedef struct aq_collider {
+  const float *sdf;     /* [sx][sy][sz] signed distance (collider_sdf) */
+  const float *normal;  /* [sx][sy][sz][3] its gradient (collider_sdf_normal) */
+  int32_t size[3];      /* sx, sy, sz */
+  int32_t offset[3];    /* node offset + (i, j, k) holds entry (i, j, k) (sdf_offset) */
+  int32_t center[3];    /* rotation centre in nodes (get_center_idx; y unused) */
+  int32_t lag;
+  int32_t velocity_from_rotated;
+  uint32_t first_step;  /* the step of the collider's substep 0 */
+  double omega;         /* angular velocity about +y */
+  double height_rate;
+} aq_collider;
+int aq_sim_set_collider(aq_sim *sim, const aq_collider *collider);
+
+/* ---- groups: many quantized simulations advanced together ---------------
+   Replaces the reference's one-process-per-candidate evaluation of bit
+   allocations (mpm_exps/run_2d_exp.py:46-70 starting
+   run_mpm_optimality_check.py per candidate).  A group holds simulations
+   with quantized storage (any record layouts, dimensions and materials),
+   each with its own state, particle count and ids, fixed at creation.  Each
+   substep's fused G2P->P2G runs as one launch per class of simulations
+   (dimension, material, record-word bucket) whose workgroups walk the
+   items of all the class's simulations, and the gather + grid update as
+   one launch per dimension over all their grid blocks; sorts, the
+   first P2G and the last G2P run per simulation.  Results equal
+   aq_advance's on each simulation with the same sort interval and step0
+   up to float rounding (accumulation order; the untiled fallback of edge
+   particles).  The group reads its simulations' settings at creation (the
+   range-violation counter included): create it after aq_sim_count_wraps.
+   The workspace (device, caller-owned) holds the launch descriptors. */
+typedef struct aq_group aq_group;
+int64_t aq_group_workspace_bytes(int nsims);
+int aq_group_create(aq_sim *const *sims, void *const *states, const int64_t *ns, int32_t *const *ids, int nsims,
+                    void *workspace, int64_t workspace_bytes, aq_group **out);
+/* nsteps substeps of every simulation (dithering step of substep t:
+   step0 + t), sorted every `sort_every` substeps (> 0); `workgroups` is the
+   grid of the fused launches (a multiple of 8 up to 8192; 0: sized from the
+   particle count). */
+int aq_group_advance(aq_group *g, int nsteps, uint32_t step0, int sort_every, int workgroups, void *stream);
+int aq_group_destroy(aq_group *g);
 
 /* Device pointer of the grid node buffer (float4 or double4 per node,
    blocked layout: (m v, m) after aq_p2g, (v, m) after the grid update;

@@ -91,6 +91,31 @@ static void adj_cauchy_back(const orc_mpm *s, const double *F /* J or F row majo
     jf_bar[0] += s->la * tr;
     return;
   }
+  if (d == 3) {
+    /* sigma = mu (F F^T - I) + la log(J) on every entry (mpm_solver.py:1027-1029):
+       F_bar = mu (sb + sb^T) F + la (sum sb) F^-T */
+    double ssum = 0.0;
+    for (int i = 0; i < 9; i++) ssum += sb[i];
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) {
+        double acc = 0.0;
+        for (int j = 0; j < 3; j++) acc += (sb[a * 3 + j] + sb[j * 3 + a]) * F[j * 3 + b];
+        jf_bar[a * 3 + b] += s->mu_0 * acc;
+      }
+    double cof[9];
+    cof[0] = F[4] * F[8] - F[5] * F[7];
+    cof[1] = F[5] * F[6] - F[3] * F[8];
+    cof[2] = F[3] * F[7] - F[4] * F[6];
+    cof[3] = F[2] * F[7] - F[1] * F[8];
+    cof[4] = F[0] * F[8] - F[2] * F[6];
+    cof[5] = F[1] * F[6] - F[0] * F[7];
+    cof[6] = F[1] * F[5] - F[2] * F[4];
+    cof[7] = F[2] * F[3] - F[0] * F[5];
+    cof[8] = F[0] * F[4] - F[1] * F[3];
+    const double J = F[0] * cof[0] + F[1] * cof[1] + F[2] * cof[2];
+    for (int i = 0; i < 9; i++) jf_bar[i] += s->la_0 * ssum * cof[i] / J;
+    return;
+  }
   /* 2D fixed-corotated: sigma = 2 mu (F - R) F^T + la J (J - 1) I */
   const double F00 = F[0], F01 = F[1], F10 = F[2], F11 = F[3];
   const double J = F00 * F11 - F01 * F10;
@@ -280,6 +305,14 @@ void orc_adj_substep_f64(const orc_mpm *s, const double *st, int64_t n, const do
     double sig[9] = {0};
     if (s->material == ORC_FLUID) {
       for (int a = 0; a < d; a++) sig[a * d + a] = s->la * (jf[0] - 1.0);
+    } else if (d == 3) {
+      const double J = jf[0] * (jf[4] * jf[8] - jf[7] * jf[5]) - jf[3] * (jf[1] * jf[8] - jf[7] * jf[2]) +
+                       jf[6] * (jf[1] * jf[5] - jf[4] * jf[2]);
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+          double fft = jf[i * 3 + 0] * jf[j * 3 + 0] + jf[i * 3 + 1] * jf[j * 3 + 1] + jf[i * 3 + 2] * jf[j * 3 + 2];
+          sig[i * 3 + j] = s->mu_0 * (fft - (i == j ? 1.0 : 0.0)) + s->la_0 * log(J);
+        }
     } else {
       const double F00 = jf[0], F01 = jf[1], F10 = jf[2], F11 = jf[3];
       const double J = F00 * F11 - F01 * F10;

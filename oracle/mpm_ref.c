@@ -16,6 +16,7 @@
 #define SFX f32
 #define FLOOR floorf
 #define SQRT sqrtf
+#define LOG logf
 #define FMAX fmaxf
 #define FMIN fminf
 #include "orc_mpm_impl.h"
@@ -23,6 +24,7 @@
 #undef SFX
 #undef FLOOR
 #undef SQRT
+#undef LOG
 #undef FMAX
 #undef FMIN
 
@@ -30,6 +32,7 @@
 #define SFX f64
 #define FLOOR floor
 #define SQRT sqrt
+#define LOG log
 #define FMAX fmax
 #define FMIN fmin
 #include "orc_mpm_impl.h"
@@ -37,6 +40,7 @@
 #undef SFX
 #undef FLOOR
 #undef SQRT
+#undef LOG
 #undef FMAX
 #undef FMIN
 

@@ -34,7 +34,16 @@ class MPM(ctypes.Structure):
                 ("dt", ctypes.c_double), ("p_vol", ctypes.c_double),
                 ("p_mass", ctypes.c_double), ("gravity", ctypes.c_double),
                 ("la", ctypes.c_double), ("mu_0", ctypes.c_double),
-                ("la_0", ctypes.c_double)]
+                ("la_0", ctypes.c_double),
+                ("axis_bounds", ctypes.c_int32), ("lb", ctypes.c_int32 * 3), ("ub", ctypes.c_int32 * 3),
+                ("bound_clip", ctypes.c_int32)]
+
+
+class Collider(ctypes.Structure):
+    _fields_ = [("sdf", ctypes.c_void_p), ("normal", ctypes.c_void_p),
+                ("size", ctypes.c_int32 * 3), ("offset", ctypes.c_int32 * 3), ("center", ctypes.c_int32 * 3),
+                ("from_rot", ctypes.c_int32), ("yshift", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("cos_t", ctypes.c_double), ("sin_t", ctypes.c_double), ("omega", ctypes.c_double)]
 
 
 def build():
@@ -98,6 +107,7 @@ def lib():
         for sfx in ("f32", "f64"):
             getattr(L, "orc_p2g_" + sfx).argtypes = [P, P, ctypes.c_int64, P, P]
             getattr(L, "orc_grid_op_" + sfx).argtypes = [P, P, P]
+            getattr(L, "orc_collide_" + sfx).argtypes = [P, P, P]
             getattr(L, "orc_g2p_" + sfx).argtypes = [P, P, P, P, ctypes.c_int64]
             getattr(L, "orc_run_plain_" + sfx).argtypes = [P, P, ctypes.c_int64, i32, P, P, i32]
             getattr(L, "orc_energy_" + sfx).restype = f64
@@ -269,6 +279,41 @@ def grid_op(s, gm, gv):
     f64 = gm.dtype == np.float64
     fn = lib().orc_grid_op_f64 if f64 else lib().orc_grid_op_f32
     fn(ctypes.byref(s), _ptr(gm), _ptr(gv))
+
+
+def set_walls(s, lb, ub, clip=False):
+    """MPMConfig.lb / ub per axis (mpm_solver.py:97-105) and use_bound_clip (:696-698)."""
+    s.axis_bounds = 1
+    for a in range(s.dim):
+        s.lb[a], s.ub[a] = int(lb[a]), int(ub[a])
+    s.bound_clip = 1 if clip else 0
+
+
+def collider_state(dt, dx, step, omega=5.0, height_rate=0.15, lag=1):
+    """(theta, height shift in nodes, omega) of the collider at substep
+    `step`: step_collider_difftaichi (mpm_solver.py:1555-1559, lag 1: substep
+    s runs with the state set after substep s - 1) or step_collider
+    (:1548-1552, lag 0, rate 0.1); collider_omega is 0 until the first step
+    sets it."""
+    k = max(step - lag, 0)
+    theta = omega * dt * k
+    height = height_rate * dt * k
+    return theta, int(height / dx + 0.5), (omega if step >= 1 else 0.0)
+
+
+def collide(s, gv, sdf, normal, theta, yshift, omega, center, offset=(0, 0, 0), from_rot=False):
+    """sdf_collision on gv [3, nodes] in place (after grid_op).  sdf
+    [sx, sy, sz] and normal [sx, sy, sz, 3] float32."""
+    sdf = np.ascontiguousarray(sdf, dtype=np.float32)
+    normal = np.ascontiguousarray(normal, dtype=np.float32)
+    c = Collider()
+    c.sdf, c.normal = sdf.ctypes.data, normal.ctypes.data
+    for a in range(3):
+        c.size[a], c.offset[a], c.center[a] = sdf.shape[a], int(offset[a]), int(center[a])
+    c.from_rot, c.yshift = 1 if from_rot else 0, int(yshift)
+    c.cos_t, c.sin_t, c.omega = float(np.cos(theta)), float(np.sin(theta)), float(omega)
+    fn = lib().orc_collide_f64 if gv.dtype == np.float64 else lib().orc_collide_f32
+    fn(ctypes.byref(s), ctypes.byref(c), _ptr(gv))
 
 
 def g2p(s, state, gv):

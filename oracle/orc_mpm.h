@@ -27,7 +27,8 @@ enum { ORC_SEPARATING = 0, ORC_STICKY = 1 };
 typedef struct orc_mpm {
   int32_t dim;       /* 2 or 3 */
   int32_t material;  /* ORC_FLUID (J, sigma = la (J-1) I) or ORC_ELASTIC
-                        (fixed-corotated, 2D only, sigma = 2 mu (F-R) F^T + la J (J-1) I) */
+                        (2D fixed-corotated, sigma = 2 mu (F-R) F^T + la J (J-1) I;
+                        3D sigma = mu (F F^T - I) + la log(J) on every entry) */
   int32_t n_grid;
   int32_t boundary;  /* ORC_SEPARATING (mpm_solver.py:659-664) or ORC_STICKY */
   int32_t bound;     /* wall thickness in cells: lb = bound, ub = n_grid - bound */
@@ -35,7 +36,27 @@ typedef struct orc_mpm {
   double dx, inv_dx, dt, p_vol, p_mass, gravity;
   double la;         /* fluid bulk modulus (MPMSolver.la) */
   double mu_0, la_0; /* elastic Lame parameters */
+  /* MPMConfig.lb / ub (mpm_solver.py:97-105): axis_bounds = 1 puts the walls
+     of axis a at lb[a], ub[a] instead of bound, n_grid - bound */
+  int32_t axis_bounds, lb[3], ub[3];
+  int32_t bound_clip; /* use_bound_clip (mpm_solver.py:696-698): G2P clamps x to [lb dx, ub dx] */
 } orc_mpm;
+
+/* The SDF collider at one substep (mpm_solver.py:1444-1545): the volume
+   (entry (i, j, k) at node offset + (i, j, k)), the rotation centre, the
+   rotation (cos, sin of theta), the height shift int(height / dx + 0.5) in
+   nodes, the angular velocity, and whether the collider velocity uses the
+   rotated index (sdf_collision) or the node (sdf_collision_no_arg). */
+typedef struct orc_collider {
+  const float *sdf, *normal;
+  int32_t size[3], offset[3], center[3];
+  int32_t from_rot, yshift, pad;
+  double cos_t, sin_t, omega;
+} orc_collider;
+
+/* sdf_collision on grid velocities gv after orc_grid_op (3D, every node) */
+void orc_collide_f32(const orc_mpm *s, const orc_collider *c, float *grid_v);
+void orc_collide_f64(const orc_mpm *s, const orc_collider *c, double *grid_v);
 
 int orc_num_comps(const orc_mpm *s);
 

@@ -48,6 +48,22 @@ static void FN(affine)(const orc_mpm *s, const REAL *st, int64_t n, int64_t p,
     REAL J = st[(int64_t)cJF * n + p];
     REAL sig = (REAL)s->la * (J - (REAL)1.0);
     for (int i = 0; i < d; i++) cauchy[i * d + i] = sig;
+  } else if (d == 3) {
+    /* 3D: mu (F F^T - I) + la log(J), the scalar added to every entry
+       (mpm_solver.py:1027-1029); J by Matrix.determinant's cofactor
+       expansion (TS/python/taichi/lang/matrix.py:625-628) */
+    REAL F[9];
+    for (int i = 0; i < 9; i++) F[i] = st[(int64_t)(cJF + i) * n + p];
+    REAL J = F[0] * (F[4] * F[8] - F[7] * F[5]) - F[3] * (F[1] * F[8] - F[7] * F[2]) +
+             F[6] * (F[1] * F[5] - F[4] * F[2]);
+    REAL mu = (REAL)s->mu_0, lg = (REAL)s->la_0 * LOG(J);
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        REAL fft = F[i * 3 + 0] * F[j * 3 + 0];
+        fft = fft + F[i * 3 + 1] * F[j * 3 + 1];
+        fft = fft + F[i * 3 + 2] * F[j * 3 + 2];
+        cauchy[i * 3 + j] = mu * (fft - (i == j ? (REAL)1 : (REAL)0)) + lg;
+      }
   } else {
     /* 2D fixed-corotated */
     REAL F00 = st[(int64_t)(cJF + 0) * n + p], F01 = st[(int64_t)(cJF + 1) * n + p];
@@ -122,11 +138,21 @@ void FN(orc_p2g)(const orc_mpm *s, const REAL *st, int64_t n, REAL *gm, REAL *gv
   FN(p2g_span)(s, st, n, 0, n, gm, gv);
 }
 
+/* walls per axis: lb = bound, ub = n_grid - bound, or MPMConfig.lb / ub
+   (mpm_solver.py:97-105) */
+static void FN(walls)(const orc_mpm *s, int lb[3], int ub[3]) {
+  for (int a = 0; a < 3; a++) {
+    lb[a] = s->axis_bounds && a < s->dim ? s->lb[a] : s->bound;
+    ub[a] = s->axis_bounds && a < s->dim ? s->ub[a] : s->n_grid - s->bound;
+  }
+}
+
 void FN(orc_grid_op)(const orc_mpm *s, const REAL *gm, REAL *gv) {
   const int d = s->dim, ng = s->n_grid;
   const int64_t nodes = d == 3 ? (int64_t)ng * ng * ng : (int64_t)ng * ng;
   const REAL g = (REAL)(s->dt * s->gravity);
-  const int lb = s->bound, ub = ng - s->bound;
+  int lb[3], ub[3];
+  FN(walls)(s, lb, ub);
   for (int64_t node = 0; node < nodes; node++) {
     int I[3];
     if (d == 3) {
@@ -145,16 +171,63 @@ void FN(orc_grid_op)(const orc_mpm *s, const REAL *gm, REAL *gv) {
     if (s->boundary == ORC_STICKY) {
       int wall = 0;
       for (int a = 0; a < d; a++)
-        if (I[a] < lb || I[a] > ub) wall = 1;
+        if (I[a] < lb[a] || I[a] > ub[a]) wall = 1;
       if (wall)
         for (int a = 0; a < d; a++) v[a] = (REAL)0;
     } else {
       for (int a = 0; a < d; a++) {
-        if (I[a] < lb && v[a] < 0) v[a] = (REAL)0;
-        if (I[a] > ub && v[a] > 0) v[a] = (REAL)0;
+        if (I[a] < lb[a] && v[a] < 0) v[a] = (REAL)0;
+        if (I[a] > ub[a] && v[a] > 0) v[a] = (REAL)0;
       }
     }
     for (int a = 0; a < d; a++) gv[(int64_t)a * nodes + node] = v[a];
+  }
+}
+
+/* sdf_collision (mpm_solver.py:1459-1521) over every node of the dense 3D
+   grid, with rot_scene (:1524-1545): the node's offset from the centre
+   rotated by -theta, shifted down by the height, truncated and clamped to
+   the walls; sdf (:1447-1452): inside where the stored value < -1e-5
+   (outside the volume the reference's field holds 0). */
+void FN(orc_collide)(const orc_mpm *s, const orc_collider *c, REAL *gv) {
+  const int ng = s->n_grid;
+  const int64_t nodes = (int64_t)ng * ng * ng;
+  int lb[3], ub[3];
+  FN(walls)(s, lb, ub);
+  const REAL ct = (REAL)c->cos_t, st = (REAL)c->sin_t, om = (REAL)c->omega, dx = (REAL)s->dx;
+  for (int64_t node = 0; node < nodes; node++) {
+    const int I[3] = {(int)(node / ((int64_t)ng * ng)), (int)((node / ng) % ng), (int)(node % ng)};
+    int idx[3] = {I[0], I[1], I[2]};
+    if (idx[0] >= 2 * c->center[0]) idx[0] -= 2 * c->center[0];
+    if (idx[2] >= 2 * c->center[2]) idx[2] -= 2 * c->center[2];
+    const REAL o0 = (REAL)(idx[0] - c->center[0]), o2 = (REAL)(idx[2] - c->center[2]);
+    /* cos(-theta) o0 + sin(-theta) o2 + C0 + 0.5; -sin(-theta) o0 + cos(-theta) o2 + C2 + 0.5 */
+    const REAL x = ct * o0 + (-st) * o2 + (REAL)c->center[0] + (REAL)0.5;
+    const REAL z = -(-st) * o0 + ct * o2 + (REAL)c->center[2] + (REAL)0.5;
+    int r[3] = {(int)x, I[1] - c->yshift, (int)z};
+    for (int a = 0; a < 3; a++) {
+      if (r[a] > ub[a]) r[a] = ub[a];
+      if (r[a] < lb[a]) r[a] = lb[a];
+    }
+    int q[3], in = 1;
+    for (int a = 0; a < 3; a++) {
+      q[a] = r[a] - c->offset[a];
+      if (q[a] < 0 || q[a] >= c->size[a]) in = 0;
+    }
+    if (!in) continue;
+    const int64_t e = ((int64_t)q[0] * c->size[1] + q[1]) * c->size[2] + q[2];
+    if (!(c->sdf[e] < -1e-5f)) continue;
+    const REAL g0 = (REAL)c->normal[3 * e], g1 = (REAL)c->normal[3 * e + 1], g2 = (REAL)c->normal[3 * e + 2];
+    const REAL nrm[3] = {ct * g0 + st * g2, g1, -st * g0 + ct * g2};
+    const int p0 = c->from_rot ? r[0] : I[0], p2 = c->from_rot ? r[2] : I[2];
+    const REAL rr[3] = {(REAL)(p0 - c->center[0]) * dx, (REAL)0, (REAL)(p2 - c->center[2]) * dx};
+    /* omega x r with omega = (0, om, 0) */
+    const REAL cv[3] = {om * rr[2] - (REAL)0 * rr[1], (REAL)0 * rr[0] - (REAL)0 * rr[2], (REAL)0 * rr[1] - om * rr[0]};
+    REAL vr[3];
+    for (int a = 0; a < 3; a++) vr[a] = gv[(int64_t)a * nodes + node] - cv[a];
+    const REAL dn = vr[0] * nrm[0] + vr[1] * nrm[1] + vr[2] * nrm[2];
+    const REAL m = FMIN(dn, (REAL)0);
+    for (int a = 0; a < 3; a++) gv[(int64_t)a * nodes + node] = cv[a] + (vr[a] - m * nrm[a]);
   }
 }
 
@@ -194,8 +267,14 @@ static void FN(g2p_span)(const orc_mpm *s, const REAL *st, const REAL *gv, REAL 
           for (int a = 0; a < d; a++)
             for (int b = 0; b < d; b++) nC[a * d + b] = nC[a * d + b] + cw * (g[a] * dpos[b]);
         }
+    int lb[3], ub[3];
+    FN(walls)(s, lb, ub);
     for (int a = 0; a < d; a++) {
-      out[(int64_t)a * n + p] = st[(int64_t)a * n + p] + dt * nv[a];
+      REAL x = st[(int64_t)a * n + p] + dt * nv[a];
+      /* new_x[k] = max(min(new_x[k], ub[k] * dx), lb[k] * dx), the bounds
+         folded from Python doubles */
+      if (s->bound_clip) x = FMAX(FMIN(x, (REAL)(ub[a] * s->dx)), (REAL)(lb[a] * s->dx));
+      out[(int64_t)a * n + p] = x;
       out[(int64_t)(d + a) * n + p] = nv[a];
     }
     for (int i = 0; i < d * d; i++) out[(int64_t)(cC + i) * n + p] = nC[i];

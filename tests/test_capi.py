@@ -110,9 +110,11 @@ def test_sim_desc_validation():
     from autoquantizer_amd import _lib
     L = _lib.lib()
     d = _lib.SimDesc()
-    d.dim, d.material, d.n_grid, d.ncomp, d.storage = 3, _lib.AQ_ELASTIC, 64, 24, _lib.AQ_STORE_F32
-    assert L.aq_sim_workspace_bytes(ctypes.byref(d)) < 0
-    assert b"2D" in L.aq_last_error()
+    d.dim, d.material, d.n_grid, d.ncomp, d.storage = 3, _lib.AQ_ELASTIC, 64, 16, _lib.AQ_STORE_F32
+    assert L.aq_sim_workspace_bytes(ctypes.byref(d)) < 0  # 3D elastic carries 24 components
+    assert b"ncomp" in L.aq_last_error()
+    d.ncomp = 24
+    assert L.aq_sim_workspace_bytes(ctypes.byref(d)) > 0
     d.material, d.ncomp = _lib.AQ_FLUID, 16
     d.n_grid = 66
     assert L.aq_sim_workspace_bytes(ctypes.byref(d)) < 0
@@ -148,3 +150,47 @@ def test_noalloc_backward_rejects_before_device_work():
     assert L.aq_mlp_backward_noalloc(p, p, p, p, p, 0, 0.0, p, None, None) == -1
     assert L.aq_mlp_backward_noalloc(p, p, p, p, p, 4, -1.0, p, None, None) == -1
     assert L.aq_mlp_backward_noalloc(p, p, p, p, p, 4, 0.0, None, None, None) == -1
+
+
+def test_group_rejects_bad_arguments_on_the_host():
+    """aq_group_* check their arguments before any device work."""
+    from autoquantizer_amd import _lib
+    L = _lib.lib()
+    assert L.aq_group_workspace_bytes(0) < 0
+    nb = L.aq_group_workspace_bytes(32)
+    assert nb >= 32 * 64 and nb % 256 == 0
+    V = ctypes.c_void_p * 1
+    sims, states = V(None), V(0x1000)
+    ns = (ctypes.c_int64 * 1)(10)
+    h = ctypes.c_void_p()
+    ws = ctypes.c_void_p(0x100000)  # never dereferenced: each call fails its checks first
+    assert L.aq_group_create(sims, states, ns, None, 1, ws, nb, None) == -1
+    assert L.aq_group_create(sims, states, ns, None, 0, ws, nb, ctypes.byref(h)) == -1
+    assert L.aq_group_create(sims, states, ns, None, 1, ws, nb, ctypes.byref(h)) == -1
+    assert b"null simulation" in L.aq_last_error()
+    assert L.aq_group_create(sims, states, ns, None, 1, ctypes.c_void_p(0x100010), nb, ctypes.byref(h)) == -1
+    assert b"aligned" in L.aq_last_error()
+    assert L.aq_group_create(sims, states, ns, None, 1, ws, 16, ctypes.byref(h)) == -3
+    assert L.aq_group_advance(None, 4, 0, 8, 0, None) == -1
+
+
+def test_collider_and_walls_rejected_on_the_host():
+    """aq_sim_set_collider and the per-axis walls check their arguments
+    before any device work."""
+    from autoquantizer_amd import _lib
+    L = _lib.lib()
+    c = _lib.Collider()
+    assert L.aq_sim_set_collider(None, ctypes.byref(c)) == -1
+    d = _lib.SimDesc()
+    d.dim, d.material, d.n_grid, d.ncomp, d.storage, d.bound = 3, _lib.AQ_FLUID, 32, 16, _lib.AQ_STORE_F32, 3
+    d.capacity = 100
+    assert L.aq_sim_workspace_bytes(ctypes.byref(d)) > 0
+    d.axis_bounds = 1
+    d.lb[0], d.lb[1], d.lb[2] = 3, 3, 3
+    d.ub[0], d.ub[1], d.ub[2] = 29, 13, 33
+    assert L.aq_sim_workspace_bytes(ctypes.byref(d)) < 0
+    assert b"per-axis walls" in L.aq_last_error()
+    d.ub[2] = 29
+    assert L.aq_sim_workspace_bytes(ctypes.byref(d)) > 0
+    d.bound_clip = 2
+    assert L.aq_sim_workspace_bytes(ctypes.byref(d)) < 0

@@ -29,49 +29,22 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
-def lattice_block(lo, size, m=100):
-    """fill_in_box_2duniform: x = lo + j / m * size, y = lo + i / m * size."""
-    i, j = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
-    return np.stack([lo[0] + 1.0 / m * j.ravel() * size[0], lo[1] + 1.0 / m * i.ravel() * size[1]])
-
-
 def elastic_scene():
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
-    import autoquantizer_amd as aq
-    lows = [((0.2 + i * 0.4) / 4, (0.2 + i * 0.8) / 4) for i in range(4)] + \
-           [((3.1 - i * 0.2) / 4, (0.2 + i * 0.8) / 4) for i in range(4)]
-    pos = np.concatenate([lattice_block(lo, (0.7 / 4, 0.7 / 4)) for lo in lows], axis=1)
-    st = np.zeros((12, pos.shape[1]))
-    st[:2] = pos
-    st[8] = st[11] = 1.0
-    cfg = aq.MPMConfig()
-    cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.E, cfg.nu = 2, False, 128, 2e-4, 5e1, 0.2
-    cfg.p_vol = (1 / 128 * 0.5) ** 2
-    cfg.n_particles = pos.shape[1]
-    cfg.compute_fp = "f64"
-    cfg.bound = [3, 3]
-    return aq.MPMSolver(cfg), st
+    from autoquantizer_amd.scenes import elastic_jelly_2d
+    make, st = elastic_jelly_2d()
+    return make("f64"), st
 
 
 def fluid_scene():
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
-    import autoquantizer_amd as aq
-    dx = 1 / 128
-    pos = lattice_block((3 * dx, 3 * dx), (1.5 / 4, 1.5 / 4))
-    st = np.zeros((9, pos.shape[1]))
-    st[:2] = pos
-    st[8] = 1.0
-    cfg = aq.MPMConfig()
-    cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.la, cfg.E = 2, True, 128, 2e-4, 5e2, 5e4
-    cfg.p_vol = (dx * 0.5) ** 2
-    cfg.n_particles = pos.shape[1]
-    cfg.compute_fp = "f64"
-    cfg.bound = [3, 3]
-    return aq.MPMSolver(cfg), st
+    from autoquantizer_amd.scenes import fluid_2d
+    make, st = fluid_2d()
+    return make("f64"), st
 
 
 def test_elastic_energy_and_ranges():
@@ -107,3 +80,33 @@ def test_fluid_ranges():
     ref = np.load(os.path.join(GOLDEN, "ranges_fluid_2d.npy"))
     np.testing.assert_allclose(ranges[:8], ref[:8], rtol=1e-6)
     assert j_dev == pytest.approx(ref[8], rel=1e-5)
+
+
+def test_reference_search_loop_on_its_scene():
+    """run_mpm_elastic_2d.py's experiment end to end on its own scene:
+    ranges (-r), error-contribution sums (-a), LagrangianEB at the script's
+    default eps = 0.1 with bits + 1 (solve, :227-242), then dithered
+    quantized runs (-f, :186-208) judged by |loss - f64| < 3 eps f64.  The
+    allocation is also the one the reference's shipped sums and ranges give,
+    to within a bit per component."""
+    from autoquantizer_amd.scenes import ELASTIC_JELLY_F64_LOSS, ELASTIC_JELLY_SUBSTEPS, elastic_jelly_2d
+    from autoquantizer_amd.search import elastic_2d_ranges, search_scheme, validate_scheme
+    from autoquantizer_amd.solvers import LagrangianEB
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    print("[golden] elastic_2d search loop", flush=True)
+    make, st = elastic_jelly_2d()
+    eps = 0.1
+    res = search_scheme(make, st, ELASTIC_JELLY_SUBSTEPS, tol_rel=eps, checkpoint_every=64)
+    assert res["loss"] == pytest.approx(ELASTIC_JELLY_F64_LOSS, abs=1e-3)
+    g = np.load(os.path.join(GOLDEN, "grads_elastic_2d.npy"))
+    r = elastic_2d_ranges(np.load(os.path.join(GOLDEN, "ranges_elastic_2d_t8193.npy")), 2)
+    ref_bits, _ = LagrangianEB(g * r ** 2, ref=ELASTIC_JELLY_F64_LOSS, tol_rel=eps)
+    print("bits", np.round(res["bits"], 2).tolist(), "from the shipped sums", np.round(ref_bits + 1, 2).tolist(),
+          flush=True)
+    assert np.all(np.abs(res["bits"] - (ref_bits + 1)) < 1.0)
+    val = validate_scheme(make("f64"), st, ELASTIC_JELLY_SUBSTEPS, res["bits"], res["ranges"],
+                          ELASTIC_JELLY_F64_LOSS, eps, runs=4)
+    print("success", val["success"], "relative errors", val["rel_error"].tolist(), flush=True)
+    assert val["success"] == 1.0

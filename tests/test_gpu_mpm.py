@@ -80,7 +80,7 @@ def step_sizes(sim, dec):
     """Quantization step of every component at every particle: fixed-point
     scale, or one digit of the (shared- or own-) exponent group (the group's
     largest member's power of two over 2^(digit bits - 1)); for unquantized
-    f32 members 2^-20 of the component's largest value.  A float member's
+    f32 members 2^-18 of the component's largest value.  A float member's
     step is at least 2^-20 of its group's largest value over all particles:
     near zero its digit step falls below the f32 rounding of the physics
     (v is a sum of grid velocities of the field's magnitude)."""
@@ -88,8 +88,11 @@ def step_sizes(sim, dec):
     for k, c in enumerate(sim.comps):
         if c["kind"] == 0:
             steps[k] = c["scale"]
-        elif c["kind"] == 3:  # unquantized f32: operand order differences, relative to the largest value
-            steps[k] = max(float(np.abs(dec[k]).max()), 1e-30) * 2.0 ** -20
+        elif c["kind"] == 3:
+            # unquantized f32: operand-order differences of a 9-term sum
+            # whose terms exceed the result (C is a difference quotient of
+            # grid velocities), relative to the component's largest value
+            steps[k] = max(float(np.abs(dec[k]).max()), 1e-30) * 2.0 ** -18
         else:
             grp = [j for j, cc in enumerate(sim.comps) if cc["kind"] == 2 and cc["exp_offset"] == c["exp_offset"]]
             mx = np.abs(dec[grp]).max(axis=0)
@@ -115,6 +118,43 @@ def test_p2g_grid(orc, kind, tiled):
     assert g[d] == approx(gm, rel=2e-5, abs=1e-6 * s.p_mass)
     mom_scale = np.abs(gv).max()
     assert np.abs(g[:d] - gv).max() <= 2e-5 * mom_scale
+
+
+@pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d"])
+def test_gather_reaches_particles_far_from_the_sort(orc, kind):
+    """After a sort, a third of the particles are moved several grid blocks
+    (in place, so the sort's work items stay current): their stencils leave
+    their tiles, they scatter through the untiled path into blocks the
+    sort's block list does not hold, and the gather must still add them.
+    Twice in a row (the second P2G reaches the same far blocks again, now
+    held by the list), against the oracle's P2G of the moved state."""
+    torch = _torch()
+    print(f"[gather list] {kind}", flush=True)
+    sim, s = scene(orc, kind, 4000, seed=23, storage="f32")
+    sim.build_pid()
+    n = sim.current_idx
+    idx = torch.arange(0, n, 3, device="cuda")
+    shift = 0.25 if kind == "fluid_3d" else 0.2  # 16 cells: 4 blocks (3D), 3 blocks (2D)
+    sim.state[0, idx] += shift
+    sim.state[1, idx] -= shift / 2
+    st = sim.get_state(original_order=False).astype(np.float32)
+    gm, gv = orc.grids(s)
+    orc.p2g(s, st, gm, gv)
+    d = s.dim
+    for rep in range(2):
+        sim.p2g_bls()
+        g = sim.grid_dense()
+        assert g[d] == approx(gm, rel=2e-5, abs=1e-6 * s.p_mass), rep
+        assert np.abs(g[:d] - gv).max() <= 2e-5 * np.abs(gv).max(), rep
+    # and back: the far blocks, now empty, are cleared by the next gather
+    sim.state[0, idx] -= shift
+    sim.state[1, idx] += shift / 2
+    st = sim.get_state(original_order=False).astype(np.float32)
+    gm, gv = orc.grids(s)
+    orc.p2g(s, st, gm, gv)
+    sim.p2g_bls()
+    g = sim.grid_dense()
+    assert g[d] == approx(gm, rel=2e-5, abs=1e-6 * s.p_mass)
 
 
 @pytest.mark.parametrize("kind", ["elastic_2d", "fluid_3d"])

@@ -14,6 +14,9 @@ def small_scene(orc, kind, n, seed=0):
     elif kind == "elastic_2d":
         s = orc.mpm_params(2, orc.ELASTIC, 32, 1e-4, E=1e4, nu=0.2, boundary=orc.STICKY)
         d = 2
+    elif kind == "elastic_3d":
+        s = orc.mpm_params(3, orc.ELASTIC, 16, 1e-4, E=5e3, nu=0.3)
+        d = 3
     else:
         s = orc.mpm_params(3, orc.FLUID, 16, 2e-4, la=400.0)
         d = 3
@@ -24,6 +27,8 @@ def small_scene(orc, kind, n, seed=0):
     st[2 * d:2 * d + d * d] = rng.uniform(-2, 2, (d * d, n))
     if kind == "elastic_2d":
         st[-4:] = np.eye(2).reshape(-1, 1) + rng.uniform(-0.05, 0.05, (4, n))
+    elif kind == "elastic_3d":
+        st[-9:] = np.eye(3).reshape(-1, 1) + rng.uniform(-0.05, 0.05, (9, n))
     else:
         st[-1] = rng.uniform(0.95, 1.05, n)
     return s, st
@@ -37,7 +42,7 @@ def loss_of(orc, s, st, nsteps, loss, direction=None):
     return float((direction * x).sum())
 
 
-@pytest.mark.parametrize("kind", ["fluid_2d", "elastic_2d", "fluid_3d"])
+@pytest.mark.parametrize("kind", ["fluid_2d", "elastic_2d", "fluid_3d", "elastic_3d"])
 @pytest.mark.parametrize("loss", ["energy", "linear"])
 def test_adjoint_matches_central_differences(orc, kind, loss):
     s, st = small_scene(orc, kind, 150, seed=1)

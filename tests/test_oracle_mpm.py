@@ -4,8 +4,13 @@
 - closed forms of the quadratic B-spline transfer used by
   mpm_exps/engine/mpm_solver.py:590-758: partition of unity, P2G mass and
   momentum conservation, G2P reproduction of an affine velocity field;
-- grid boundary conditions (mpm_solver.py:659-664) and the sticky variant;
-- rest state and free fall closed forms.
+- grid boundary conditions (mpm_solver.py:659-664) and the sticky variant,
+  per-axis walls (MPMConfig.lb / ub) and the bound clip (:696-698);
+- rest state and free fall closed forms;
+- the SDF collider (sdf_collision, rot_scene, :1444-1545): a static
+  half-space removes only the velocity into it, a quarter turn maps nodes
+  to voxels by the rotated offsets, and a node inside a spinning collider
+  with no velocity of its own is carried at omega x r.
 """
 import numpy as np
 import pytest
@@ -220,5 +225,164 @@ def test_fixed_corotated_stress_closed_form(orc, case):
     sigma = A / (-dt * s.p_vol * 4 / dx ** 2)
     J = sx * ty
     want = 2 * mu * R @ np.diag([(sx - 1) * sx, (ty - 1) * ty]) @ R.T + la * J * (J - 1) * np.eye(2)
+    scale = max(np.abs(want).max(), mu)
+    assert np.abs(sigma - want).max() <= 1e-9 * scale, (sigma, want)
+
+
+def _grid3(orc, n=16, bound=3):
+    s = orc.mpm_params(3, orc.FLUID, n, 1e-4, gravity=0.0, bound=bound)
+    gm, gv = orc.grids(s)
+    gm[:] = 1.0
+    return s, gm, gv
+
+
+def test_per_axis_walls(orc):
+    """Separating walls at lb / ub per axis (mpm_solver.py:97-105, 659-664):
+    velocities into a wall vanish beyond it, everything else is kept."""
+    s, gm, gv = _grid3(orc)
+    orc.set_walls(s, [2, 3, 4], [13, 6, 11])
+    rng = np.random.default_rng(0)
+    gv[:] = rng.uniform(-1, 1, gv.shape).astype(np.float32)
+    v0 = gv.copy()
+    orc.grid_op(s, gm, gv)
+    I = np.indices((16, 16, 16)).reshape(3, -1)
+    lb, ub = np.array([2, 3, 4])[:, None], np.array([13, 6, 11])[:, None]
+    want = np.where(((I < lb) & (v0 < 0)) | ((I > ub) & (v0 > 0)), 0.0, v0 / np.float32(1 + 1e-10))
+    np.testing.assert_allclose(gv, want, rtol=1e-6, atol=0)
+
+
+def test_bound_clip(orc):
+    """use_bound_clip: G2P positions are clamped to [lb dx, ub dx] per axis;
+    velocities are not."""
+    s = orc.mpm_params(2, orc.FLUID, 32, 1e-2, gravity=0.0)
+    orc.set_walls(s, [3, 5], [29, 20], clip=True)
+    n = 200
+    rng = np.random.default_rng(1)
+    st = np.zeros((9, n), dtype=np.float32)
+    st[:2] = rng.uniform(4 / 32, 19 / 32, (2, n))
+    st[8] = 1.0
+    gm, gv = orc.grids(s)
+    gv[0] = -100.0  # a uniform grid velocity that carries every particle through the walls
+    gv[1] = 100.0
+    out = orc.g2p(s, st, gv)
+    assert np.all(out[0] == np.float32(3 / 32)) and np.all(out[1] == np.float32(20 / 32))
+    assert np.allclose(out[2], -100.0) and np.allclose(out[3], 100.0)
+    s.bound_clip = 0
+    free = orc.g2p(s, st, gv)
+    assert np.all(free[0] < 3 / 32) and np.all(free[1] > 20 / 32)
+
+
+def test_collider_static_half_space(orc):
+    """Every voxel inside, normal +y, no rotation or spin: v_y < 0 becomes 0,
+    every other velocity is kept (v_rel - min(v_rel . n, 0) n)."""
+    s, gm, gv = _grid3(orc)
+    rng = np.random.default_rng(2)
+    gv[:] = rng.uniform(-1, 1, gv.shape).astype(np.float32)
+    orc.grid_op(s, gm, gv)
+    v0 = gv.copy()
+    sdf = np.full((16, 16, 16), -1.0, np.float32)
+    nrm = np.zeros((16, 16, 16, 3), np.float32)
+    nrm[..., 1] = 1.0
+    orc.collide(s, gv, sdf, nrm, 0.0, 0, 0.0, (8, 8, 8))
+    np.testing.assert_array_equal(gv[0], v0[0])
+    np.testing.assert_array_equal(gv[2], v0[2])
+    np.testing.assert_array_equal(gv[1], np.maximum(v0[1], 0))
+
+
+def test_collider_quarter_turn_maps_nodes(orc):
+    """theta = pi/2: node (C0 + a, y, C2 + b) reads voxel (C0 - b, y - shift,
+    C2 + a) (the offset rotated by -theta, 0.5 added and truncated), and the
+    stored normal (1, 0, 0) acts as (cos theta, 0, -sin theta) = (0, 0, -1)."""
+    s, gm, gv = _grid3(orc)
+    C = (8, 8, 8)
+    gv[2] = 1.0  # moving along +z, into a collider whose normal is -z
+    orc.grid_op(s, gm, gv)
+    v0 = gv.copy()
+    sdf = np.zeros((16, 16, 16), np.float32)
+    nrm = np.zeros((16, 16, 16, 3), np.float32)
+    vox = (6, 7, 10)
+    sdf[vox] = -0.5
+    nrm[vox] = (1.0, 0.0, 0.0)
+    shift = 2
+    orc.collide(s, gv, sdf, nrm, np.pi / 2, shift, 0.0, C)
+    hit = np.argwhere((gv != v0).any(axis=0).reshape(16, 16, 16))
+    # vox = (C0 - b, y - shift, C2 + a) -> a = 2, b = 2, y = 9
+    assert hit.tolist() == [[C[0] + 2, 9, C[2] + 2]]
+    assert abs(gv[2].reshape(16, 16, 16)[C[0] + 2, 9, C[2] + 2]) < 1e-6
+
+
+def test_collider_carries_resting_nodes(orc):
+    """A node at rest inside a collider spinning at omega about the vertical
+    line through C, with the normal along x: the collider's velocity
+    omega e_y x r = (omega r_z, 0, -omega r_x) is imposed only along the
+    normal, and only where the collider moves into the node (no friction):
+    v = (max(omega r_z, 0), 0, 0)."""
+    s, gm, gv = _grid3(orc)
+    sdf = np.full((16, 16, 16), -1.0, np.float32)
+    nrm = np.zeros((16, 16, 16, 3), np.float32)
+    nrm[..., 0] = 1.0
+    om = 5.0
+    orc.grid_op(s, gm, gv)  # all at rest
+    orc.collide(s, gv, sdf, nrm, 0.0, 0, om, (8, 8, 8))
+    I = np.indices((16, 16, 16)).reshape(3, -1)
+    rx, rz = (I[0] - 8) / 16.0, (I[2] - 8) / 16.0
+    cvx = om * rz
+    np.testing.assert_allclose(gv[0], np.maximum(cvx, 0.0), atol=1e-6)
+    np.testing.assert_allclose(gv[2], 0.0, atol=1e-6)
+    assert np.all(gv[1] == 0)
+
+
+def test_collider_schedule(orc):
+    """step_collider_difftaichi (lag 1): substep s runs with theta =
+    omega dt (s - 1); collider_omega is 0 during substep 0."""
+    th, sh, om = orc.collider_state(4e-4, 1 / 64, 0)
+    assert th == 0 and sh == 0 and om == 0
+    th, sh, om = orc.collider_state(4e-4, 1 / 64, 1)
+    assert th == 0 and om == 5.0
+    th, sh, om = orc.collider_state(4e-4, 1 / 64, 1001)
+    assert th == approx(5.0 * 4e-4 * 1000) and sh == int(0.15 * 4e-4 * 1000 * 64 + 0.5)
+
+
+@pytest.mark.parametrize("case", ["rotation", "uniform_stretch", "shear"])
+def test_elastic_3d_stress_closed_form(orc, case):
+    """The oracle's 3D elastic stress (mpm_solver.py:1027-1029:
+    mu (F F^T - I) + la log(J), the scalar broadcast to every entry), read
+    back through its P2G as in the 2D test (sum_i w_i (x_i - x_p)(x_i - x_p)^T
+    = dx^2/4 I for the tensor-product quadratic B-spline).  Closed forms: a
+    rotation gives F F^T = I and J = 1, no stress; F = s I gives
+    mu (s^2 - 1) I + 3 la log(s) everywhere; a simple shear F = I + g e_x e_y^T
+    keeps J = 1, so sigma = mu (F F^T - I)."""
+    E, nu, dt, ng = 5e4, 0.3, 1e-4, 32
+    mu, la = E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+    s = orc.mpm_params(3, orc.ELASTIC, ng, dt, gravity=0.0, E=E, nu=nu)
+    if case == "rotation":
+        a, b = 0.7, -0.4
+        Rz = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+        Rx = np.array([[1, 0, 0], [0, np.cos(b), -np.sin(b)], [0, np.sin(b), np.cos(b)]])
+        F = Rz @ Rx
+        want = np.zeros((3, 3))
+    elif case == "uniform_stretch":
+        sc = 1.1
+        F = sc * np.eye(3)
+        want = mu * (sc * sc - 1) * np.eye(3) + 3 * la * np.log(sc)
+    else:
+        F = np.eye(3)
+        F[0, 1] = 0.3
+        want = mu * (F @ F.T - np.eye(3))
+    st = np.zeros((24, 1), dtype=np.float64)
+    xp = np.array([0.5 + 0.37 / ng, 0.5 + 0.61 / ng, 0.5 + 0.13 / ng])
+    st[:3, 0] = xp
+    st[15:, 0] = F.reshape(-1)
+    gm, gv = orc.grids(s, f64=True)
+    orc.p2g(s, st, gm, gv)
+    dx = 1.0 / ng
+    nodes = np.nonzero(gm)[0]
+    assert len(nodes) == 27
+    A = np.zeros((3, 3))
+    for nd in nodes:
+        I = np.array([nd // (ng * ng), (nd // ng) % ng, nd % ng])
+        A += np.outer(gv[:, nd], I * dx - xp)
+    A *= 4.0 / dx ** 2
+    sigma = A / (-dt * s.p_vol * 4 / dx ** 2)
     scale = max(np.abs(want).max(), mu)
     assert np.abs(sigma - want).max() <= 1e-9 * scale, (sigma, want)
