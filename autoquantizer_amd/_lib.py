@@ -149,7 +149,7 @@ def lib():
     L.aq_advance_timed.argtypes = [P, P, i64, i32, u32, i32, P, P, P, P, P]
     L.aq_adj_workspace_bytes.restype = i64
     L.aq_adj_workspace_bytes.argtypes = [P]
-    L.aq_adj_substep.argtypes = [P, P, i64, P, P, P, P, i64, P]
+    L.aq_adj_substep.argtypes = [P, u32, P, i64, P, P, P, P, i64, P]
     L.aq_sim_specialize.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]
     L.aq_jit_build.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, i64]
     L.aq_noise_add.argtypes = [P, P, i64, P, P, u64, u32, P]

@@ -33,14 +33,15 @@ class SubstepAdjoint:
             _lib.check(-1)
         self.ws = torch.empty(int(nbytes), dtype=torch.uint8, device=solver.device)
 
-    def step(self, state_t, adj_next, adj_out, gsum=None):
+    def step(self, state_t, adj_next, adj_out, gsum=None, step=0):
         """adj_out = dL/ds_t from adj_next = dL/ds_{t+1}; all [ncomp, capacity]
         tensors of the solver's dtype.  gsum (float64 [ncomp] on the device):
-        sum_p adj_out^2 is added to it."""
+        sum_p adj_out^2 is added to it.  step: the substep s_t -> s_{t+1}
+        ran at (the collider's schedule)."""
         s = self.solver
-        _lib.check(_lib.lib().aq_adj_substep(s._h, _ptr(state_t), s.current_idx, _ptr(adj_next), _ptr(adj_out),
-                                             _ptr(gsum) if gsum is not None else None, _ptr(self.ws),
-                                             self.ws.numel(), _stream()))
+        _lib.check(_lib.lib().aq_adj_substep(s._h, int(step), _ptr(state_t), s.current_idx, _ptr(adj_next),
+                                             _ptr(adj_out), _ptr(gsum) if gsum is not None else None,
+                                             _ptr(self.ws), self.ws.numel(), _stream()))
 
 
 def energy_loss(solver, state):
@@ -53,12 +54,13 @@ def energy_loss(solver, state):
     return (v * v).sum()
 
 
-def _advance_plain(solver, state, nsteps):
+def _advance_plain(solver, state, nsteps, step0=0):
     """nsteps plain substeps of `state` (written in place; LDS-tiled when the
-    solver's last sort sorted this buffer, else untiled)."""
+    solver's last sort sorted this buffer, else untiled), the first at
+    substep step0 (the collider's schedule)."""
     if nsteps <= 0:
         return
-    _lib.check(_lib.lib().aq_advance(solver._h, _ptr(state), solver.current_idx, int(nsteps), 0, 0, None,
+    _lib.check(_lib.lib().aq_advance(solver._h, _ptr(state), solver.current_idx, int(nsteps), int(step0), 0, None,
                                      _stream()))
 
 
@@ -107,9 +109,16 @@ class _Schedule:
     precision that runs the reverse pass (e.g. the fp32 adjoint kernels
     around an f64 forward): each of its segments sorts a copy of the
     checkpoint with its own sort, and the forward's states are cast and
-    permuted into that order."""
+    permuted into that order.
 
-    def __init__(self, solver, nsteps, checkpoint_every, noise=None, adjoint_solver=None):
+    Substep t of the schedule (s_t -> s_{t+1}) runs at the solver's
+    step_count + t, as advance() would run it (the collider's schedule).
+
+    grad_clip (use_grad_clip, accu_grad at mpm_solver.py:1234-1252): the
+    squared x adjoint of state t+1, particle p, axis k enters the sums only
+    where it differs from state t's by at most grad_clip."""
+
+    def __init__(self, solver, nsteps, checkpoint_every, noise=None, adjoint_solver=None, grad_clip=None):
         if checkpoint_every < 1:
             raise ValueError("checkpoint_every must be >= 1")
         if noise is not None and len(noise.sigma) != solver.ncomp:
@@ -122,15 +131,17 @@ class _Schedule:
         self.noise = noise
         self.adj_solver = adjoint_solver if adjoint_solver is not None else solver
         self.checkpoints = []
+        self.step0 = int(solver.step_count)
+        self.grad_clip = None if grad_clip is None else float(grad_clip)
 
     def _advance(self, s, ids, t0, nsteps):
         """nsteps substeps of s (sorted, particle ids `ids`), the first
         producing state t0 + 1."""
         if self.noise is None:
-            _advance_plain(self.solver, s, nsteps)
+            _advance_plain(self.solver, s, nsteps, self.step0 + t0)
             return
         for t in range(t0 + 1, t0 + nsteps + 1):
-            _advance_plain(self.solver, s, 1)
+            _advance_plain(self.solver, s, 1, self.step0 + t - 1)
             self.noise.add(self.solver, s, t, ids)
 
     def forward(self, state0):
@@ -152,7 +163,10 @@ class _Schedule:
         a = adj_T.clone()
         b = torch.empty_like(a)
         n = self.solver.current_idx
+        d = self.solver.dim
         mixed = self.adj_solver is not self.solver
+        clip = self.grad_clip is not None and gsum is not None
+        xsum = torch.zeros(d, dtype=torch.float64, device=a.device) if clip else None
         for j in reversed(range(len(self.checkpoints))):
             t0 = j * self.k
             t1 = min(t0 + self.k, self.nsteps)
@@ -179,10 +193,17 @@ class _Schedule:
                     wa[:, :n] = st[:, gather].to(a.dtype)
                 else:
                     wa.copy_(st)
-                adj.step(wa, a, b, gsum if t >= 1 else None)
+                adj.step(wa, a, b, gsum if t >= 1 else None, self.step0 + t)
+                if clip and t + 1 <= self.nsteps - 1:
+                    # state t+1's x adjoint (a) against state t's (b), same order
+                    ga, gb = a[:d, :n].double(), b[:d, :n].double()
+                    keep = (ga - gb).abs() <= self.grad_clip
+                    xsum += torch.where(keep, ga * ga, torch.zeros_like(ga)).sum(dim=1)
                 a, b = b, a
             b[:, idx_a] = a[:, :n]
             a[:, :n] = b[:, :n]
+        if clip:
+            gsum[:d] = xsum
         return a
 
 
@@ -196,8 +217,8 @@ class Rollout(torch.autograd.Function):
     in solver.adjoint_dsigma (float64 [ncomp] on the device)."""
 
     @staticmethod
-    def forward(ctx, state0, solver, nsteps, checkpoint_every=16, noise=None, adjoint_solver=None):
-        sched = _Schedule(solver, nsteps, checkpoint_every, noise, adjoint_solver)
+    def forward(ctx, state0, solver, nsteps, checkpoint_every=16, noise=None, adjoint_solver=None, grad_clip=None):
+        sched = _Schedule(solver, nsteps, checkpoint_every, noise, adjoint_solver, grad_clip)
         out = sched.forward(state0.detach())
         ctx.sched = sched
         return out
@@ -213,18 +234,20 @@ class Rollout(torch.autograd.Function):
         grad0 = sched.backward(grad_T.contiguous().to(adt), gsum, adj, dsig)
         solver.adjoint_gsum = gsum
         solver.adjoint_dsigma = dsig
-        return grad0.to(grad_T.dtype), None, None, None, None, None
+        return grad0.to(grad_T.dtype), None, None, None, None, None, None
 
 
-def error_contributions(solver, nsteps, loss=energy_loss, checkpoint_every=16):
+def error_contributions(solver, nsteps, loss=energy_loss, checkpoint_every=16, grad_clip=None):
     """The reference's AD run (run_forward_with_ad_nlogn + get_gradients):
-    nsteps substeps from the solver's current state, loss(solver, s_T), and
-    per component g_c = sum_{t=1}^{T-1} sum_p (dL/ds_{t,p,c})^2 (f64).
-    Returns (loss value, dL/ds_0 as [ncomp, n] numpy, g as numpy [ncomp]).
-    The solver's state is left unchanged."""
+    nsteps substeps from the solver's current state (substep t at
+    step_count + t), loss(solver, s_T), and per component g_c =
+    sum_{t=1}^{T-1} sum_p (dL/ds_{t,p,c})^2 (f64); grad_clip: the
+    reference's use_grad_clip threshold for the x sums (0.1 in its 3D fluid
+    AD run).  Returns (loss value, dL/ds_0 as [ncomp, n] numpy, g as numpy
+    [ncomp]).  The solver's state is left unchanged."""
     n = solver.current_idx
     s0 = solver.state.clone().requires_grad_(True)
-    sT = Rollout.apply(s0, solver, nsteps, checkpoint_every)
+    sT = Rollout.apply(s0, solver, nsteps, checkpoint_every, None, None, grad_clip)
     L = loss(solver, sT)
     L.backward()
     torch.cuda.current_stream().synchronize()

@@ -14,6 +14,7 @@
 // velocity component zero its adjoint.
 #pragma once
 
+#include "aq_gather.hpp"  // collider_probe
 #include "aq_mpm_kernels.hpp"
 
 namespace aq {
@@ -186,9 +187,13 @@ __global__ void __launch_bounds__(256) k_adj_g2p(const T *__restrict__ st, const
 }
 
 // Grid-update adjoint on the active blocks: gbar (adjoint of the velocity)
-// becomes the adjoint of (m v, m) in place; pre holds the P2G result.
+// becomes the adjoint of (m v, m) in place; pre holds the P2G result.  With
+// the collider (3D), first its adjoint: where the node is inside and the
+// velocity after the walls moves into the collider (v_rel . n < 0), the
+// update v - (v_rel . n) n has Jacobian I - n n^T.
 template <int DIM, class T>
-__global__ void __launch_bounds__(256) k_adj_grid(ParamsT<T> P, int32_t nblocks, const uint32_t *__restrict__ act,
+__global__ void __launch_bounds__(256) k_adj_grid(ParamsT<T> P, ColliderK K, int32_t nblocks,
+                                                  const uint32_t *__restrict__ act,
                                                   const vec4_t<T> *__restrict__ pre, T *__restrict__ gbar) {
   const int32_t b = (int32_t)(blockIdx.x * 4u + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
@@ -216,6 +221,30 @@ __global__ void __launch_bounds__(256) k_adj_grid(ParamsT<T> P, int32_t nblocks,
   T v[3] = {inv_m * mv[0], inv_m * mv[1], inv_m * mv[2]};
   v[1] = v[1] - P.g_dt;
   T gb[3] = {gbar[node * 4 + 0], gbar[node * 4 + 1], gbar[node * 4 + 2]};
+  if constexpr (DIM == 3) {
+    T n[3], cv[3];
+    if (K.on && collider_probe<T>(P, K, I, n, cv)) {
+      T vw[3] = {v[0], v[1], v[2]};  // the velocity after the walls, as the forward collides it
+      if (P.boundary == AQ_STICKY) {
+        bool wall = false;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) wall = wall || I[d] < P.lb[d] || I[d] > P.ub[d];
+        if (wall) vw[0] = vw[1] = vw[2] = (T)0;
+      } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          if (I[d] < P.lb[d] && vw[d] < (T)0) vw[d] = (T)0;
+          if (I[d] > P.ub[d] && vw[d] > (T)0) vw[d] = (T)0;
+        }
+      }
+      const T dn = (vw[0] - cv[0]) * n[0] + (vw[1] - cv[1]) * n[1] + (vw[2] - cv[2]) * n[2];
+      if (dn < (T)0) {
+        const T ng = gb[0] * n[0] + gb[1] * n[1] + gb[2] * n[2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) gb[a] -= ng * n[a];
+      }
+    }
+  }
   if (P.boundary == AQ_STICKY) {
     bool wall = false;
 #pragma unroll

@@ -4,11 +4,12 @@
 #pragma once
 #include "aq_sim.hpp"
 
-// sdf_collision(_no_arg) of node I (mpm_solver.py:1459-1521) with rot_scene
-// (:1524-1545) and sdf (:1447-1452) on the updated velocity v.
+// The collider at node I (rot_scene, mpm_solver.py:1524-1545, and sdf,
+// :1447-1452): whether I maps to an inside voxel, and then the voxel's
+// normal rotated by theta (n) and the collider velocity omega e_y x r (cv).
 template <class T>
-__device__ __forceinline__ void collide_node(const ParamsT<T> &P, const ColliderK &K, const int (&I)[3],
-                                             T (&v)[3]) {
+__device__ __forceinline__ bool collider_probe(const ParamsT<T> &P, const ColliderK &K, const int (&I)[3],
+                                               T (&n)[3], T (&cv)[3]) {
   int w[3] = {I[0], I[1], I[2]};
   if (w[0] >= 2 * K.center[0]) w[0] -= 2 * K.center[0];
   if (w[2] >= 2 * K.center[2]) w[2] -= 2 * K.center[2];
@@ -28,15 +29,29 @@ __device__ __forceinline__ void collide_node(const ParamsT<T> &P, const Collider
     q[a] = id[a] - K.offset[a];
     vol = vol && q[a] >= 0 && q[a] < K.size[a];
   }
-  if (!vol) return;  // outside the volume: the stored value is 0, not inside
+  if (!vol) return false;  // outside the volume: the stored value is 0, not inside
   const int64_t e = ((int64_t)q[0] * K.size[1] + q[1]) * K.size[2] + q[2];
-  if (!(K.sdf[e] < -1e-5f)) return;
+  if (!(K.sdf[e] < -1e-5f)) return false;
   const T g0 = (T)K.normal[3 * e], g1 = (T)K.normal[3 * e + 1], g2 = (T)K.normal[3 * e + 2];
-  const T n[3] = {c * g0 + sn * g2, g1, -sn * g0 + c * g2};
+  n[0] = c * g0 + sn * g2;
+  n[1] = g1;
+  n[2] = -sn * g0 + c * g2;
   const int p0 = K.from_rot ? id[0] : I[0], p2 = K.from_rot ? id[2] : I[2];
   const T om = (T)K.omega;
   const T r0 = (T)(p0 - K.center[0]) * P.dx, r2 = (T)(p2 - K.center[2]) * P.dx;
-  const T cv[3] = {om * r2, (T)0, -(om * r0)};  // omega e_y x (r0, 0, r2)
+  cv[0] = om * r2;  // omega e_y x (r0, 0, r2)
+  cv[1] = (T)0;
+  cv[2] = -(om * r0);
+  return true;
+}
+
+// sdf_collision(_no_arg) of node I (mpm_solver.py:1459-1521) on the updated
+// velocity v: v = cv + (v_rel - min(v_rel . n, 0) n), v_rel = v - cv.
+template <class T>
+__device__ __forceinline__ void collide_node(const ParamsT<T> &P, const ColliderK &K, const int (&I)[3],
+                                             T (&v)[3]) {
+  T n[3], cv[3];
+  if (!collider_probe<T>(P, K, I, n, cv)) return;
   T vr[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) vr[a] = v[a] - cv[a];

@@ -1268,7 +1268,7 @@ extern "C" int aq_sim_specialize(aq_sim *s, const char *src_dir, const char *inc
 }
 
 // The collider at substep `step` (aq_collider's schedule).
-static ColliderK collider_at(const aq_sim *s, uint32_t step) {
+ColliderK collider_at(const aq_sim *s, uint32_t step) {
   ColliderK K{};
   if (!s->has_collider) return K;
   const aq_collider &c = s->col;

@@ -197,6 +197,8 @@ static inline bool items_valid(const aq_sim *s, const void *state, int64_t n) {
 StateK make_state(const aq_sim *s, const void *state, const int32_t *ids);
 CodecK make_codec(const aq_sim *s, uint32_t step);
 int launch_gather(aq_sim *s, bool tiled, bool op, uint32_t step, hipStream_t st);
+// the collider of substep `step` (off when none is set or in 2D)
+ColliderK collider_at(const aq_sim *s, uint32_t step);
 // aq_particles.hip
 int launch_p2g(aq_sim *s, const void *state, int64_t n, bool *tiled, hipStream_t st);
 int launch_g2p(aq_sim *s, void *state, int64_t n, const int32_t *ids, uint32_t step, hipStream_t st);

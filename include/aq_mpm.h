@@ -322,19 +322,22 @@ int aq_group_destroy(aq_group *g);
 int aq_sim_grid(aq_sim *sim, void **grid, int64_t *nodes);
 
 /* ---- adjoint of the plain substep: replaces the reference's Taichi-autodiff
-   p2g.grad / grid_op.grad / g2p.grad and accu_grad
-   (mpm_solver.py:800-858, 1167-1188, 1232-1271) ---------------------------- */
+   p2g.grad / grid_op.grad / sdf_collision_no_arg.grad / g2p.grad and
+   accu_grad (mpm_solver.py:800-858, 1167-1198, 1232-1271, 1476-1504) ----- */
 /* Workspace of aq_adj_substep: the P2G grid and the grid adjoint. */
 int64_t aq_adj_workspace_bytes(const aq_sim *sim);
 /* One reverse substep on plain storage (AQ_STORE_F32 or AQ_STORE_F64):
    adj_out = dL/ds_t from adj_next = dL/ds_{t+1}, for the state s_t in
    `state` (value planes, stride capacity; adj_next and adj_out alike).
    Recomputes s_t's P2G (LDS-tiled when the last aq_sort of `sim` sorted
-   this buffer, else untiled) and grid update, then runs the G2P, grid and
-   P2G adjoint kernels.  If gsum (device, double[ncomp]) is not NULL,
-   sum_p adj_out[c][p]^2 is added to gsum[c] (the reference's accu_grad). */
-int aq_adj_substep(aq_sim *sim, const void *state, int64_t n, const void *adj_next, void *adj_out,
-                   double *gsum, void *workspace, int64_t workspace_bytes, void *stream);
+   this buffer, else untiled) and grid update (with the SDF collider of
+   substep `step`, the step aq_advance ran s_t -> s_{t+1} at), then runs the
+   G2P, grid (walls and collider) and P2G adjoint kernels.  If gsum (device,
+   double[ncomp]) is not NULL, sum_p adj_out[c][p]^2 is added to gsum[c]
+   (the reference's accu_grad).  The bound clip has no adjoint here
+   (AQ_ERR_INVALID). */
+int aq_adj_substep(aq_sim *sim, uint32_t step, const void *state, int64_t n, const void *adj_next,
+                   void *adj_out, double *gsum, void *workspace, int64_t workspace_bytes, void *stream);
 
 /* Modelled quantization noise of the adjoint error-gradient run: each
    quantized store after a substep adds a uniform rounding error of width

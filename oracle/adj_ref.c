@@ -4,7 +4,9 @@
  * Reverse mode through one substep s_t -> s_{t+1} of mpm_solver.py:
  * p2g (:800-833), grid_op (:633-665), g2p (:835-858), the functions the
  * reference differentiates with Taichi's autodiff in
- * nlogn_loop_forward_grad (:1167-1188), and the error-contribution sums of
+ * nlogn_loop_forward_grad (:1167-1198), with the SDF collider's
+ * sdf_collision_no_arg (:1476-1504) when one is given, and the
+ * error-contribution sums of
  * accu_grad / get_gradients (:1232-1302): g_c = sum over the intermediate
  * states t = 1..T-1 and particles p of (dL/ds_{t,p,c})^2, in f64.
  * Loss: compute_ek (:1043-1052) or a linear functional of the final state.
@@ -158,8 +160,8 @@ static void adj_cauchy_back(const orc_mpm *s, const double *F /* J or F row majo
   for (int i = 0; i < 4; i++) jf_bar[i] += Fb[i];
 }
 
-void orc_adj_substep_f64(const orc_mpm *s, const double *st, int64_t n, const double *adj_next,
-                         double *adj, double *work) {
+void orc_adj_substep_col_f64(const orc_mpm *s, const orc_collider *col, const double *st, int64_t n,
+                             const double *adj_next, double *adj, double *work) {
   const int d = s->dim, nc = orc_num_comps(s);
   const int cV = d, cC = 2 * d, cJF = 2 * d + d * d, njf = nc - cJF;
   const int64_t nodes = adj_nodes(s);
@@ -174,6 +176,7 @@ void orc_adj_substep_f64(const orc_mpm *s, const double *st, int64_t n, const do
   orc_p2g_f64(s, st, n, gm, gmv);
   memcpy(gv, gmv, sizeof(double) * (size_t)nodes * (size_t)d);
   orc_grid_op_f64(s, gm, gv);
+  if (col && d == 3) orc_collide_f64(s, col, gv);
   memset(gbar, 0, sizeof(double) * (size_t)nodes * (size_t)(d + 1));
   memset(adj, 0, sizeof(double) * (size_t)nc * (size_t)n);
 
@@ -253,8 +256,16 @@ void orc_adj_substep_f64(const orc_mpm *s, const double *st, int64_t n, const do
     for (int i = 0; i < njf; i++) adj[(int64_t)(cJF + i) * n + p] = jfb[i];
   }
 
-  /* grid update backward */
-  const int lb = s->bound, ub = s->n_grid - s->bound, ng = s->n_grid;
+  /* grid update backward (walls per axis, mpm_solver.py:97-105), after the
+     collider's: sdf_collision_no_arg (:1476-1504) moves a node inside the
+     collider whose velocity after the walls approaches it (v_rel . n < 0)
+     to v - (v_rel . n) n, Jacobian I - n n^T */
+  int lb[3], ub[3];
+  for (int a = 0; a < 3; a++) {
+    lb[a] = s->axis_bounds && a < d ? s->lb[a] : s->bound;
+    ub[a] = s->axis_bounds && a < d ? s->ub[a] : s->n_grid - s->bound;
+  }
+  const int ng = s->n_grid;
   const double gdt = s->dt * s->gravity;
   for (int64_t node = 0; node < nodes; node++) {
     int I[3];
@@ -273,16 +284,34 @@ void orc_adj_substep_f64(const orc_mpm *s, const double *st, int64_t n, const do
     v[1] -= gdt;
     double gb[3];
     for (int a = 0; a < d; a++) gb[a] = gbar[(int64_t)a * nodes + node];
+    double vw[3] = {v[0], v[1], d == 3 ? v[2] : 0.0};
+    int wall = 0;
     if (s->boundary == ORC_STICKY) {
-      int wall = 0;
       for (int a = 0; a < d; a++)
-        if (I[a] < lb || I[a] > ub) wall = 1;
+        if (I[a] < lb[a] || I[a] > ub[a]) wall = 1;
+      if (wall)
+        for (int a = 0; a < d; a++) vw[a] = 0.0;
+    } else {
+      for (int a = 0; a < d; a++) {
+        if (I[a] < lb[a] && v[a] < 0) vw[a] = 0.0;
+        if (I[a] > ub[a] && v[a] > 0) vw[a] = 0.0;
+      }
+    }
+    double nrm[3], cv[3];
+    if (col && d == 3 && orc_collider_probe_f64(s, col, I, nrm, cv)) {
+      const double dn = (vw[0] - cv[0]) * nrm[0] + (vw[1] - cv[1]) * nrm[1] + (vw[2] - cv[2]) * nrm[2];
+      if (dn < 0.0) {
+        const double g = gb[0] * nrm[0] + gb[1] * nrm[1] + gb[2] * nrm[2];
+        for (int a = 0; a < 3; a++) gb[a] -= g * nrm[a];
+      }
+    }
+    if (s->boundary == ORC_STICKY) {
       if (wall)
         for (int a = 0; a < d; a++) gb[a] = 0.0;
     } else {
       for (int a = 0; a < d; a++) {
-        if (I[a] < lb && v[a] < 0) gb[a] = 0.0;
-        if (I[a] > ub && v[a] > 0) gb[a] = 0.0;
+        if (I[a] < lb[a] && v[a] < 0) gb[a] = 0.0;
+        if (I[a] > ub[a] && v[a] > 0) gb[a] = 0.0;
       }
     }
     double mb = 0.0;
@@ -365,6 +394,11 @@ void orc_adj_substep_f64(const orc_mpm *s, const double *st, int64_t n, const do
     adj_cauchy_back(s, jf, sb, jfb);
     for (int i = 0; i < njf; i++) adj[(int64_t)(cJF + i) * n + p] += jfb[i];
   }
+}
+
+void orc_adj_substep_f64(const orc_mpm *s, const double *st, int64_t n, const double *adj_next, double *adj,
+                         double *work) {
+  orc_adj_substep_col_f64(s, NULL, st, n, adj_next, adj, work);
 }
 
 int64_t orc_adj_work_doubles(const orc_mpm *s) { return adj_nodes(s) * (int64_t)(2 * s->dim + 2 + s->dim); }

@@ -122,6 +122,7 @@ def lib():
         L.orc_adj_work_doubles.restype = ctypes.c_int64
         L.orc_adj_work_doubles.argtypes = [P]
         L.orc_adj_substep_f64.argtypes = [P, P, ctypes.c_int64, P, P, P]
+        L.orc_adj_substep_col_f64.argtypes = [P, P, P, ctypes.c_int64, P, P, P]
         L.orc_adjoint_run_f64.restype = ctypes.c_double
         L.orc_adjoint_run_f64.argtypes = [P, P, ctypes.c_int64, i32, i32, P, P, P]
         _lib = L
@@ -301,17 +302,26 @@ def collider_state(dt, dx, step, omega=5.0, height_rate=0.15, lag=1):
     return theta, int(height / dx + 0.5), (omega if step >= 1 else 0.0)
 
 
-def collide(s, gv, sdf, normal, theta, yshift, omega, center, offset=(0, 0, 0), from_rot=False):
-    """sdf_collision on gv [3, nodes] in place (after grid_op).  sdf
-    [sx, sy, sz] and normal [sx, sy, sz, 3] float32."""
+def collider(sdf, normal, theta, yshift, omega, center, offset=(0, 0, 0), from_rot=False):
+    """The collider of one substep as the C struct; the arrays it points to
+    (float32 copies of sdf [sx, sy, sz] and normal [sx, sy, sz, 3]) are kept
+    on it."""
     sdf = np.ascontiguousarray(sdf, dtype=np.float32)
     normal = np.ascontiguousarray(normal, dtype=np.float32)
     c = Collider()
     c.sdf, c.normal = sdf.ctypes.data, normal.ctypes.data
+    c._keep = (sdf, normal)
     for a in range(3):
         c.size[a], c.offset[a], c.center[a] = sdf.shape[a], int(offset[a]), int(center[a])
     c.from_rot, c.yshift = 1 if from_rot else 0, int(yshift)
     c.cos_t, c.sin_t, c.omega = float(np.cos(theta)), float(np.sin(theta)), float(omega)
+    return c
+
+
+def collide(s, gv, sdf, normal, theta, yshift, omega, center, offset=(0, 0, 0), from_rot=False):
+    """sdf_collision on gv [3, nodes] in place (after grid_op).  sdf
+    [sx, sy, sz] and normal [sx, sy, sz, 3] float32."""
+    c = collider(sdf, normal, theta, yshift, omega, center, offset, from_rot)
     fn = lib().orc_collide_f64 if gv.dtype == np.float64 else lib().orc_collide_f32
     fn(ctypes.byref(s), ctypes.byref(c), _ptr(gv))
 
@@ -359,13 +369,18 @@ def limits(s, state):
     return mx, mn
 
 
-def adj_substep(s, state, adj_next):
-    """dL/ds_t from dL/ds_{t+1} for one plain f64 substep from `state` [nc, n]."""
+def adj_substep(s, state, adj_next, col=None):
+    """dL/ds_t from dL/ds_{t+1} for one plain f64 substep from `state` [nc, n];
+    col: the substep's collider (collider(...)) or None."""
     st = np.ascontiguousarray(state, dtype=np.float64)
     an = np.ascontiguousarray(adj_next, dtype=np.float64)
     out = np.zeros_like(st)
     work = np.zeros(lib().orc_adj_work_doubles(ctypes.byref(s)), dtype=np.float64)
-    lib().orc_adj_substep_f64(ctypes.byref(s), _ptr(st), st.shape[1], _ptr(an), _ptr(out), _ptr(work))
+    if col is None:
+        lib().orc_adj_substep_f64(ctypes.byref(s), _ptr(st), st.shape[1], _ptr(an), _ptr(out), _ptr(work))
+    else:
+        lib().orc_adj_substep_col_f64(ctypes.byref(s), ctypes.byref(col), _ptr(st), st.shape[1], _ptr(an),
+                                      _ptr(out), _ptr(work))
     return out
 
 

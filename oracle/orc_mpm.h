@@ -57,6 +57,10 @@ typedef struct orc_collider {
 /* sdf_collision on grid velocities gv after orc_grid_op (3D, every node) */
 void orc_collide_f32(const orc_mpm *s, const orc_collider *c, float *grid_v);
 void orc_collide_f64(const orc_mpm *s, const orc_collider *c, double *grid_v);
+/* the collider at node I (3D): 1 when inside, with the rotated normal and
+   the collider velocity */
+int orc_collider_probe_f32(const orc_mpm *s, const orc_collider *c, const int I[3], float nrm[3], float cv[3]);
+int orc_collider_probe_f64(const orc_mpm *s, const orc_collider *c, const int I[3], double nrm[3], double cv[3]);
 
 int orc_num_comps(const orc_mpm *s);
 
@@ -86,6 +90,10 @@ void orc_run_quant_f32(const orc_mpm *s, const orc_comp *comps, int nwords,
 int64_t orc_adj_work_doubles(const orc_mpm *s);
 void orc_adj_substep_f64(const orc_mpm *s, const double *st, int64_t n, const double *adj_next,
                          double *adj, double *work);
+/* the same with the SDF collider of the substep (NULL: none) after the grid
+   update */
+void orc_adj_substep_col_f64(const orc_mpm *s, const orc_collider *col, const double *st, int64_t n,
+                             const double *adj_next, double *adj, double *work);
 /* nsteps plain f64 substeps from state0 and their adjoint: returns the loss
    (kind 0: compute_ek; kind 1: sum(dir * s_T)), adj0 = dL/ds_0 and
    gsum[c] = sum over states t = 1..nsteps-1 and particles of (dL/ds_{t,p,c})^2. */

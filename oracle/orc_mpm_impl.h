@@ -184,45 +184,57 @@ void FN(orc_grid_op)(const orc_mpm *s, const REAL *gm, REAL *gv) {
   }
 }
 
-/* sdf_collision (mpm_solver.py:1459-1521) over every node of the dense 3D
-   grid, with rot_scene (:1524-1545): the node's offset from the centre
-   rotated by -theta, shifted down by the height, truncated and clamped to
-   the walls; sdf (:1447-1452): inside where the stored value < -1e-5
-   (outside the volume the reference's field holds 0). */
-void FN(orc_collide)(const orc_mpm *s, const orc_collider *c, REAL *gv) {
-  const int ng = s->n_grid;
-  const int64_t nodes = (int64_t)ng * ng * ng;
+/* The collider at node I: rot_scene (mpm_solver.py:1524-1545), the node's
+   offset from the centre rotated by -theta, shifted down by the height,
+   truncated and clamped to the walls; sdf (:1447-1452): inside where the
+   stored value < -1e-5 (outside the volume the reference's field holds 0).
+   Returns 1 when inside, with the normal rotated by theta (nrm) and the
+   collider velocity omega e_y x r (cv). */
+int FN(orc_collider_probe)(const orc_mpm *s, const orc_collider *c, const int I[3], REAL nrm[3], REAL cv[3]) {
   int lb[3], ub[3];
   FN(walls)(s, lb, ub);
   const REAL ct = (REAL)c->cos_t, st = (REAL)c->sin_t, om = (REAL)c->omega, dx = (REAL)s->dx;
+  int idx[3] = {I[0], I[1], I[2]};
+  if (idx[0] >= 2 * c->center[0]) idx[0] -= 2 * c->center[0];
+  if (idx[2] >= 2 * c->center[2]) idx[2] -= 2 * c->center[2];
+  const REAL o0 = (REAL)(idx[0] - c->center[0]), o2 = (REAL)(idx[2] - c->center[2]);
+  /* cos(-theta) o0 + sin(-theta) o2 + C0 + 0.5; -sin(-theta) o0 + cos(-theta) o2 + C2 + 0.5 */
+  const REAL x = ct * o0 + (-st) * o2 + (REAL)c->center[0] + (REAL)0.5;
+  const REAL z = -(-st) * o0 + ct * o2 + (REAL)c->center[2] + (REAL)0.5;
+  int r[3] = {(int)x, I[1] - c->yshift, (int)z};
+  for (int a = 0; a < 3; a++) {
+    if (r[a] > ub[a]) r[a] = ub[a];
+    if (r[a] < lb[a]) r[a] = lb[a];
+  }
+  int q[3];
+  for (int a = 0; a < 3; a++) {
+    q[a] = r[a] - c->offset[a];
+    if (q[a] < 0 || q[a] >= c->size[a]) return 0;
+  }
+  const int64_t e = ((int64_t)q[0] * c->size[1] + q[1]) * c->size[2] + q[2];
+  if (!(c->sdf[e] < -1e-5f)) return 0;
+  const REAL g0 = (REAL)c->normal[3 * e], g1 = (REAL)c->normal[3 * e + 1], g2 = (REAL)c->normal[3 * e + 2];
+  nrm[0] = ct * g0 + st * g2;
+  nrm[1] = g1;
+  nrm[2] = -st * g0 + ct * g2;
+  const int p0 = c->from_rot ? r[0] : I[0], p2 = c->from_rot ? r[2] : I[2];
+  const REAL rr[3] = {(REAL)(p0 - c->center[0]) * dx, (REAL)0, (REAL)(p2 - c->center[2]) * dx};
+  /* omega x r with omega = (0, om, 0) */
+  cv[0] = om * rr[2] - (REAL)0 * rr[1];
+  cv[1] = (REAL)0 * rr[0] - (REAL)0 * rr[2];
+  cv[2] = (REAL)0 * rr[1] - om * rr[0];
+  return 1;
+}
+
+/* sdf_collision (mpm_solver.py:1459-1521) over every node of the dense 3D
+   grid: v = cv + (v_rel - min(v_rel . n, 0) n), v_rel = v - cv. */
+void FN(orc_collide)(const orc_mpm *s, const orc_collider *c, REAL *gv) {
+  const int ng = s->n_grid;
+  const int64_t nodes = (int64_t)ng * ng * ng;
   for (int64_t node = 0; node < nodes; node++) {
     const int I[3] = {(int)(node / ((int64_t)ng * ng)), (int)((node / ng) % ng), (int)(node % ng)};
-    int idx[3] = {I[0], I[1], I[2]};
-    if (idx[0] >= 2 * c->center[0]) idx[0] -= 2 * c->center[0];
-    if (idx[2] >= 2 * c->center[2]) idx[2] -= 2 * c->center[2];
-    const REAL o0 = (REAL)(idx[0] - c->center[0]), o2 = (REAL)(idx[2] - c->center[2]);
-    /* cos(-theta) o0 + sin(-theta) o2 + C0 + 0.5; -sin(-theta) o0 + cos(-theta) o2 + C2 + 0.5 */
-    const REAL x = ct * o0 + (-st) * o2 + (REAL)c->center[0] + (REAL)0.5;
-    const REAL z = -(-st) * o0 + ct * o2 + (REAL)c->center[2] + (REAL)0.5;
-    int r[3] = {(int)x, I[1] - c->yshift, (int)z};
-    for (int a = 0; a < 3; a++) {
-      if (r[a] > ub[a]) r[a] = ub[a];
-      if (r[a] < lb[a]) r[a] = lb[a];
-    }
-    int q[3], in = 1;
-    for (int a = 0; a < 3; a++) {
-      q[a] = r[a] - c->offset[a];
-      if (q[a] < 0 || q[a] >= c->size[a]) in = 0;
-    }
-    if (!in) continue;
-    const int64_t e = ((int64_t)q[0] * c->size[1] + q[1]) * c->size[2] + q[2];
-    if (!(c->sdf[e] < -1e-5f)) continue;
-    const REAL g0 = (REAL)c->normal[3 * e], g1 = (REAL)c->normal[3 * e + 1], g2 = (REAL)c->normal[3 * e + 2];
-    const REAL nrm[3] = {ct * g0 + st * g2, g1, -st * g0 + ct * g2};
-    const int p0 = c->from_rot ? r[0] : I[0], p2 = c->from_rot ? r[2] : I[2];
-    const REAL rr[3] = {(REAL)(p0 - c->center[0]) * dx, (REAL)0, (REAL)(p2 - c->center[2]) * dx};
-    /* omega x r with omega = (0, om, 0) */
-    const REAL cv[3] = {om * rr[2] - (REAL)0 * rr[1], (REAL)0 * rr[0] - (REAL)0 * rr[2], (REAL)0 * rr[1] - om * rr[0]};
+    REAL nrm[3], cv[3];
+    if (!FN(orc_collider_probe)(s, c, I, nrm, cv)) continue;
     REAL vr[3];
     for (int a = 0; a < 3; a++) vr[a] = gv[(int64_t)a * nodes + node] - cv[a];
     const REAL dn = vr[0] * nrm[0] + vr[1] * nrm[1] + vr[2] * nrm[2];
