@@ -82,7 +82,7 @@ class SimDesc(ctypes.Structure):
                 ("la_0", ctypes.c_double), ("seed", ctypes.c_uint64),
                 ("comps", Component * AQ_MAX_COMPS),
                 ("axis_bounds", ctypes.c_int32), ("lb", ctypes.c_int32 * 3), ("ub", ctypes.c_int32 * 3),
-                ("bound_clip", ctypes.c_int32)]
+                ("bound_clip", ctypes.c_int32), ("compute_f64", ctypes.c_int32)]
 
 
 class Collider(ctypes.Structure):
@@ -106,7 +106,7 @@ EXPORTS = [
     "aq_update_limits", "aq_advance_limits", "aq_energy", "aq_sim_grid", "aq_advance_timed",
     "aq_adj_workspace_bytes", "aq_adj_substep", "aq_noise_add", "aq_noise_dot", "aq_codec_selftest", "aq_mlp_forward",
     "aq_mlp_backward", "aq_adam_step", "aq_sim_specialize", "aq_jit_build", "aq_mlp_backward_noalloc",
-    "aq_sim_count_wraps", "aq_group_workspace_bytes", "aq_group_create", "aq_group_advance", "aq_group_destroy",
+    "aq_sim_count_wraps", "aq_sim_pack", "aq_sim_unpack", "aq_group_workspace_bytes", "aq_group_create", "aq_group_advance", "aq_group_destroy",
     "aq_sim_set_collider",
 ]
 
@@ -159,6 +159,8 @@ def lib():
     f32 = ctypes.c_float
     L.aq_mlp_backward.argtypes = [P, P, P, P, P, i32, f32, P, P, P, P]
     L.aq_sim_count_wraps.argtypes = [P, P]
+    L.aq_sim_pack.argtypes = [P, P, i64, u32, P, P]
+    L.aq_sim_unpack.argtypes = [P, P, i64, P, P]
     L.aq_mlp_backward_noalloc.argtypes = [P, P, P, P, P, i32, f32, P, P, P]
     L.aq_adam_step.argtypes = [P, P, P, P, i32, i32, f32, f32, f32, f32, P]
     L.aq_group_workspace_bytes.restype = i64

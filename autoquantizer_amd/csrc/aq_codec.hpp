@@ -59,6 +59,9 @@ struct CodecK {
   uint32_t stream;       // fields lie in encode order at increasing offsets (StreamSink applies)
   uint32_t pad;
   unsigned long long *wraps;  // per component: stores whose code wrapped (null: not counted)
+  // f64 compute: per component {1.0 / scale, scale, offset, 0} as doubles
+  // (CompK holds them rounded to float); null for f32 compute
+  const double *cd;
 };
 
 // Descriptors are loaded where they are used, a few at a time: the empty asm
@@ -233,6 +236,54 @@ __device__ __forceinline__ float dec_fixed(uint32_t q, const CompK &c) {
   return c.offs == 0.0f ? t : t + c.offs;
 }
 
+// ---- f64 compute: fixed-point and int members encoded and decoded in
+// double (float_to_custom_int / reconstruct_custom_float with compute type
+// f64, codegen_llvm_quant.cpp:44-80, 624-641); custom floats keep f32
+// compute (a CustomFloatType with an exponent computes in f32,
+// TS/taichi/ir/type.cpp:129-155).
+// v_cvt_i32_f64 / v_cvt_u32_f64 truncate toward zero, clamp to the integer
+// range and turn NaN into 0, as cvt_sat does for f32.
+__device__ __forceinline__ uint32_t cvt_sat_f64(double y, bool is_signed) {
+  uint32_t r;
+  if (is_signed)
+    asm("v_cvt_i32_f64 %0, %1" : "=v"(r) : "v"(y));
+  else
+    asm("v_cvt_u32_f64 %0, %1" : "=v"(r) : "v"(y));
+  return r;
+}
+
+// y = (v - offset) * (1 / scale) [+ u - 1/2] + copysign(1/2, .), all in
+// double; returns the saturated integer of y.
+__device__ __forceinline__ uint32_t enc_fixed_sat_f64(double v, const CompK &c, const double *cd, double u,
+                                                      double &y) {
+  y = v - cd[2];
+  y = y * cd[0];
+  if (c.dither) {
+    y = y + u;
+    y = y - 0.5;
+  }
+  y = y + __builtin_copysign(0.5, y);
+  return cvt_sat_f64(y, c.is_signed);
+}
+
+__device__ __forceinline__ double dec_fixed_f64(uint32_t q, const CompK &c, const double *cd) {
+  const double cf = c.is_signed ? (double)sext(q, c.nbits) : (double)(q & lowmask(c.nbits));
+  const double t = cf * cd[1];
+  return cd[2] == 0.0 ? t : t + cd[2];
+}
+
+__device__ __forceinline__ bool code_wraps_f64(double y, uint32_t q, uint32_t nb, bool is_signed) {
+  if (y != y) return true;
+  if (is_signed) {
+    if (nb >= 32u) return y >= 2147483648.0 || y < -2147483648.0;
+    const uint32_t sh = 32u - nb;
+    return (int32_t)(q << sh) >> sh != (int32_t)q;
+  }
+  if (y <= -1.0) return true;
+  if (nb >= 32u) return y >= 4294967296.0;
+  return (q >> nb) != 0u;
+}
+
 __device__ __forceinline__ uint32_t f32_exp(float f) {
   return (__float_as_uint(f) >> 23) & 0xFFu;
 }
@@ -340,6 +391,17 @@ __device__ __forceinline__ float decode_with(G &&get, const CompK &c) {
   return c.shared ? dec_shared(e, q, c) : dec_float(e, q, c);
 }
 
+// The same in f64 compute: cd = CodecK::cd + 4 c.
+template <class G>
+__device__ __forceinline__ double decode_with_f64(G &&get, const CompK &c, const double *cd) {
+  if (c.kind == K_FIXED) return dec_fixed_f64(get(c.off, c.nbits), c, cd);
+  if (c.kind == K_INT) {
+    const uint32_t q = get(c.off, c.nbits);
+    return c.is_signed ? (double)sext(q, c.nbits) : (double)q;
+  }
+  return (double)decode_with(get, c);
+}
+
 // Decode component c of a record held in registers.
 template <int NW32>
 __device__ __forceinline__ float decode_comp(const uint32_t (&r)[NW32], const CompK &c) {
@@ -355,6 +417,10 @@ __device__ __forceinline__ uint32_t row_get(const uint32_t *row, uint32_t off, u
 
 __device__ __forceinline__ float decode_row(const uint32_t *row, const CompK &c) {
   return decode_with([&](uint32_t off, uint32_t nb) { return row_get(row, off, nb); }, c);
+}
+
+__device__ __forceinline__ double decode_row_f64(const uint32_t *row, const CompK &c, const double *cd) {
+  return decode_with_f64([&](uint32_t off, uint32_t nb) { return row_get(row, off, nb); }, c, cd);
 }
 
 // Encode one member (fixed, own-exponent float, f32 or int) into a row.
@@ -516,5 +582,78 @@ __device__ __forceinline__ void encode_record(const float (&v)[NC], const CodecK
   if (count_wraps)
     for (uint32_t m = wrapped; m; m &= m - 1u) atomicAdd(cod.wraps + __builtin_ctz(m), 1ull);
 }
+
+#ifndef AQ_JIT_LAYOUT
+// encode_record in f64 compute: fixed-point and int members from the double
+// values (dithering u as in f32 compute, exact in double), unquantized f32
+// members and custom floats from the values rounded to float (custom floats
+// compute in f32).  dec[] gets the decoded values in double.
+template <int NC, class Sink>
+__device__ __forceinline__ void encode_record_f64(const double (&v)[NC], const CodecK &cod, uint32_t slot,
+                                                  Sink &sink, double (&dec)[NC]) {
+  uint32_t sm = cod.shared_mask, gf = cod.gfirst_mask, gl = cod.glast_mask, kb = cod.key_base;
+  asm volatile("" : "+s"(sm), "+s"(gf), "+s"(gl), "+s"(kb));
+  uint32_t gmax[NC];
+  uint32_t run = 0u;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    gmax[c] = 0u;
+    if ((sm >> c) & 1u) {
+      const uint32_t e = f32_exp((float)v[c]);
+      run = ((gf >> c) & 1u) ? e : (run > e ? run : e);
+      gmax[c] = run;
+    }
+  }
+#pragma unroll
+  for (int c = NC - 1; c >= 0; --c) {
+    if ((sm >> c) & 1u) {
+      run = ((gl >> c) & 1u) ? gmax[c] : run;
+      gmax[c] = run;
+    }
+  }
+  const bool count_wraps = cod.wraps != nullptr;
+  uint32_t wrapped = 0u;
+  ConstU32 *table = nullptr;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (c % COMP_BATCH == 0) table = comp_table(cod);
+    const CompK k = comp_load(table, c);
+    const double *cd = cod.cd + 4 * c;
+    if ((sm >> c) & 1u) {
+      const uint32_t e = shared_exp_field(gmax[c], k);
+      if ((gf >> c) & 1u) sink.put(k.eoff, k.exp_bits, e);
+      const uint32_t d = enc_shared((float)v[c], gmax[c], k);
+      sink.put(k.off, k.nbits, d);
+      dec[c] = (double)dec_shared_enc(e, d, k);
+    } else if (k.kind == K_FIXED) {
+      const double u = k.dither ? (double)dither_u(slot, hash32((uint32_t)c >> 1 ^ kb), c) : 0.0;
+      double y;
+      const uint32_t qs = enc_fixed_sat_f64(v[c], k, cd, u, y);
+      const uint32_t q = qs & lowmask(k.nbits);
+      if (count_wraps && code_wraps_f64(y, qs, k.nbits, k.is_signed)) wrapped |= 1u << c;
+      sink.put(k.off, k.nbits, q);
+      dec[c] = dec_fixed_f64(q, k, cd);
+    } else if (k.kind == K_FLOAT) {
+      uint32_t e;
+      const uint32_t d = enc_float((float)v[c], k, e);
+      sink.put(k.eoff, k.exp_bits, e);
+      sink.put(k.off, k.nbits, d);
+      dec[c] = (double)dec_float(e, d, k);
+    } else if (k.kind == K_F32) {
+      const float f = (float)v[c];
+      sink.put(k.off, 32u, __float_as_uint(f));
+      dec[c] = (double)f;
+    } else {
+      const uint32_t qs = cvt_sat_f64(v[c], k.is_signed);
+      const uint32_t q = qs & lowmask(k.nbits);
+      if (count_wraps && code_wraps_f64(v[c], qs, k.nbits, k.is_signed)) wrapped |= 1u << c;
+      sink.put(k.off, k.nbits, q);
+      dec[c] = k.is_signed ? (double)sext(q, k.nbits) : (double)q;
+    }
+  }
+  if (count_wraps)
+    for (uint32_t m = wrapped; m; m &= m - 1u) atomicAdd(cod.wraps + __builtin_ctz(m), 1ull);
+}
+#endif
 
 }  // namespace aq

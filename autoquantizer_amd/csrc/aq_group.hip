@@ -126,6 +126,7 @@ extern "C" int aq_group_create(aq_sim *const *sims, void *const *states, const i
     const aq_sim *s = sims[i];
     if (!s) return fail(AQ_ERR_INVALID, "null simulation handle");
     if (s->d.storage != AQ_STORE_QUANT) return fail(AQ_ERR_INVALID, "group members need quantized storage");
+    if (s->d.compute_f64) return fail(AQ_ERR_INVALID, "group members compute in f32");
     if (s->has_collider) return fail(AQ_ERR_INVALID, "group members cannot have an SDF collider");
     if (ns[i] < 1 || ns[i] > s->d.capacity) return fail(AQ_ERR_CAPACITY, "particle count outside [1, capacity]");
     if (!states[i]) return fail(AQ_ERR_INVALID, "null state");
@@ -244,10 +245,10 @@ extern "C" int aq_group_destroy(aq_group *g) {
 
 template <class F>
 static void dispatch_class(int dim, int mat, int bucket, F &&f) {
-  if (dim == 3 && mat == AQ_FLUID) dispatch_quant<3, AQ_FLUID>(bucket, f);
-  else if (dim == 3) dispatch_quant<3, AQ_ELASTIC>(bucket, f);
-  else if (mat == AQ_FLUID) dispatch_quant<2, AQ_FLUID>(bucket, f);
-  else dispatch_quant<2, AQ_ELASTIC>(bucket, f);
+  if (dim == 3 && mat == AQ_FLUID) dispatch_quant<3, AQ_FLUID, float>(bucket, f);
+  else if (dim == 3) dispatch_quant<3, AQ_ELASTIC, float>(bucket, f);
+  else if (mat == AQ_FLUID) dispatch_quant<2, AQ_FLUID, float>(bucket, f);
+  else dispatch_quant<2, AQ_ELASTIC, float>(bucket, f);
 }
 
 static int group_sort(aq_group *g, hipStream_t st) {

@@ -844,6 +844,9 @@ static int check_desc(const aq_sim_desc *d) {
       if (d->lb[a] < 0 || d->lb[a] >= d->ub[a] || d->ub[a] > d->n_grid)
         return fail(AQ_ERR_INVALID, "per-axis walls need 0 <= lb < ub <= n_grid");
   if (d->bound_clip != 0 && d->bound_clip != 1) return fail(AQ_ERR_INVALID, "bound_clip must be 0 or 1");
+  if (d->compute_f64 != 0 && d->compute_f64 != 1) return fail(AQ_ERR_INVALID, "compute_f64 must be 0 or 1");
+  if (d->compute_f64 && (d->storage != AQ_STORE_QUANT || d->dim != 2))
+    return fail(AQ_ERR_INVALID, "compute_f64 applies to quantized storage in 2D");
   return AQ_OK;
 }
 
@@ -870,7 +873,7 @@ enum {
 static void ws_plan(const aq_sim_desc *d, int64_t *offs /*[WS_N]*/, int64_t *total) {
   const int64_t nblocks = blocks_of(d);
   const int64_t nodes = nblocks * 64;
-  const int64_t node_bytes = d->storage == AQ_STORE_F64 ? 32 : 16;
+  const int64_t node_bytes = f64_compute(*d) ? 32 : 16;
   const int64_t cap = d->capacity;
   const int64_t state_bytes = d->storage == AQ_STORE_QUANT ? cap * 8 * d->nwords
                               : cap * d->ncomp * (d->storage == AQ_STORE_F64 ? 8 : 4);
@@ -893,7 +896,7 @@ static void ws_plan(const aq_sim_desc *d, int64_t *offs /*[WS_N]*/, int64_t *tot
   sz[WS_NEAR] = nblocks * 4;
   sz[WS_TILES] = (int64_t)max_items_of(d) * tile_slots(d) * node_bytes;
   sz[WS_LIMITS] = (int64_t)kLimitBlocks * 2 * AQ_MAX_COMPS * 4;  // partial max / min of k_limits
-  sz[WS_LAYOUT] = AQ_MAX_COMPS * (int64_t)sizeof(CompK);
+  sz[WS_LAYOUT] = AQ_MAX_COMPS * (int64_t)(sizeof(CompK) + 4 * sizeof(double));  // CompK table, f64 factors
   int64_t o = 0;
   for (int i = 0; i < WS_N; ++i) {
     offs[i] = o;
@@ -967,7 +970,7 @@ extern "C" int aq_sim_create(const aq_sim_desc *d, void *ws, int64_t ws_bytes, a
   s->pd = make_params<double>(d);
   s->nblocks = (int32_t)blocks_of(d);
   s->nodes = (int64_t)s->nblocks * 64;
-  s->node_bytes = d->storage == AQ_STORE_F64 ? 32 : 16;
+  s->node_bytes = f64_compute(*d) ? 32 : 16;
   char *base = (char *)ws;
   s->grid_v = base + offs[WS_GRID_V];
   s->grid_f = base + offs[WS_GRID_F];
@@ -991,6 +994,16 @@ extern "C" int aq_sim_create(const aq_sim_desc *d, void *ws, int64_t ws_bytes, a
   s->near = (uint32_t *)(base + offs[WS_NEAR]);
   s->lim_part = (float *)(base + offs[WS_LIMITS]);
   s->lay_dev = (CompK *)(base + offs[WS_LAYOUT]);
+  s->lay_f64 = (double *)(s->lay_dev + AQ_MAX_COMPS);
+  double lf[AQ_MAX_COMPS * 4];
+  std::memset(lf, 0, sizeof(lf));
+  if (d->storage == AQ_STORE_QUANT)
+    for (int c = 0; c < d->ncomp; ++c) {
+      const double sc = d->comps[c].scale;
+      lf[4 * c + 0] = sc != 0.0 ? 1.0 / sc : 0.0;  // float_to_custom_int's 1 / scale, in the compute type
+      lf[4 * c + 1] = sc;
+      lf[4 * c + 2] = d->comps[c].offset;
+    }
   s->max_items = max_items_of(d);
   s->have_items = false;
   s->items_state = nullptr;
@@ -999,6 +1012,7 @@ extern "C" int aq_sim_create(const aq_sim_desc *d, void *ws, int64_t ws_bytes, a
   // grids and block flags start at zero; the layout table goes to the device
   hipError_t e = hipMemset(ws, 0, (size_t)offs[WS_KEY]);
   if (e == hipSuccess) e = hipMemcpy(s->lay_dev, s->lay.c, sizeof(s->lay.c), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(s->lay_f64, lf, sizeof(lf), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     delete s;
@@ -1060,6 +1074,7 @@ CodecK make_codec(const aq_sim *s, uint32_t step) {
   c.stream = s->stream_order ? 1u : 0u;
   c.pad = 0u;
   c.wraps = s->wraps;
+  c.cd = s->d.storage == AQ_STORE_QUANT && s->d.compute_f64 ? s->lay_f64 : nullptr;
   return c;
 }
 
@@ -1251,6 +1266,7 @@ extern "C" int aq_sim_specialize(aq_sim *s, const char *src_dir, const char *inc
   }
   if (!src_dir) return AQ_OK;  // back to the generic kernel
   if (s->d.storage != AQ_STORE_QUANT) return fail(AQ_ERR_INVALID, "specialisation applies to quantized storage");
+  if (s->d.compute_f64) return fail(AQ_ERR_INVALID, "the specialised kernel computes in f32");
   std::vector<char> code;
   int rc = jit_code(*s, src_dir, include_dir, cache_dir, code, nullptr);
   if (rc) return rc;

@@ -57,6 +57,10 @@ struct StateK {
   int32_t pad;
 };
 
+// Whether the compute type is double (a type trait that hipRTC sources can use too).
+template <class T> struct IsF64 { static constexpr bool value = false; };
+template <> struct IsF64<double> { static constexpr bool value = true; };
+
 // Noise key of a particle: its id when ids are tracked, else its slot.
 __device__ __forceinline__ uint32_t noise_slot(const StateK &s, int64_t p) {
   return (s.dither_any && s.ids) ? (uint32_t)s.ids[p] : (uint32_t)p;
@@ -135,7 +139,10 @@ __device__ __forceinline__ void finish_state(const StateK &s, const CodecK &cod,
         continue;
       }
       if (!G2P_ONLY && c % COMP_BATCH == 0) table = comp_table(cod);
-      v[c] = (T)decode_row(row, comp_load(table, c));
+      if constexpr (IsF64<T>::value)
+        v[c] = decode_row_f64(row, comp_load(table, c), cod.cd + 4 * c);  // f64 compute
+      else
+        v[c] = (T)decode_row(row, comp_load(table, c));
     }
   }
 }
@@ -155,6 +162,31 @@ __device__ __forceinline__ void load_state(const StateK &s, const CodecK &cod, i
 template <int ST, int NW32, int NC, class T>
 __device__ __forceinline__ void store_state(const StateK &s, const CodecK &cod, int64_t p, T (&v)[NC],
                                             uint32_t *row) {
+#ifndef AQ_JIT_LAYOUT
+  if constexpr (ST == AQ_STORE_QUANT && IsF64<T>::value) {
+    // f64 compute of quantized records (the specialised kernel is f32 only)
+    uint64_t *w = (uint64_t *)s.ptr;
+    double dec[NC];
+    if (cod.stream) {
+      StreamSink sink{w, s.cap, p, (uint32_t)s.nw, 0u, 0ull};
+      encode_record_f64<NC>(v, cod, noise_slot(s, p), sink, dec);
+      sink.finish();
+    } else {
+#pragma unroll
+      for (int i = 0; i < NW32; ++i) row[i] = 0u;
+      RowSink sink{row};
+      encode_record_f64<NC>(v, cod, noise_slot(s, p), sink, dec);
+#pragma unroll
+      for (int i = 0; i < NW32 / 2; ++i)
+        if (i < s.nw)
+          __builtin_nontemporal_store((uint64_t)row[2 * i] | ((uint64_t)row[2 * i + 1] << 32),
+                                      w + (int64_t)i * s.cap + p);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = dec[c];
+    return;
+  }
+#endif
   if constexpr (ST == AQ_STORE_QUANT) {
     float f[NC], dec[NC];
 #pragma unroll

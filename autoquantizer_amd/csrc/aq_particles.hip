@@ -43,3 +43,64 @@ int launch_g2p(aq_sim *s, void *state, int64_t n, const int32_t *ids, uint32_t s
   LAUNCH_CHECK();
   return AQ_OK;
 }
+
+// ---- records <-> values in the simulation's compute type -------------------
+template <int DIM, int MAT, int ST, int NW32, class T>
+__global__ void __launch_bounds__(256) k_sim_pack(StateK s, CodecK cod, const T *__restrict__ vals, int64_t n) {
+  __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  constexpr int NC = Comps<DIM, MAT>::NC;
+  T v[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) v[c] = vals[(int64_t)c * n + p];
+  store_state<ST, NW32, NC, T>(s, cod, p, v, rows + threadIdx.x * RowsOf<ST, NW32>::RS);
+}
+
+template <int DIM, int MAT, int ST, int NW32, class T>
+__global__ void __launch_bounds__(256) k_sim_unpack(StateK s, CodecK cod, T *__restrict__ vals, int64_t n) {
+  __shared__ uint32_t rows[256 * RowsOf<ST, NW32>::RS];
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  constexpr int NC = Comps<DIM, MAT>::NC;
+  T v[NC];
+  load_state<DIM, ST, NW32, NC, T>(s, cod, p, 0xFFFFFFFFu, rows + threadIdx.x * RowsOf<ST, NW32>::RS, v);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) vals[(int64_t)c * n + p] = v[c];
+}
+
+extern "C" int aq_sim_pack(aq_sim *s, const void *vals, int64_t n, uint32_t step, uint64_t *words, void *stream) {
+  int rc = check_n(s, words, n);
+  if (rc) return rc;
+  if (s->d.storage != AQ_STORE_QUANT) return fail(AQ_ERR_INVALID, "records belong to quantized storage");
+  if (n && !vals) return fail(AQ_ERR_INVALID, "null values");
+  if (n == 0) return AQ_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const StateK sk = make_state(s, words, nullptr);
+  const CodecK cod = make_codec(s, step);
+  dispatch(s, [&](auto tag_) {
+    TAG_TYPES(tag_)
+    hipLaunchKernelGGL((k_sim_pack<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, cod,
+                       (const T *)vals, n);
+  });
+  LAUNCH_CHECK();
+  return AQ_OK;
+}
+
+extern "C" int aq_sim_unpack(aq_sim *s, const uint64_t *words, int64_t n, void *vals, void *stream) {
+  int rc = check_n(s, words, n);
+  if (rc) return rc;
+  if (s->d.storage != AQ_STORE_QUANT) return fail(AQ_ERR_INVALID, "records belong to quantized storage");
+  if (n && !vals) return fail(AQ_ERR_INVALID, "null values");
+  if (n == 0) return AQ_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const StateK sk = make_state(s, words, nullptr);
+  const CodecK cod = make_codec(s, 0u);
+  dispatch(s, [&](auto tag_) {
+    TAG_TYPES(tag_)
+    hipLaunchKernelGGL((k_sim_unpack<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, cod,
+                       (T *)vals, n);
+  });
+  LAUNCH_CHECK();
+  return AQ_OK;
+}

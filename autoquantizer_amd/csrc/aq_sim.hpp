@@ -77,6 +77,7 @@ struct aq_sim {
   aq_sim_desc d;
   LayoutK lay;
   CompK *lay_dev;  // lay.c in the workspace, read by the MPM kernels
+  double *lay_f64; // f64 compute: {1 / scale, scale, offset, 0} per component (workspace)
   ParamsT<float> pf;
   ParamsT<double> pd;
   int64_t nodes;
@@ -119,23 +120,33 @@ template <int D_, int M_, int S_, int W_, class T_> struct Tag {
   using T = T_;
 };
 
-template <int D_, int M_, class F>
+template <int D_, int M_, class T_, class F>
 static inline void dispatch_quant(int bucket, F &&f) {
-  if (bucket == 4) f(Tag<D_, M_, AQ_STORE_QUANT, 4, float>{});
-  else if (bucket == 8) f(Tag<D_, M_, AQ_STORE_QUANT, 8, float>{});
-  else if (bucket == 16) f(Tag<D_, M_, AQ_STORE_QUANT, 16, float>{});
-  else f(Tag<D_, M_, AQ_STORE_QUANT, 32, float>{});
+  if (bucket == 4) f(Tag<D_, M_, AQ_STORE_QUANT, 4, T_>{});
+  else if (bucket == 8) f(Tag<D_, M_, AQ_STORE_QUANT, 8, T_>{});
+  else if (bucket == 16) f(Tag<D_, M_, AQ_STORE_QUANT, 16, T_>{});
+  else f(Tag<D_, M_, AQ_STORE_QUANT, 32, T_>{});
+}
+
+// Whether the simulation computes in double: plain f64 storage, or
+// quantized records with compute_f64 (2D only).
+static inline bool f64_compute(const aq_sim_desc &d) {
+  return d.storage == AQ_STORE_F64 || (d.storage == AQ_STORE_QUANT && d.compute_f64);
 }
 
 template <class F>
 static inline void dispatch(const aq_sim *s, F &&f) {
   const aq_sim_desc &d = s->d;
-  if (d.storage == AQ_STORE_QUANT) {
+  if (d.storage == AQ_STORE_QUANT && d.compute_f64) {
     const int b = nw32_bucket(d.nwords);
-    if (d.dim == 3 && d.material == AQ_FLUID) dispatch_quant<3, AQ_FLUID>(b, f);
-    else if (d.dim == 3) dispatch_quant<3, AQ_ELASTIC>(b, f);
-    else if (d.material == AQ_FLUID) dispatch_quant<2, AQ_FLUID>(b, f);
-    else dispatch_quant<2, AQ_ELASTIC>(b, f);
+    if (d.material == AQ_FLUID) dispatch_quant<2, AQ_FLUID, double>(b, f);
+    else dispatch_quant<2, AQ_ELASTIC, double>(b, f);
+  } else if (d.storage == AQ_STORE_QUANT) {
+    const int b = nw32_bucket(d.nwords);
+    if (d.dim == 3 && d.material == AQ_FLUID) dispatch_quant<3, AQ_FLUID, float>(b, f);
+    else if (d.dim == 3) dispatch_quant<3, AQ_ELASTIC, float>(b, f);
+    else if (d.material == AQ_FLUID) dispatch_quant<2, AQ_FLUID, float>(b, f);
+    else dispatch_quant<2, AQ_ELASTIC, float>(b, f);
   } else if (d.storage == AQ_STORE_F32) {
     if (d.dim == 3 && d.material == AQ_FLUID) f(Tag<3, AQ_FLUID, AQ_STORE_F32, 4, float>{});
     else if (d.dim == 3) f(Tag<3, AQ_ELASTIC, AQ_STORE_F32, 4, float>{});
@@ -152,7 +163,7 @@ static inline void dispatch(const aq_sim *s, F &&f) {
 // Dispatch over (dim, compute type) for the grid kernels.
 template <class F>
 static inline void dispatch_grid(const aq_sim *s, F &&f) {
-  if (s->d.storage == AQ_STORE_F64) {
+  if (f64_compute(s->d)) {
     if (s->d.dim == 3) f(Tag<3, 0, 0, 0, double>{});
     else f(Tag<2, 0, 0, 0, double>{});
   } else {

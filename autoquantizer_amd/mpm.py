@@ -185,12 +185,16 @@ class MPMSolver:
         self.ncomp = len(self.names)
         if types is None and cfg.use_quantaichi:
             types, groups = reference_layout(cfg)
+        self.compute_f64 = False
         if types is not None:
             if str(cfg.compute_fp) in ("f64", "float64"):
-                # init_quantaichi (mpm_solver.py:372-374) computes in cfg.compute_fp, f64 by default; the
-                # quantized kernels here compute in f32
-                warnings.warn("quantized storage computes in f32; cfg.compute_fp='f64' applies to plain storage only",
-                              stacklevel=3)
+                # init_quantaichi (mpm_solver.py:372-374) computes in cfg.compute_fp, f64 by default (the
+                # reference's 2D runs); quantized records compute in f64 here in 2D
+                if self.dim == 2:
+                    self.compute_f64 = True
+                else:
+                    warnings.warn("3D quantized storage computes in f32; cfg.compute_fp='f64' applies to 2D "
+                                  "quantized and to plain storage", stacklevel=3)
             self.storage = _lib.AQ_STORE_QUANT
             if groups is None:
                 groups = [("pack", 0, self.names)]
@@ -227,6 +231,7 @@ class MPMSolver:
             for a in range(self.dim):
                 d.lb[a], d.ub[a] = self.lb[a], self.ub[a]
         d.bound_clip = 1 if self.bound_clip else 0
+        d.compute_f64 = 1 if self.compute_f64 else 0
         for k, c in enumerate(self.comps):
             for key, val in c.items():
                 setattr(d.comps[k], key, val)
@@ -258,12 +263,17 @@ class MPMSolver:
         """Compile the fused substep kernel for this solver's record layout
         (hipRTC, aq_sim_specialize) and use it for every later advance();
         code objects are cached in cache_dir (default: AQ_JIT_CACHE or a
-        directory under the system temp dir).  Quantized storage only."""
+        directory under the system temp dir).  Quantized storage only; with
+        f64 compute the generic kernel stays (the specialised one is f32):
+        returns whether the kernel was specialised."""
         if self.storage != _lib.AQ_STORE_QUANT:
             raise ValueError("specialisation applies to quantized storage")
+        if self.compute_f64:
+            return False
         cache = _lib.jit_cache_dir(cache_dir)
         _lib.check(_lib.lib().aq_sim_specialize(self._h, _lib.CSRC_DIR.encode(), _lib.INCLUDE_DIR.encode(),
                                                 cache.encode()))
+        return True
 
     def unspecialize(self):
         """Back to the generic (descriptor-driven) fused kernel."""
@@ -279,7 +289,12 @@ class MPMSolver:
         if n > self.capacity:
             raise ValueError("more particles than capacity")
         self.current_idx = n
-        if self.storage == _lib.AQ_STORE_QUANT:
+        if self.storage == _lib.AQ_STORE_QUANT and self.compute_f64:
+            # encoded in double, as the kernels of an f64-compute simulation store
+            v = torch.as_tensor(np.ascontiguousarray(vals[:, :n], dtype=np.float64), device=self.device)
+            _lib.check(_lib.lib().aq_sim_pack(self._h, _ptr(v), n, int(dither_step) & 0xFFFFFFFF, _ptr(self.state),
+                                              _stream()))
+        elif self.storage == _lib.AQ_STORE_QUANT:
             v = torch.as_tensor(np.ascontiguousarray(vals[:, :n], dtype=np.float32), device=self.device)
             tmp = torch.zeros((self.nwords, n), dtype=torch.int64, device=self.device)
             _lib.check(_lib.lib().aq_pack(self._carr, self.ncomp, self.nwords, _ptr(v), None,
@@ -316,7 +331,10 @@ class MPMSolver:
     def get_state(self, original_order=True):
         """Decoded state [ncomp, n] as float64 numpy (particle order of set_state)."""
         n = self.current_idx
-        if self.storage == _lib.AQ_STORE_QUANT:
+        if self.storage == _lib.AQ_STORE_QUANT and self.compute_f64:
+            vals = torch.empty((self.ncomp, n), dtype=torch.float64, device=self.device)
+            _lib.check(_lib.lib().aq_sim_unpack(self._h, _ptr(self.state), n, _ptr(vals), _stream()))
+        elif self.storage == _lib.AQ_STORE_QUANT:
             w = self.state[:, :n].contiguous()
             vals = torch.empty((self.ncomp, n), dtype=torch.float32, device=self.device)
             _lib.check(_lib.lib().aq_unpack(self._carr, self.ncomp, self.nwords, _ptr(w), _ptr(vals), n,
@@ -493,7 +511,7 @@ class MPMSolver:
         g = ctypes.c_void_p()
         nodes = ctypes.c_int64()
         _lib.check(_lib.lib().aq_sim_grid(self._h, ctypes.byref(g), ctypes.byref(nodes)))
-        dt = torch.float64 if self.storage == _lib.AQ_STORE_F64 else torch.float32
+        dt = torch.float64 if self.storage == _lib.AQ_STORE_F64 or self.compute_f64 else torch.float32
         esz = 8 if dt == torch.float64 else 4
         off = g.value - self.workspace.data_ptr()
         nbytes = nodes.value * 4 * esz

@@ -83,13 +83,14 @@ def clamp_bits(bits):
     return np.clip(np.rint(np.asarray(bits, dtype=np.float64)), 1, 32).astype(np.int32)
 
 
-def quantized_solver(plain, bits, ranges, dither=True, use_bitpack=True, seed=0, capacity=None):
+def quantized_solver(plain, bits, ranges, dither=True, use_bitpack=True, seed=0, capacity=None, compute_fp="f32"):
     """The quantized solver of a scheme, as the reference's run(quant=True)
     builds it (run_mpm_elastic_2d.py:186-191 -> init_quantaichi,
     mpm_solver.py:364-455): the plain solver's scene with every component a
     signed fixed-point type of bits[c] (clamped to [1, 32]) and range
     ranges[c] (J with offset 1), in one bit_pack (or the reference's
-    bit_struct words), dithered.  Computes in f32."""
+    bit_struct words), dithered.  compute_fp: "f64" computes in double (the
+    reference's 2D runs, 2D only), "f32" in float."""
     from .mpm import MPMSolver
     cfg = copy.copy(plain.param)
     cfg.use_quantaichi = True
@@ -97,7 +98,7 @@ def quantized_solver(plain, bits, ranges, dither=True, use_bitpack=True, seed=0,
     cfg.use_dithering = bool(dither)
     cfg.quant_bits = clamp_bits(bits)
     cfg.quant_ranges = np.asarray(ranges, dtype=np.float64)
-    cfg.compute_fp = "f32"
+    cfg.compute_fp = compute_fp
     cfg.seed = int(seed)
     return MPMSolver(cfg, capacity=capacity or plain.capacity, device=plain.device)
 
@@ -124,13 +125,15 @@ def success_fraction(losses, ref_loss, eps):
     return float(np.mean(d < 3.0 * eps * abs(ref_loss)))
 
 
-def validate_scheme(plain, init_state, nsteps, bits, ranges, ref_loss, eps, runs=20, sort_every=16, seed=0):
+def validate_scheme(plain, init_state, nsteps, bits, ranges, ref_loss, eps, runs=20, sort_every=16, seed=0,
+                    compute_fp="f32"):
     """The reference's experiment for one allocation: `runs` dithered
-    quantized runs of the scene, their losses against the f64 loss, and the
-    success fraction.  Returns dict(success, losses, rel_error (each run's
-    |loss - f64| / f64), bits (clamped), seconds)."""
+    quantized runs of the scene (compute in compute_fp), their losses
+    against the f64 loss, and the success fraction.  Returns dict(success,
+    losses, rel_error (each run's |loss - f64| / f64), bits (clamped),
+    seconds)."""
     t0 = time.perf_counter()
-    q = quantized_solver(plain, bits, ranges, seed=seed)
+    q = quantized_solver(plain, bits, ranges, seed=seed, compute_fp=compute_fp)
     L = quantized_losses(q, init_state, nsteps, runs, sort_every=sort_every)
     torch.cuda.synchronize()
     return dict(success=success_fraction(L, ref_loss, eps), losses=L, rel_error=np.abs(L - ref_loss) / abs(ref_loss),

@@ -260,8 +260,9 @@ def reference_search(eps_list=(0.1, 0.01), runs=20, checkpoint_every=64):
     blocks, 80,000 particles, 128^2, dt 2e-4, 8192 substeps; FP64 loss
     6893.409): ranges (-r) and error-contribution sums (-a) from FP64 runs,
     LagrangianEB at each eps (the script's default 0.1, and 0.01) with
-    bits + 1, then `runs` dithered quantized runs (f32 compute) judged by
-    |loss - f64| < 3 eps f64."""
+    bits + 1, then `runs` dithered quantized runs judged by
+    |loss - f64| < 3 eps f64: computing in f64 as the reference's 2D runs
+    do (init_quantaichi's compute_fp), and in f32 beside it."""
     import torch
     from autoquantizer_amd.scenes import ELASTIC_JELLY_SUBSTEPS, elastic_jelly_2d
     from autoquantizer_amd.search import search_scheme, validate_scheme
@@ -282,13 +283,19 @@ def reference_search(eps_list=(0.1, 0.01), runs=20, checkpoint_every=64):
     for eps in eps_list:
         bits, _ = LagrangianEB(np.asarray(res["grads"]) * np.asarray(res["ranges"]) ** 2, ref=L64, tol_rel=eps)
         bits = bits + 1
-        val = validate_scheme(plain, init, nsteps, bits, res["ranges"], L64, eps, runs=runs)
+        val = validate_scheme(plain, init, nsteps, bits, res["ranges"], L64, eps, runs=runs, compute_fp="f64")
+        v32 = validate_scheme(plain, init, nsteps, bits, res["ranges"], L64, eps, runs=runs, compute_fp="f32")
         out["eps"][str(eps)] = {"bits_clamped": [int(b) for b in val["bits"]], "record_bits": val["record_bits"],
                                 "compression_vs_fp32": val["record_bits"] / (32.0 * len(bits)),
+                                "compute": "f64",
                                 "success": val["success"],
                                 "rel_error_median": float(np.median(val["rel_error"])),
                                 "rel_error_max": float(np.max(val["rel_error"])),
-                                "runs_seconds": val["seconds"]}
+                                "runs_seconds": val["seconds"],
+                                "f32_compute": {"success": v32["success"],
+                                                "rel_error_median": float(np.median(v32["rel_error"])),
+                                                "rel_error_max": float(np.max(v32["rel_error"])),
+                                                "runs_seconds": v32["seconds"]}}
     return out
 
 

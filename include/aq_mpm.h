@@ -161,6 +161,12 @@ typedef struct aq_sim_desc {
   /* use_bound_clip (mpm_solver.py:696-698): G2P clamps every new position
      component to [lb dx, ub dx]. */
   int32_t bound_clip;
+  /* AQ_STORE_QUANT in 2D: compute in f64 (init_quantaichi's compute type,
+     mpm_solver.py:372-374, f64 in the reference's 2D runs): fixed-point and
+     int members are encoded and decoded in double, custom floats keep f32
+     compute (as the reference's CustomFloatType); the grid is double.  The
+     run-time specialised kernel is f32 only (aq_sim_specialize refuses). */
+  int32_t compute_f64;
 } aq_sim_desc;
 
 typedef struct aq_sim aq_sim;
@@ -171,6 +177,17 @@ int64_t aq_sim_workspace_bytes(const aq_sim_desc *desc);
 int aq_sim_create(const aq_sim_desc *desc, void *workspace, int64_t workspace_bytes,
                   aq_sim **out);
 int aq_sim_destroy(aq_sim *sim);
+
+/* Records of particles [0, n) of a quantized simulation from values in its
+   compute type (float, or double with compute_f64), planes vals[c * n + p],
+   encoded as its kernels store them (dithering noise of `step`, keyed by
+   the slot p); words[w * capacity + p].  aq_sim_unpack decodes them back.
+   Replaces the Python-scope stores and loads of the reference's quantized
+   fields (x[s, p] = ..., codegen_llvm_quant.cpp:884-952, 529-784) in the
+   simulation's compute type; aq_pack / aq_unpack are the f32 codec of a
+   bare layout. */
+int aq_sim_pack(aq_sim *sim, const void *vals, int64_t n, uint32_t step, uint64_t *words, void *stream);
+int aq_sim_unpack(aq_sim *sim, const uint64_t *words, int64_t n, void *vals, void *stream);
 
 /* Run-time specialisation to the simulation's record layout (replaces the
    per-scheme kernel compilation of the reference's Taichi JIT,

@@ -394,10 +394,15 @@ void orc_unpack_range_f32(const orc_comp *comps, int nc, int nwords,
   }
 }
 
+/* f64 compute: fixed-point and int members in double; custom floats and
+   unquantized f32 members keep f32 (a CustomFloatType with an exponent
+   computes in f32, TS/taichi/ir/type.cpp:129-155). */
 static double decode_one_f64(const orc_comp *c, const uint64_t *rec) {
   uint32_t q = orc_get_bits(rec, c->bit_offset, c->nbits);
   if (c->kind == ORC_FIXED)
     return orc_fixed_decode_f64(q, c->nbits, c->is_signed, c->scale, c->offset);
+  if (c->kind == ORC_INT)
+    return c->is_signed ? (double)orc_sext(q, c->nbits) : (double)(q & mask32(c->nbits));
   return (double)decode_one(c, rec);
 }
 
@@ -424,6 +429,9 @@ void orc_pack_f64(const orc_comp *comps, int nc, int nwords, const double *vals,
                      orc_fixed_encode_f64(vals[(int64_t)k * n + p], c->scale,
                                           c->offset, c->nbits, c->is_signed,
                                           c->dither, u));
+      } else if (c->kind == ORC_INT) {
+        const double x = vals[(int64_t)k * n + p];
+        orc_put_bits(rec, c->bit_offset, c->nbits, c->is_signed ? (uint32_t)cvt_i32_f64(x) : cvt_u32_f64(x));
       } else {
         encode_one(c, rec, v[k], 0.0f);
       }

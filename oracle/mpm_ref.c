@@ -169,6 +169,25 @@ void orc_run_quant_f32(const orc_mpm *s, const orc_comp *comps, int nwords,
   }
 }
 
+/* whole substeps on quantized storage with f64 compute (the reference's 2D
+   quantized runs, init_quantaichi with compute_fp f64): decode in double,
+   substep in double, encode in double (orc_pack_f64). */
+void orc_run_quant_f64(const orc_mpm *s, const orc_comp *comps, int nwords,
+                       uint64_t *words, int64_t n, int nsteps, uint64_t seed,
+                       uint32_t step0, double *grid_m, double *grid_v,
+                       double *scratch, int nthreads) {
+  int nc = orc_num_comps(s);
+  double *st = scratch, *out = scratch + (size_t)nc * (size_t)n;
+  for (int t = 0; t < nsteps; t++) {
+    orc_unpack_f64(comps, nc, nwords, words, st, n);
+    zero_grid_f64(s, grid_m, grid_v);
+    par_p2g_f64(s, st, n, grid_m, grid_v, nthreads);
+    orc_grid_op_f64(s, grid_m, grid_v);
+    par_g2p_f64(s, st, grid_v, out, n, nthreads);
+    orc_pack_f64(comps, nc, nwords, out, NULL, seed, step0 + (uint32_t)t, words, n);
+  }
+}
+
 void orc_limits_f32(const orc_mpm *s, const float *state, int64_t n, float *mx, float *mn) {
   int nc = orc_num_comps(s);
   for (int c = 0; c < nc; c++) {

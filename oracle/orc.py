@@ -117,6 +117,8 @@ def lib():
         L.orc_run_quant_f32.restype = None
         L.orc_run_quant_f32.argtypes = [P, P, i32, P, ctypes.c_int64, i32, ctypes.c_uint64, u32,
                                         P, P, P, i32]
+        L.orc_run_quant_f64.argtypes = [P, P, i32, P, ctypes.c_int64, i32, ctypes.c_uint64, u32,
+                                        P, P, P, i32]
         L.orc_limits_f32.restype = None
         L.orc_limits_f32.argtypes = [P, P, ctypes.c_int64, P, P]
         L.orc_adj_work_doubles.restype = ctypes.c_int64
@@ -343,14 +345,14 @@ def run_plain(s, state, nsteps, nthreads=1):
     return gm, gv
 
 
-def run_quant(s, comps, nwords, words, nsteps, seed=0, step0=0, nthreads=1):
-    """In place on words [nwords, n]."""
+def run_quant(s, comps, nwords, words, nsteps, seed=0, step0=0, nthreads=1, f64=False):
+    """In place on words [nwords, n]; f64: compute in double."""
     ca = comps_array(comps)
     n = words.shape[1]
-    gm, gv = grids(s)
-    scratch = np.zeros(2 * len(comps) * n, dtype=np.float32)
-    lib().orc_run_quant_f32(ctypes.byref(s), ca, nwords, _ptr(words), n, nsteps, seed, step0,
-                            _ptr(gm), _ptr(gv), _ptr(scratch), nthreads)
+    gm, gv = grids(s, f64=f64)
+    scratch = np.zeros(2 * len(comps) * n, dtype=np.float64 if f64 else np.float32)
+    fn = lib().orc_run_quant_f64 if f64 else lib().orc_run_quant_f32
+    fn(ctypes.byref(s), ca, nwords, _ptr(words), n, nsteps, seed, step0, _ptr(gm), _ptr(gv), _ptr(scratch), nthreads)
     return gm, gv
 
 

@@ -84,6 +84,13 @@ void orc_run_quant_f32(const orc_mpm *s, const orc_comp *comps, int nwords,
                        uint32_t step0, float *grid_m, float *grid_v,
                        float *scratch /* 2 * nc * n floats */, int nthreads);
 
+/* the same with f64 compute: values decoded, stepped and encoded in double;
+   scratch holds 2 * nc * n doubles. */
+void orc_run_quant_f64(const orc_mpm *s, const orc_comp *comps, int nwords,
+                       uint64_t *words, int64_t n, int nsteps, uint64_t seed,
+                       uint32_t step0, double *grid_m, double *grid_v,
+                       double *scratch, int nthreads);
+
 /* Adjoint (reverse mode) of one plain f64 substep from state st:
    adj = dL/ds_t given adj_next = dL/ds_{t+1} (adj_ref.c); work holds
    orc_adj_work_doubles(s) doubles. */

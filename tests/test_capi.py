@@ -120,6 +120,20 @@ def test_sim_desc_validation():
     assert L.aq_sim_workspace_bytes(ctypes.byref(d)) < 0
     d.n_grid, d.capacity = 64, 1000
     assert L.aq_sim_workspace_bytes(ctypes.byref(d)) > 64 ** 3 * 16 * 2
+    # f64 compute belongs to quantized storage in 2D
+    d.compute_f64 = 1
+    assert L.aq_sim_workspace_bytes(ctypes.byref(d)) < 0
+    assert b"compute_f64" in L.aq_last_error()
+    d.storage, d.nwords = _lib.AQ_STORE_QUANT, 4
+    assert L.aq_sim_workspace_bytes(ctypes.byref(d)) < 0  # 3D
+    d2 = _lib.SimDesc()
+    d2.dim, d2.material, d2.n_grid, d2.ncomp, d2.storage, d2.nwords = 2, _lib.AQ_ELASTIC, 64, 12, _lib.AQ_STORE_QUANT, 3
+    d2.capacity, d2.compute_f64 = 1000, 1
+    w64 = L.aq_sim_workspace_bytes(ctypes.byref(d2))
+    d2.compute_f64 = 0
+    w32 = L.aq_sim_workspace_bytes(ctypes.byref(d2))
+    assert w64 > w32 > 0  # double grids and tiles
+    d.storage, d.nwords, d.compute_f64 = _lib.AQ_STORE_F32, 0, 0
     # creation also needs a positive particle mass (checked before the workspace)
     h = ctypes.c_void_p()
     d.p_mass = 0.0

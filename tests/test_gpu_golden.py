@@ -86,7 +86,8 @@ def test_reference_search_loop_on_its_scene():
     """run_mpm_elastic_2d.py's experiment end to end on its own scene:
     ranges (-r), error-contribution sums (-a), LagrangianEB at the script's
     default eps = 0.1 with bits + 1 (solve, :227-242), then dithered
-    quantized runs (-f, :186-208) judged by |loss - f64| < 3 eps f64.  The
+    quantized runs computing in f64 as the script's do (-f, :186-208)
+    judged by |loss - f64| < 3 eps f64.  The
     allocation is also the one the reference's shipped sums and ranges give,
     to within a bit per component."""
     from autoquantizer_amd.scenes import ELASTIC_JELLY_F64_LOSS, ELASTIC_JELLY_SUBSTEPS, elastic_jelly_2d
@@ -107,6 +108,6 @@ def test_reference_search_loop_on_its_scene():
           flush=True)
     assert np.all(np.abs(res["bits"] - (ref_bits + 1)) < 1.0)
     val = validate_scheme(make("f64"), st, ELASTIC_JELLY_SUBSTEPS, res["bits"], res["ranges"],
-                          ELASTIC_JELLY_F64_LOSS, eps, runs=4)
+                          ELASTIC_JELLY_F64_LOSS, eps, runs=4, compute_fp="f64")
     print("success", val["success"], "relative errors", val["rel_error"].tolist(), flush=True)
     assert val["success"] == 1.0

@@ -386,3 +386,31 @@ def test_elastic_3d_stress_closed_form(orc, case):
     sigma = A / (-dt * s.p_vol * 4 / dx ** 2)
     scale = max(np.abs(want).max(), mu)
     assert np.abs(sigma - want).max() <= 1e-9 * scale, (sigma, want)
+
+
+def test_quant_run_f64_matches_plain_then_quantize(orc):
+    """orc_run_quant_f64 (the reference's 2D quantized runs compute in f64)
+    is decode -> substep -> encode in double, dithered with the substep's
+    noise.  Its records also differ from an f32 encode of the same values
+    (28-bit codes of a range that is not a power of two resolve below f32's
+    rounding), so the check would fail if any stage fell back to f32."""
+    s = orc.mpm_params(2, orc.ELASTIC, 64, 1e-4, E=1e4, boundary=orc.STICKY)
+    n = 256
+    st = random_state(orc, s, n, seed=12, f64=True)
+    comps = []
+    off = 0
+    for c in range(12):
+        comps.append(dict(kind=orc.FIXED, nbits=28, is_signed=1, bit_offset=off, dither=1,
+                          scale=orc.fixed_scale(28, 7.26239, True)))
+        off += 28
+    nw = (off + 63) // 64
+    w = orc.pack(comps, nw, st, seed=3, step=4, f64=True)
+    dec = orc.unpack(comps, nw, w, f64=True)
+    assert np.abs(dec - st).max() <= comps[0]["scale"] * (1 + 1e-9)  # dithered: within one step
+    w2 = w.copy()
+    orc.run_quant(s, comps, nw, w2, 1, seed=3, step0=5, f64=True)
+    ref = dec.copy()
+    orc.run_plain(s, ref, 1)
+    assert np.array_equal(w2, orc.pack(comps, nw, ref, seed=3, step=5, f64=True))
+    w32 = orc.pack(comps, nw, ref.astype(np.float32), seed=3, step=5)
+    assert not np.array_equal(w2, w32)
