@@ -37,6 +37,7 @@ extern "C" int aq_adj_substep(aq_sim *s, uint32_t step, const void *state, int64
   bool tiled;
   if ((rc = launch_p2g(s, state, n, &tiled, st))) return rc;
   if ((rc = launch_gather(s, tiled, false, 0u, st))) return rc;
+  if ((rc = settle_list(s, st))) return rc;
   HIP_TRY(hipMemcpyAsync(pre, s->grid_v, (size_t)(s->nodes * s->node_bytes), hipMemcpyDeviceToDevice, st));
   if ((rc = aq_grid_op(s, step, stream))) return rc;
   const ColliderK K = collider_at(s, step);

@@ -378,13 +378,16 @@ __global__ void __launch_bounds__(256) k_gather(ParamsT<T> P, ColliderK K, int32
 
 // Gather over the block list of the last sort (tiled P2G): gcnt[0] blocks
 // of glist (near a particle or active at the sort, and those first reached
-// since), then the gcnt[1] blocks of flist (flagged by untiled scatters
+// since), then the gcnt[fslot] blocks of flist (flagged by untiled scatters
 // since the last gather) that glist does not hold.  A flagged block first
 // reached since the sort is appended to glist (at gcnt[3], the append
 // cursor), so that later gathers visit it and zero its grid once nothing
-// reaches it; k_glist_commit, launched after the gather, commits the
-// appended entries and empties flist.  (The last workgroup to finish doing
-// that instead costs 2048 serialised atomics on one address: +35 us.)
+// reaches it.  The commit (gcnt[0] = gcnt[3], gcnt[fslot] = 0) waits for
+// the next kernel: the next tiled kernel's first thread does it (the
+// kernels append into the other counter meanwhile), else k_glist_commit.
+// (Committing in the last workgroup to finish costs 2048 serialised
+// atomics on one address, +35 us; a kernel of its own after every gather
+// 3-4.5 us.)
 template <int DIM, class T, bool OP>
 __global__ void __launch_bounds__(256) k_gather_list(ParamsT<T> P, ColliderK K, const int32_t *__restrict__ item_off,
                                                      const uint32_t *__restrict__ near,
@@ -392,8 +395,9 @@ __global__ void __launch_bounds__(256) k_gather_list(ParamsT<T> P, ColliderK K, 
                                                      vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags,
                                                      vec4_t<T> *__restrict__ grid_v, uint32_t *__restrict__ act,
                                                      int32_t *__restrict__ glist, const int32_t *__restrict__ flist,
-                                                     uint32_t *__restrict__ gcnt, uint32_t *__restrict__ listed) {
-  const uint32_t gc = gcnt[0], fc = gcnt[1];
+                                                     uint32_t *__restrict__ gcnt, uint32_t *__restrict__ listed,
+                                                     int32_t fslot) {
+  const uint32_t gc = gcnt[0], fc = gcnt[fslot];
   const uint32_t nw = gridDim.x * 4u;
   const int lane = threadIdx.x & 63;
   for (uint32_t i = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * 4u + (threadIdx.x >> 6)));
@@ -415,11 +419,12 @@ __global__ void __launch_bounds__(256) k_gather_list(ParamsT<T> P, ColliderK K, 
   }
 }
 
-// After a list gather: the blocks it appended join glist; flist is empty.
-__global__ void k_glist_commit(uint32_t *__restrict__ gcnt) {
+// The commit of a list gather when no tiled kernel carries it: the blocks
+// it appended join glist; its flist counter (slot) is cleared.
+__global__ void k_glist_commit(uint32_t *__restrict__ gcnt, int32_t slot) {
   if (threadIdx.x == 0) {
     gcnt[0] = gcnt[3];
-    gcnt[1] = 0u;
+    gcnt[slot] = 0u;
   }
 }
 
@@ -1155,9 +1160,9 @@ static int jit_source(const aq_sim &v, std::string &src) {
                 "extern \"C\" __global__ void __launch_bounds__(%d) __attribute__((amdgpu_waves_per_eu(%d)))\n"
                 "aq_jit_tiled(aq::StateK s, aq::CodecK cod, aq::ParamsT<float> P, int64_t n, const int4 *items,\n"
                 "             const int32_t *nitems, const float4 *grid_v, float4 *tiles, float4 *grid_f,\n"
-                "             uint32_t *fflags, int32_t *flist, uint32_t *fcnt) {\n"
+                "             uint32_t *fflags, int32_t *flist, uint32_t *gcnt, uint32_t ctl) {\n"
                 "  aq::tiled_body<%d, %d, AQ_STORE_QUANT, %d, float, true, true>(s, cod, P, n, items, nitems, grid_v,\n"
-                "                                                               tiles, grid_f, fflags, flist, fcnt);\n}\n",
+                "                                                               tiles, grid_f, fflags, flist, gcnt, ctl);\n}\n",
                 kItem, kJitWaves, v.d.dim, v.d.material, nw32_bucket(v.d.nwords));
   src += buf;
   return AQ_OK;
@@ -1309,6 +1314,16 @@ ColliderK collider_at(const aq_sim *s, uint32_t step) {
   return K;
 }
 
+int settle_list(aq_sim *s, hipStream_t st) {
+  if (s->commit_pending) {
+    hipLaunchKernelGGL(k_glist_commit, dim3(1), dim3(64), 0, st, s->gcnt, (int32_t)s->commit_pending);
+    s->commit_pending = 0;
+    LAUNCH_CHECK();
+  }
+  s->list_parity = 0;
+  return AQ_OK;
+}
+
 // gather grid in list mode: enough wavefronts for the blocks of a compact
 // scene, a loop over the list beyond that
 static unsigned lblocks(const aq_sim *s) {
@@ -1320,6 +1335,11 @@ int launch_gather(aq_sim *s, bool tiled, bool op, uint32_t step, hipStream_t st)
   const int32_t *ioff = tiled ? s->item_off : nullptr;
   const ColliderK K = collider_at(s, step);
   if (tiled) {
+    if (s->commit_pending) {  // two list gathers with no tiled kernel between them
+      hipLaunchKernelGGL(k_glist_commit, dim3(1), dim3(64), 0, st, s->gcnt, (int32_t)s->commit_pending);
+      s->commit_pending = 0;
+    }
+    const int32_t fslot = 1 + s->list_parity;
     dispatch_grid(s, [&](auto tag_) {
       TAG_TYPES(tag_)
       (void)MAT;
@@ -1329,14 +1349,15 @@ int launch_gather(aq_sim *s, bool tiled, bool op, uint32_t step, hipStream_t st)
         hipLaunchKernelGGL((k_gather_list<DIM, T, true>), dim3(lblocks(s)), dim3(256), 0, st, params_of<T>(s), K,
                            ioff, s->near, (const vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags,
                            (vec4_t<T> *)s->grid_v, s->act, s->glist, (const int32_t *)s->flist, s->gcnt,
-                           s->listed);
+                           s->listed, fslot);
       else
         hipLaunchKernelGGL((k_gather_list<DIM, T, false>), dim3(lblocks(s)), dim3(256), 0, st, params_of<T>(s), K,
                            ioff, s->near, (const vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags,
                            (vec4_t<T> *)s->grid_v, s->act, s->glist, (const int32_t *)s->flist, s->gcnt,
-                           s->listed);
+                           s->listed, fslot);
     });
-    hipLaunchKernelGGL(k_glist_commit, dim3(1), dim3(64), 0, st, s->gcnt);
+    s->commit_pending = (uint32_t)fslot;
+    s->list_parity ^= 1;
     LAUNCH_CHECK();
     return AQ_OK;
   }
@@ -1364,7 +1385,8 @@ extern "C" int aq_p2g(aq_sim *s, const void *state, int64_t n, void *stream) {
   hipStream_t st = (hipStream_t)stream;
   bool tiled;
   if ((rc = launch_p2g(s, state, n, &tiled, st))) return rc;
-  return launch_gather(s, tiled, false, 0u, st);
+  if ((rc = launch_gather(s, tiled, false, 0u, st))) return rc;
+  return settle_list(s, st);
 }
 
 extern "C" int aq_grid_op(aq_sim *s, uint32_t step, void *stream) {
@@ -1443,8 +1465,11 @@ extern "C" int aq_sort(aq_sim *s, void *state, int64_t n, int32_t *ids, void *st
     hipLaunchKernelGGL(k_near<3>, dim3((m + 255) / 256), dim3(256), 0, st, s->scount, (int32_t)s->pf.nb, s->near, m);
   else
     hipLaunchKernelGGL(k_near<2>, dim3((m + 255) / 256), dim3(256), 0, st, s->scount, (int32_t)s->pf.nb, s->near, m);
-  // the gather list of the new order (counters cleared: glist, flist, cursor)
+  // the gather list of the new order (counters cleared: glist, both flist
+  // counters, cursor)
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, s->gcnt, 0u, (int64_t)16);
+  s->commit_pending = 0;
+  s->list_parity = 0;
   hipLaunchKernelGGL(k_glist_build, dim3((m + 255) / 256), dim3(256), 0, st, (const uint32_t *)s->near,
                      (const uint32_t *)s->act, s->listed, s->glist, s->gcnt, m);
   LAUNCH_CHECK();
@@ -1556,6 +1581,7 @@ static int advance_impl(aq_sim *s, void *state, int64_t n, int nsteps, uint32_t 
   }
   TIMED(tm, Timer::ENDS, rc = launch_g2p(s, state, n, ids, step0 + (uint32_t)(nsteps - 1), st));
   if (rc) return rc;
+  if ((rc = settle_list(s, st))) return rc;
   if (mxmn) rc = launch_limits(s, state, n, mxmn, st);
   return rc;
 }

@@ -86,10 +86,16 @@ struct aq_sim {
   void *grid_v, *grid_f, *tiles;
   uint32_t *fflags, *act;
   // gather list (tiled path): gcnt[0] blocks in glist (near a particle or
-  // active at the last sort, plus blocks first reached since), gcnt[1]
-  // blocks in flist (flagged by untiled scatters since the last gather),
-  // gcnt[3] the append cursor of glist; listed[b]: b is in glist
+  // active at the last sort, plus blocks first reached since); gcnt[1] and
+  // gcnt[2] count the blocks in flist that untiled scatters flagged, for
+  // the next gather of parity 0 and 1 (tiled kernels append into the next
+  // gather's counter while the last gather's counter waits to be cleared);
+  // gcnt[3] the append cursor of glist; listed[b]: b is in glist.
+  // After a list gather, commit_pending is the counter it consumed: the next
+  // tiled kernel (or k_glist_commit) sets gcnt[0] = gcnt[3] and clears it.
   uint32_t *gcnt, *listed;
+  int32_t list_parity = 0;     // parity of the next list gather
+  uint32_t commit_pending = 0; // counter slot to clear at the next commit (0: none)
   int32_t *glist, *flist;
   uint32_t *skey, *srank, *scount, *ccount, *soff, *spart;
   void *tmp_state;
@@ -200,6 +206,16 @@ static inline int check_n(const aq_sim *s, const void *state, int64_t n) {
   return AQ_OK;
 }
 
+// Control word of a tiled kernel of one simulation: bits 0-1 the counter
+// slot its untiled-scatter appends go to (the next gather's), bits 2-3 the
+// slot to clear while committing the last gather's appends (0: none), which
+// the kernel's first thread does before any append.
+static inline uint32_t take_list_ctl(aq_sim *s) {
+  const uint32_t ctl = (uint32_t)(1 + s->list_parity) | (s->commit_pending << 2);
+  s->commit_pending = 0;
+  return ctl;
+}
+
 static inline bool items_valid(const aq_sim *s, const void *state, int64_t n) {
   return s->have_items && s->items_state == state && s->items_n == n;
 }
@@ -210,6 +226,11 @@ CodecK make_codec(const aq_sim *s, uint32_t step);
 int launch_gather(aq_sim *s, bool tiled, bool op, uint32_t step, hipStream_t st);
 // the collider of substep `step` (off when none is set or in 2D)
 ColliderK collider_at(const aq_sim *s, uint32_t step);
+// Ends a call's use of the gather list: commits the last list gather if no
+// tiled kernel did, then both flist counters are zero and the next gather
+// takes parity 0 (every call leaves this state, so that a captured call
+// replays on the state it was recorded from).
+int settle_list(aq_sim *s, hipStream_t st);
 // aq_particles.hip
 int launch_p2g(aq_sim *s, const void *state, int64_t n, bool *tiled, hipStream_t st);
 int launch_g2p(aq_sim *s, void *state, int64_t n, const int32_t *ids, uint32_t step, hipStream_t st);

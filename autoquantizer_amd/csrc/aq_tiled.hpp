@@ -475,7 +475,15 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
                                            const int4 *__restrict__ items, const int32_t *__restrict__ nitems,
                                            const vec4_t<T> *__restrict__ grid_v, vec4_t<T> *__restrict__ tiles,
                                            vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags,
-                                           int32_t *flist, uint32_t *fcnt) {
+                                           int32_t *flist, uint32_t *gcnt, uint32_t ctl) {
+  // commit of the last list gather's appends (no thread of this kernel
+  // reads gcnt[0] or the cleared counter; appends go to the other one)
+  const uint32_t commit = (ctl >> 2) & 3u;
+  if (commit && blockIdx.x == 0 && threadIdx.x == 0) {
+    gcnt[0] = gcnt[3];
+    gcnt[commit] = 0u;
+  }
+  uint32_t *fcnt = flist ? gcnt + (ctl & 3u) : nullptr;
   const OneSim<T> src{{s, cod, P, n, items, grid_v, tiles, grid_f, fflags, flist, fcnt}, *nitems};
   tiled_run<DIM, MAT, ST, NW32, T, DO_G2P, DO_P2G>(src);
 }
@@ -484,9 +492,10 @@ template <int DIM, int MAT, int ST, int NW32, class T, bool DO_G2P, bool DO_P2G>
 __global__ void __launch_bounds__(kItem) __attribute__((amdgpu_waves_per_eu(TiledWaves<ST>::W)))
 k_tiled(StateK s, CodecK cod, ParamsT<T> P, int64_t n, const int4 *__restrict__ items,
         const int32_t *__restrict__ nitems, const vec4_t<T> *__restrict__ grid_v, vec4_t<T> *__restrict__ tiles,
-        vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags, int32_t *flist, uint32_t *fcnt) {
+        vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags, int32_t *flist, uint32_t *gcnt,
+        uint32_t ctl) {
   tiled_body<DIM, MAT, ST, NW32, T, DO_G2P, DO_P2G>(s, cod, P, n, items, nitems, grid_v, tiles, grid_f, fflags,
-                                                    flist, fcnt);
+                                                    flist, gcnt, ctl);
 }
 
 // The fused G2P -> P2G substep of a group of quantized simulations (one
