@@ -20,11 +20,13 @@ int launch_g2p2g(aq_sim *s, void *state, int64_t n, const int32_t *ids, uint32_t
     uint32_t *ff = s->fflags;
     int32_t *fl = s->flist;
     uint32_t *gc = s->gcnt;
+    uint32_t *bm = s->bmargin + (int64_t)s->list_parity * s->nblocks;
     uint32_t ctl = take_list_ctl(s);
-    void *args[] = {&sk, &cod, &P, &n, &items, &nitems, &gv, &tl, &gf, &ff, &fl, &gc, &ctl};
+    void *args[] = {&sk, &cod, &P, &n, &items, &nitems, &gv, &tl, &gf, &ff, &fl, &gc, &ctl, &bm};
     HIP_TRY(hipModuleLaunchKernel(s->jit_fn, tblocks(s, n), 1, 1, kItem, 1, 1, 0, st, args, nullptr));
     return AQ_OK;
   }
+  uint32_t *bm = s->bmargin + (int64_t)s->list_parity * s->nblocks;  // the next gather's flags
   const uint32_t ctl = *tiled ? take_list_ctl(s) : 0u;
   dispatch(s, [&](auto tag_) {
     TAG_TYPES(tag_)
@@ -32,7 +34,7 @@ int launch_g2p2g(aq_sim *s, void *state, int64_t n, const int32_t *ids, uint32_t
       hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, true, true>), dim3(tblocks(s, n)), dim3(kItem), 0, st, sk,
                          cod, params_of<T>(s), n, s->items, s->item_off + s->nblocks,
                          (const vec4_t<T> *)s->grid_v, (vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags,
-                         s->flist, s->gcnt, ctl);
+                         s->flist, s->gcnt, ctl, bm);
     else
       hipLaunchKernelGGL((k_g2p2g<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, cod,
                          params_of<T>(s), n, (const vec4_t<T> *)s->grid_v, (vec4_t<T> *)s->grid_f, s->fflags);

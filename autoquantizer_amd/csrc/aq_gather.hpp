@@ -125,15 +125,23 @@ template <class T> struct GatherArgs {
 };
 
 // Gather of block b (uniform over the wavefront) of one simulation.
+// bm_read (null: every tile whole): per block, bit i set when its item i
+// wrote its tile's drift margin (bit 31: items 31 and later, which always
+// do); the margin nodes of the other items were not written and are not
+// read.  bm_clear (null: none): the flags the next tiled kernel sets,
+// cleared here for block b (every block with items is gathered).
 template <int DIM, class T, bool OP>
 __device__ __forceinline__ void gather_block(const ParamsT<T> &P, const ColliderK &K, int32_t b,
                                              const int32_t *__restrict__ item_off,
                                              const uint32_t *__restrict__ near, const vec4_t<T> *__restrict__ tiles,
                                              vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags,
-                                             vec4_t<T> *__restrict__ grid_v, uint32_t *__restrict__ act) {
+                                             vec4_t<T> *__restrict__ grid_v, uint32_t *__restrict__ act,
+                                             const uint32_t *__restrict__ bm_read = nullptr,
+                                             uint32_t *__restrict__ bm_clear = nullptr) {
   using TL = Tile<DIM>;
   constexpr int E = TL::E, TN = TL::TN, NN = 1 << DIM;
   const int lane = threadIdx.x & 63;
+  if (bm_clear && lane == 0) bm_clear[b] = 0u;
   const bool fall = fflags[b] != 0u;
   // a block with no items in its neighbourhood and no untiled scatter has
   // nothing to gather: skip the item ranges of its neighbours
@@ -152,26 +160,36 @@ __device__ __forceinline__ void gather_block(const ParamsT<T> &P, const Collider
   block_coords<DIM>(b, nb, bc);
   lane_coords<DIM>(lane, loc);
   int32_t r0[NN], r1[NN], q[NN];
+  bool mg[NN];  // the node is in the drift margin of direction k's tiles
+  uint32_t bf[NN];
 #pragma unroll
   for (int k = 0; k < NN; ++k) {
     int d[3] = {0, 0, 0}, t[3] = {0, 0, 0};
     bool ok = item_off != nullptr;
+    mg[k] = false;
 #pragma unroll
     for (int a = 0; a < DIM; ++a) {
       d[a] = ((k >> a) & 1) ? (loc[a] == E - 1 ? 1 : -1) : 0;
       const int c = bc[a] + d[a];
       t[a] = loc[a] + 1 - d[a] * E;
       ok = ok && c >= 0 && c < nb && t[a] < TL::TE;  // 2D: only 3 nodes reach past the edge
+      mg[k] = mg[k] || t[a] == 0 || t[a] == TL::TE - 1;
     }
     q[k] = TL::store_pos(t[0], t[1], t[2]);
     r0[k] = r1[k] = 0;
+    bf[k] = 0xFFFFFFFFu;
     if (ok) {
       const int32_t nbid = DIM == 3 ? ((bc[0] + d[0]) * nb + (bc[1] + d[1])) * nb + (bc[2] + d[2])
                                     : (bc[0] + d[0]) * nb + (bc[1] + d[1]);
       r0[k] = item_off[nbid];
       r1[k] = item_off[nbid + 1];
+      if (bm_read && mg[k]) bf[k] = bm_read[nbid];
     }
   }
+  // items whose margin was not written contribute nothing there
+#pragma unroll
+  for (int k = 0; k < NN; ++k)
+    if (!mg[k]) bf[k] = 0xFFFFFFFFu;
   bool mine = false;
 #pragma unroll
   for (int k = 0; k < NN; ++k) mine = mine || r1[k] > r0[k];
@@ -192,8 +210,8 @@ __device__ __forceinline__ void gather_block(const ParamsT<T> &P, const Collider
   for (int k = 0; k < NN; ++k) {
     c0[k].x = c0[k].y = c0[k].z = c0[k].w = (T)0;
     c1[k] = c0[k];
-    if (r1[k] > r0[k]) c0[k] = tiles[(int64_t)r0[k] * TN + q[k]];
-    if (r1[k] > r0[k] + 1) c1[k] = tiles[(int64_t)(r0[k] + 1) * TN + q[k]];
+    if (r1[k] > r0[k] && (bf[k] & 1u)) c0[k] = tiles[(int64_t)r0[k] * TN + q[k]];
+    if (r1[k] > r0[k] + 1 && (bf[k] & 2u)) c1[k] = tiles[(int64_t)(r0[k] + 1) * TN + q[k]];
   }
   vec4_t<T> sum;
   sum.x = sum.y = sum.z = sum.w = (T)0;
@@ -208,6 +226,8 @@ __device__ __forceinline__ void gather_block(const ParamsT<T> &P, const Collider
     sum.z += c1[k].z;
     sum.w += c1[k].w;
     for (int32_t it = r0[k] + 2; it < r1[k]; ++it) {
+      const int ib = it - r0[k];
+      if (!((bf[k] >> (ib < 31 ? ib : 31)) & 1u)) continue;
       const vec4_t<T> c = tiles[(int64_t)it * TN + q[k]];
       sum.x += c.x;
       sum.y += c.y;

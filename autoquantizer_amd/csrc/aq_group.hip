@@ -206,6 +206,8 @@ extern "C" int aq_group_create(aq_sim *const *sims, void *const *states, const i
       a.fflags = s->fflags;
       a.flist = nullptr;  // the group's gather visits every block
       a.fcnt = nullptr;
+      a.bmargin = nullptr;  // and reads whole tiles
+      a.item_off = s->item_off;
       std::memcpy(host.data() + od + (int64_t)k * (int64_t)sizeof(a), &a, sizeof(a));
       const GatherArgs<float> q = gather_args(s);
       std::memcpy(host.data() + og + (int64_t)k * (int64_t)sizeof(q), &q, sizeof(q));
@@ -290,7 +292,8 @@ extern "C" int aq_group_advance(aq_group *g, int nsteps, uint32_t step0, int sor
   if ((rc = group_sort(g, st))) return rc;
   for (size_t i = 0; i < g->sims.size(); ++i) {
     bool tiled;
-    if ((rc = launch_p2g(g->sims[i], g->states[i], g->ns[i], &tiled, st))) return rc;
+    // whole tiles: the group's gather reads them so
+    if ((rc = launch_p2g(g->sims[i], g->states[i], g->ns[i], &tiled, st, true))) return rc;
   }
   if ((rc = group_gather(g, st))) return rc;
   for (int t = 0; t + 1 < nsteps; ++t) {

@@ -396,7 +396,8 @@ __global__ void __launch_bounds__(256) k_gather_list(ParamsT<T> P, ColliderK K, 
                                                      vec4_t<T> *__restrict__ grid_v, uint32_t *__restrict__ act,
                                                      int32_t *__restrict__ glist, const int32_t *__restrict__ flist,
                                                      uint32_t *__restrict__ gcnt, uint32_t *__restrict__ listed,
-                                                     int32_t fslot) {
+                                                     int32_t fslot, const uint32_t *__restrict__ bm_read,
+                                                     uint32_t *__restrict__ bm_clear) {
   const uint32_t gc = gcnt[0], fc = gcnt[fslot];
   const uint32_t nw = gridDim.x * 4u;
   const int lane = threadIdx.x & 63;
@@ -415,7 +416,7 @@ __global__ void __launch_bounds__(256) k_gather_list(ParamsT<T> P, ColliderK K, 
       }
     }
     b = __builtin_amdgcn_readfirstlane(b);
-    gather_block<DIM, T, OP>(P, K, b, item_off, near, tiles, grid_f, fflags, grid_v, act);
+    gather_block<DIM, T, OP>(P, K, b, item_off, near, tiles, grid_f, fflags, grid_v, act, bm_read, bm_clear);
   }
 }
 
@@ -541,7 +542,12 @@ __global__ void __launch_bounds__(256) k_sort_keys(StateK s, CodecK cod, ParamsT
         }
       }
 #pragma unroll
-      for (int a = 0; a < DIM; ++a) x[a] = (T)decode_row(row, comp_at(cod, a));
+      for (int a = 0; a < DIM; ++a) {
+        if constexpr (IsF64<T>::value)
+          x[a] = decode_row_f64(row, comp_at(cod, a), cod.cd + 4 * a);  // f64 compute
+        else
+          x[a] = (T)decode_row(row, comp_at(cod, a));
+      }
     } else {
       const T *vals = (const T *)s.ptr;
 #pragma unroll
@@ -886,7 +892,7 @@ static void ws_plan(const aq_sim_desc *d, int64_t *offs /*[WS_N]*/, int64_t *tot
   int64_t sz[WS_N];
   sz[WS_GRID_V] = nodes * node_bytes;
   sz[WS_GRID_F] = nodes * node_bytes;
-  sz[WS_FLAGS] = nblocks * 4 * 2;  // fflags, act
+  sz[WS_FLAGS] = nblocks * 4 * 4;  // fflags, act, two drift-margin flag arrays
   sz[WS_GLIST] = 64 + nblocks * 4 * 3;  // counters; listed flags, block list, flagged-block list
   sz[WS_KEY] = cap * 4;
   sz[WS_RANK] = cap * 4;
@@ -981,6 +987,7 @@ extern "C" int aq_sim_create(const aq_sim_desc *d, void *ws, int64_t ws_bytes, a
   s->grid_f = base + offs[WS_GRID_F];
   s->fflags = (uint32_t *)(base + offs[WS_FLAGS]);
   s->act = s->fflags + s->nblocks;
+  s->bmargin = s->act + s->nblocks;
   s->gcnt = (uint32_t *)(base + offs[WS_GLIST]);
   s->listed = s->gcnt + 16;
   s->glist = (int32_t *)(s->listed + s->nblocks);
@@ -1160,9 +1167,10 @@ static int jit_source(const aq_sim &v, std::string &src) {
                 "extern \"C\" __global__ void __launch_bounds__(%d) __attribute__((amdgpu_waves_per_eu(%d)))\n"
                 "aq_jit_tiled(aq::StateK s, aq::CodecK cod, aq::ParamsT<float> P, int64_t n, const int4 *items,\n"
                 "             const int32_t *nitems, const float4 *grid_v, float4 *tiles, float4 *grid_f,\n"
-                "             uint32_t *fflags, int32_t *flist, uint32_t *gcnt, uint32_t ctl) {\n"
+                "             uint32_t *fflags, int32_t *flist, uint32_t *gcnt, uint32_t ctl, uint32_t *bmargin) {\n"
                 "  aq::tiled_body<%d, %d, AQ_STORE_QUANT, %d, float, true, true>(s, cod, P, n, items, nitems, grid_v,\n"
-                "                                                               tiles, grid_f, fflags, flist, gcnt, ctl);\n}\n",
+                "                                                               tiles, grid_f, fflags, flist, gcnt, ctl,\n"
+                "                                                               bmargin);\n}\n",
                 kItem, kJitWaves, v.d.dim, v.d.material, nw32_bucket(v.d.nwords));
   src += buf;
   return AQ_OK;
@@ -1320,6 +1328,13 @@ int settle_list(aq_sim *s, hipStream_t st) {
     s->commit_pending = 0;
     LAUNCH_CHECK();
   }
+  if (s->list_parity == 1) {
+    // the last gather read parity 0's drift-margin flags, which still hold
+    // the last tiled kernel's: cleared for the parity the next call starts with
+    const int64_t m = s->nblocks;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, s->bmargin, 0u, m);
+    LAUNCH_CHECK();
+  }
   s->list_parity = 0;
   return AQ_OK;
 }
@@ -1340,6 +1355,8 @@ int launch_gather(aq_sim *s, bool tiled, bool op, uint32_t step, hipStream_t st)
       s->commit_pending = 0;
     }
     const int32_t fslot = 1 + s->list_parity;
+    const uint32_t *bm_read = s->bmargin + (int64_t)s->list_parity * s->nblocks;
+    uint32_t *bm_clear = s->bmargin + (int64_t)(1 - s->list_parity) * s->nblocks;
     dispatch_grid(s, [&](auto tag_) {
       TAG_TYPES(tag_)
       (void)MAT;
@@ -1349,12 +1366,12 @@ int launch_gather(aq_sim *s, bool tiled, bool op, uint32_t step, hipStream_t st)
         hipLaunchKernelGGL((k_gather_list<DIM, T, true>), dim3(lblocks(s)), dim3(256), 0, st, params_of<T>(s), K,
                            ioff, s->near, (const vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags,
                            (vec4_t<T> *)s->grid_v, s->act, s->glist, (const int32_t *)s->flist, s->gcnt,
-                           s->listed, fslot);
+                           s->listed, fslot, bm_read, bm_clear);
       else
         hipLaunchKernelGGL((k_gather_list<DIM, T, false>), dim3(lblocks(s)), dim3(256), 0, st, params_of<T>(s), K,
                            ioff, s->near, (const vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags,
                            (vec4_t<T> *)s->grid_v, s->act, s->glist, (const int32_t *)s->flist, s->gcnt,
-                           s->listed, fslot);
+                           s->listed, fslot, bm_read, bm_clear);
     });
     s->commit_pending = (uint32_t)fslot;
     s->list_parity ^= 1;
@@ -1468,6 +1485,8 @@ extern "C" int aq_sort(aq_sim *s, void *state, int64_t n, int32_t *ids, void *st
   // the gather list of the new order (counters cleared: glist, both flist
   // counters, cursor)
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, s->gcnt, 0u, (int64_t)16);
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((2 * (int64_t)m + 255) / 256)), dim3(256), 0, st, s->bmargin, 0u,
+                     2 * (int64_t)m);  // drift-margin flags of both parities
   s->commit_pending = 0;
   s->list_parity = 0;
   hipLaunchKernelGGL(k_glist_build, dim3((m + 255) / 256), dim3(256), 0, st, (const uint32_t *)s->near,

@@ -5,18 +5,21 @@
 #include "aq_untiled.hpp"
 
 // P2G of `state` (tiled when the last sort describes it); *tiled tells which.
-int launch_p2g(aq_sim *s, const void *state, int64_t n, bool *tiled, hipStream_t st) {
+// whole: every tile written whole (for a gather that reads them so), else
+// drift margins only where the next list gather's flags say.
+int launch_p2g(aq_sim *s, const void *state, int64_t n, bool *tiled, hipStream_t st, bool whole) {
   *tiled = n > 0 && items_valid(s, state, n);
   if (n == 0) return AQ_OK;
   StateK sk = make_state(s, state, nullptr);
   CodecK cod = make_codec(s, 0u);
+  uint32_t *bm = whole ? nullptr : s->bmargin + (int64_t)s->list_parity * s->nblocks;  // the next gather's flags
   const uint32_t ctl = *tiled ? take_list_ctl(s) : 0u;
   dispatch(s, [&](auto tag_) {
     TAG_TYPES(tag_)
     if (*tiled)
       hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, false, true>), dim3(tblocks(s, n)), dim3(kItem), 0, st, sk,
                          cod, params_of<T>(s), n, s->items, s->item_off + s->nblocks, (const vec4_t<T> *)nullptr,
-                         (vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags, s->flist, s->gcnt, ctl);
+                         (vec4_t<T> *)s->tiles, (vec4_t<T> *)s->grid_f, s->fflags, s->flist, s->gcnt, ctl, bm);
     else
       hipLaunchKernelGGL((k_p2g<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, cod,
                          params_of<T>(s), n, (vec4_t<T> *)s->grid_f, s->fflags);
@@ -37,7 +40,7 @@ int launch_g2p(aq_sim *s, void *state, int64_t n, const int32_t *ids, uint32_t s
       hipLaunchKernelGGL((k_tiled<DIM, MAT, ST, NW32, T, true, false>), dim3(tblocks(s, n)), dim3(kItem), 0, st, sk,
                          cod, params_of<T>(s), n, s->items, s->item_off + s->nblocks,
                          (const vec4_t<T> *)s->grid_v, (vec4_t<T> *)nullptr, (vec4_t<T> *)nullptr,
-                         (uint32_t *)nullptr, (int32_t *)nullptr, s->gcnt, ctl);
+                         (uint32_t *)nullptr, (int32_t *)nullptr, s->gcnt, ctl, (uint32_t *)nullptr);
     else
       hipLaunchKernelGGL((k_g2p<DIM, MAT, ST, NW32, T>), dim3(pblocks(n)), dim3(256), 0, st, sk, cod,
                          params_of<T>(s), n, (const vec4_t<T> *)s->grid_v);

@@ -94,6 +94,10 @@ struct aq_sim {
   // After a list gather, commit_pending is the counter it consumed: the next
   // tiled kernel (or k_glist_commit) sets gcnt[0] = gcnt[3] and clears it.
   uint32_t *gcnt, *listed;
+  // per grid block, two arrays (parity of the gather that reads them): bit i
+  // set when the block's item i wrote its tile's drift margin (bit 31: items
+  // 31 and later, which always write it)
+  uint32_t *bmargin;
   int32_t list_parity = 0;     // parity of the next list gather
   uint32_t commit_pending = 0; // counter slot to clear at the next commit (0: none)
   int32_t *glist, *flist;
@@ -232,7 +236,7 @@ ColliderK collider_at(const aq_sim *s, uint32_t step);
 // replays on the state it was recorded from).
 int settle_list(aq_sim *s, hipStream_t st);
 // aq_particles.hip
-int launch_p2g(aq_sim *s, const void *state, int64_t n, bool *tiled, hipStream_t st);
+int launch_p2g(aq_sim *s, const void *state, int64_t n, bool *tiled, hipStream_t st, bool whole = false);
 int launch_g2p(aq_sim *s, void *state, int64_t n, const int32_t *ids, uint32_t step, hipStream_t st);
 // aq_fused.hip
 int launch_g2p2g(aq_sim *s, void *state, int64_t n, const int32_t *ids, uint32_t step, bool *tiled,

@@ -89,6 +89,9 @@ template <int DIM> struct Tile {
 // LDS slot of the node at each position of the tile buffer (the inverse of
 // Tile::store_pos), built at compile time: the write-out walks the buffer
 // in order, so each wavefront stores a contiguous run.
+// Bit 15 marks the drift margin: nodes with t = 0 or TE - 1 on some axis,
+// which only particles that left their sort cell reach.
+constexpr uint16_t kMarginBit = 0x8000u;
 template <int DIM> struct TileOrder {
   uint16_t slot[Tile<DIM>::TN];
 };
@@ -97,7 +100,11 @@ template <int DIM> constexpr TileOrder<DIM> make_tile_order() {
   TileOrder<DIM> o{};
   for (int x = 0; x < TL::TE; ++x)
     for (int y = 0; y < TL::TE; ++y)
-      for (int z = 0; z < (DIM == 3 ? TL::TE : 1); ++z) o.slot[TL::store_pos(x, y, z)] = (uint16_t)TL::slot(x, y, z);
+      for (int z = 0; z < (DIM == 3 ? TL::TE : 1); ++z) {
+        const bool margin = x == 0 || x == TL::TE - 1 || y == 0 || y == TL::TE - 1 ||
+                            (DIM == 3 && (z == 0 || z == TL::TE - 1));
+        o.slot[TL::store_pos(x, y, z)] = (uint16_t)(TL::slot(x, y, z) | (margin ? kMarginBit : 0u));
+      }
   return o;
 }
 __constant__ constexpr TileOrder<3> kTileOrder3 = make_tile_order<3>();
@@ -241,6 +248,11 @@ template <class T> struct TiledArgs {
   uint32_t *fflags;
   int32_t *flist;  // blocks newly flagged by untiled scatters (null: flags only)
   uint32_t *fcnt;
+  // per grid block, bit i: the block's item i (bit 31: items 31 and later,
+  // which always write it) wrote its tile's drift margin; null: every item
+  // writes its whole tile
+  uint32_t *bmargin;
+  const int32_t *item_off;  // first item of each block (with bmargin)
 };
 
 // Work of one simulation: its `count` items.
@@ -301,6 +313,7 @@ __device__ __forceinline__ void tiled_run(const Src &src) {
   __shared__ unsigned long long ts[DO_P2G ? NP : 1][DO_P2G ? TS : 1];
   constexpr int NWAVES = kItem / 64;
   __shared__ T bpart[NWAVES][4];
+  __shared__ uint32_t bdrift[NWAVES];
   __shared__ uint32_t rows[kItem * RowsOf<ST, NW32>::RS];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   constexpr bool G2P_ONLY_LOAD = DO_G2P;  // G2P needs only the words of x and J/F
@@ -383,6 +396,7 @@ __device__ __forceinline__ void tiled_run(const Src &src) {
     bool inside = false;
     if constexpr (DO_P2G) {
       T B[4] = {(T)0, (T)0, (T)0, (T)0};
+      bool drift = false;  // the stencil reaches the tile's drift margin
       if (has) {
         S = stencil_of<DIM, MAT, T>(out, P);
         R = p2g_terms<DIM, MAT, T>(out, P, S);
@@ -394,19 +408,32 @@ __device__ __forceinline__ void tiled_run(const Src &src) {
           // a weight negative, and the packed mass sums need them >= 0: such a
           // particle scatters through the float-atomic path
           inside = inside && rel[a] >= 0 && rel[a] + 2 < TE && S.fx[a] >= (T)0.5 && S.fx[a] <= (T)1.5;
+          drift = drift || rel[a] == 0 || rel[a] == TE - 3;
         }
         if (inside) p2g_bounds<DIM, T>(R, S, B);
+        drift = drift && inside;
       }
 #pragma unroll
       for (int c = 0; c < 4; ++c) B[c] = wave_sum<T>(B[c]);
+      const bool wdrift = __ballot(drift) != 0ull;
       if (lane == 0) {
 #pragma unroll
         for (int c = 0; c < 4; ++c) bpart[wave][c] = B[c];
+        bdrift[wave] = wdrift ? 1u : 0u;
       }
     }
     __syncthreads();  // item bounds complete; G2P done with the tile
     int k[4] = {0, 0, 0, 0};
+    bool margin = true;  // the write-out covers the drift margin
     if constexpr (DO_P2G) {
+      if (A.bmargin) {
+        uint32_t d = 0u;
+#pragma unroll
+        for (int w = 0; w < NWAVES; ++w) d |= bdrift[w];
+        const int ib = li - A.item_off[item.x];  // the item's number in its block
+        margin = d != 0u || ib >= 31;
+        if (margin && tid == 0) atomicOr(A.bmargin + item.x, 1u << (ib < 31 ? ib : 31));
+      }
       bool finite = true;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
@@ -445,7 +472,11 @@ __device__ __forceinline__ void tiled_run(const Src &src) {
       __syncthreads();  // tile sums complete
       vec4_t<T> *dst = A.tiles + (int64_t)li * TN;
       for (int nd = tid; nd < TN; nd += kItem) {  // nd: position in the buffer
-        const int q = wsl[nd];
+        const int ws = wsl[nd];
+        // without drift the margin's LDS sums are zero (never added to, so
+        // they need no clearing either) and the gather does not read them
+        if (!margin && (ws & kMarginBit)) continue;
+        const int q = ws & (kMarginBit - 1);
         vec4_t<T> val;
         if constexpr (NP == 2) {
           int32_t mx, m, my, mz;
@@ -475,7 +506,7 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
                                            const int4 *__restrict__ items, const int32_t *__restrict__ nitems,
                                            const vec4_t<T> *__restrict__ grid_v, vec4_t<T> *__restrict__ tiles,
                                            vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags,
-                                           int32_t *flist, uint32_t *gcnt, uint32_t ctl) {
+                                           int32_t *flist, uint32_t *gcnt, uint32_t ctl, uint32_t *bmargin) {
   // commit of the last list gather's appends (no thread of this kernel
   // reads gcnt[0] or the cleared counter; appends go to the other one)
   const uint32_t commit = (ctl >> 2) & 3u;
@@ -484,7 +515,9 @@ __device__ __forceinline__ void tiled_body(StateK s, CodecK cod, ParamsT<T> P, i
     gcnt[commit] = 0u;
   }
   uint32_t *fcnt = flist ? gcnt + (ctl & 3u) : nullptr;
-  const OneSim<T> src{{s, cod, P, n, items, grid_v, tiles, grid_f, fflags, flist, fcnt}, *nitems};
+  const int32_t nblocks = DIM == 3 ? P.nb * P.nb * P.nb : P.nb * P.nb;
+  const OneSim<T> src{{s, cod, P, n, items, grid_v, tiles, grid_f, fflags, flist, fcnt, bmargin, nitems - nblocks},
+                      *nitems};
   tiled_run<DIM, MAT, ST, NW32, T, DO_G2P, DO_P2G>(src);
 }
 
@@ -493,9 +526,9 @@ __global__ void __launch_bounds__(kItem) __attribute__((amdgpu_waves_per_eu(Tile
 k_tiled(StateK s, CodecK cod, ParamsT<T> P, int64_t n, const int4 *__restrict__ items,
         const int32_t *__restrict__ nitems, const vec4_t<T> *__restrict__ grid_v, vec4_t<T> *__restrict__ tiles,
         vec4_t<T> *__restrict__ grid_f, uint32_t *__restrict__ fflags, int32_t *flist, uint32_t *gcnt,
-        uint32_t ctl) {
+        uint32_t ctl, uint32_t *bmargin) {
   tiled_body<DIM, MAT, ST, NW32, T, DO_G2P, DO_P2G>(s, cod, P, n, items, nitems, grid_v, tiles, grid_f, fflags,
-                                                    flist, gcnt, ctl);
+                                                    flist, gcnt, ctl, bmargin);
 }
 
 // The fused G2P -> P2G substep of a group of quantized simulations (one
