@@ -553,3 +553,63 @@ def test_wrap_counter(orc):
     assert want[3:6].sum() > 0  # the scene does overflow v
     np.testing.assert_array_equal(got, want)
     sim_q.count_wraps(False)
+
+
+def test_nan_inf_injection_flagged():
+    """Failure detection from the host: NaN and +Inf injected into two grid
+    nodes that hold mass reach the particles of their stencils in the next
+    G2P.  The range-violation counter of the quantized simulation counts, per
+    fixed-point member, exactly the stores whose value was not finite (the
+    same G2P on fp32 storage with the same grid shows which), and the limits
+    detector of the fp32 simulation reports an infinite v range."""
+    _torch()
+    import autoquantizer_amd as aq
+    from autoquantizer_amd import layouts
+    n = 3000
+    names, t, g, sh = layouts.fluid_3d(v_range=8.0)
+    sims = []
+    for storage in ("quant", "f32"):
+        cfg = aq.MPMConfig()
+        cfg.dim, cfg.use_fluid, cfg.n_grid, cfg.dt, cfg.la = 3, True, 64, 2e-4, 400.0
+        cfg.p_vol = (1 / 64 * 0.5) ** 3
+        cfg.n_particles = n
+        cfg.seed = 23
+        cfg.compute_fp = "f32"
+        sims.append(aq.MPMSolver(cfg, types=t, groups=g, shared=sh) if storage == "quant" else aq.MPMSolver(cfg))
+    sim_q, sim_f = sims
+    rng = np.random.default_rng(41)
+    vals = np.zeros((16, n))
+    vals[:3] = rng.uniform(0.3, 0.5, (3, n))
+    vals[3:6] = rng.uniform(-0.5, 0.5, (3, n))
+    vals[15] = 1.0
+    sim_q.set_state(vals)
+    sim_f.set_state(sim_q.get_state(original_order=True))
+    sim_q.build_pid()
+    sim_f.build_pid()
+    sim_q.p2g_bls()
+    sim_q.grid_op()
+    grid = sim_q.grid_velocity()
+    heavy = np.flatnonzero(grid[:, 3] > 0)
+    assert heavy.size > 100
+    grid[heavy[heavy.size // 3], 0] = np.nan
+    grid[heavy[2 * heavy.size // 3], 1] = np.inf
+    sim_q.load_grid(grid)
+    sim_f.load_grid(grid)
+    print(f"[inject] NaN at node {heavy[heavy.size // 3]}, +Inf at node {heavy[2 * heavy.size // 3]}", flush=True)
+    sim_q.count_wraps()
+    sim_q.g2p_bls()
+    sim_f.g2p_bls()
+    got = sim_q.wrap_counts()
+    sim_q.count_wraps(False)
+    out = sim_f.get_state(original_order=True)
+    want = np.zeros(16, dtype=np.int64)
+    for c, cc in enumerate(sim_q.comps):
+        if cc["kind"] == 0:  # fixed point: x, v, J (the scene keeps every finite value in range)
+            want[c] = int(np.sum(~np.isfinite(out[c])))
+    print("non-finite stores per component:", got.tolist(), flush=True)
+    assert want[3:6].sum() > 0 and want[:3].sum() > 0
+    np.testing.assert_array_equal(got, want)
+    sim_f.init_limits()
+    sim_f.detect_bound_kernel()
+    lim = sim_f.detect_bound()
+    assert np.isinf(lim[4]), lim  # +Inf reached v_y
