@@ -1163,16 +1163,21 @@ static int jit_source(const aq_sim &v, std::string &src) {
                 "#define AQ_JIT_NW %d\n#define AQ_JIT_G2P_MASK %uu\n#include \"aq_tiled.hpp\"\n",
                 c.shared_mask, c.gfirst_mask, c.glast_mask, v.lay.nw, k.g2p_mask);
   src += buf;
-  std::snprintf(buf, sizeof buf,
-                "extern \"C\" __global__ void __launch_bounds__(%d) __attribute__((amdgpu_waves_per_eu(%d)))\n"
-                "aq_jit_tiled(aq::StateK s, aq::CodecK cod, aq::ParamsT<float> P, int64_t n, const int4 *items,\n"
-                "             const int32_t *nitems, const float4 *grid_v, float4 *tiles, float4 *grid_f,\n"
-                "             uint32_t *fflags, int32_t *flist, uint32_t *gcnt, uint32_t ctl, uint32_t *bmargin) {\n"
-                "  aq::tiled_body<%d, %d, AQ_STORE_QUANT, %d, float, true, true>(s, cod, P, n, items, nitems, grid_v,\n"
-                "                                                               tiles, grid_f, fflags, flist, gcnt, ctl,\n"
-                "                                                               bmargin);\n}\n",
-                kItem, kJitWaves, v.d.dim, v.d.material, nw32_bucket(v.d.nwords));
-  src += buf;
+  // the fused kernel, and the P2G-only and G2P-only kernels that open and
+  // close a run of substeps
+  static const char *const kinds[3][3] = {
+      {"aq_jit_tiled", "true", "true"}, {"aq_jit_p2g", "false", "true"}, {"aq_jit_g2p", "true", "false"}};
+  for (const auto &kd : kinds) {
+    std::snprintf(buf, sizeof buf,
+                  "extern \"C\" __global__ void __launch_bounds__(%d) __attribute__((amdgpu_waves_per_eu(%d)))\n"
+                  "%s(aq::StateK s, aq::CodecK cod, aq::ParamsT<float> P, int64_t n, const int4 *items,\n"
+                  "   const int32_t *nitems, const float4 *grid_v, float4 *tiles, float4 *grid_f,\n"
+                  "   uint32_t *fflags, int32_t *flist, uint32_t *gcnt, uint32_t ctl, uint32_t *bmargin) {\n"
+                  "  aq::tiled_body<%d, %d, AQ_STORE_QUANT, %d, float, %s, %s>(s, cod, P, n, items, nitems, grid_v,\n"
+                  "      tiles, grid_f, fflags, flist, gcnt, ctl, bmargin);\n}\n",
+                  kItem, kJitWaves, kd[0], v.d.dim, v.d.material, nw32_bucket(v.d.nwords), kd[1], kd[2]);
+    src += buf;
+  }
   return AQ_OK;
 }
 
@@ -1275,7 +1280,7 @@ extern "C" int aq_sim_specialize(aq_sim *s, const char *src_dir, const char *inc
   if (s->jit_mod) {
     (void)hipModuleUnload(s->jit_mod);
     s->jit_mod = nullptr;
-    s->jit_fn = nullptr;
+    s->jit_fn = s->jit_fn_p2g = s->jit_fn_g2p = nullptr;
   }
   if (!src_dir) return AQ_OK;  // back to the generic kernel
   if (s->d.storage != AQ_STORE_QUANT) return fail(AQ_ERR_INVALID, "specialisation applies to quantized storage");
@@ -1285,14 +1290,19 @@ extern "C" int aq_sim_specialize(aq_sim *s, const char *src_dir, const char *inc
   if (rc) return rc;
   hipModule_t mod;
   HIP_TRY(hipModuleLoadData(&mod, code.data()));
-  hipFunction_t fn;
-  const hipError_t e = hipModuleGetFunction(&fn, mod, "aq_jit_tiled");
-  if (e != hipSuccess) {
-    (void)hipModuleUnload(mod);
-    return fail(AQ_ERR_HIP, hipGetErrorString(e));
+  hipFunction_t fn[3];
+  const char *const names[3] = {"aq_jit_tiled", "aq_jit_p2g", "aq_jit_g2p"};
+  for (int i = 0; i < 3; ++i) {
+    const hipError_t e = hipModuleGetFunction(&fn[i], mod, names[i]);
+    if (e != hipSuccess) {
+      (void)hipModuleUnload(mod);
+      return fail(AQ_ERR_HIP, hipGetErrorString(e));
+    }
   }
   s->jit_mod = mod;
-  s->jit_fn = fn;
+  s->jit_fn = fn[0];
+  s->jit_fn_p2g = fn[1];
+  s->jit_fn_g2p = fn[2];
   return AQ_OK;
 }
 

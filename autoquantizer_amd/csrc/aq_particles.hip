@@ -13,7 +13,20 @@ int launch_p2g(aq_sim *s, const void *state, int64_t n, bool *tiled, hipStream_t
   StateK sk = make_state(s, state, nullptr);
   CodecK cod = make_codec(s, 0u);
   uint32_t *bm = whole ? nullptr : s->bmargin + (int64_t)s->list_parity * s->nblocks;  // the next gather's flags
-  const uint32_t ctl = *tiled ? take_list_ctl(s) : 0u;
+  uint32_t ctl = *tiled ? take_list_ctl(s) : 0u;
+  if (*tiled && s->jit_fn_p2g) {
+    ParamsT<float> P = s->pf;
+    const int4 *items = s->items;
+    const int32_t *nitems = s->item_off + s->nblocks;
+    const void *gv = nullptr;
+    void *tl = s->tiles, *gf = s->grid_f;
+    uint32_t *ff = s->fflags;
+    int32_t *fl = s->flist;
+    uint32_t *gc = s->gcnt;
+    void *args[] = {&sk, &cod, &P, &n, &items, &nitems, &gv, &tl, &gf, &ff, &fl, &gc, &ctl, &bm};
+    HIP_TRY(hipModuleLaunchKernel(s->jit_fn_p2g, tblocks(s, n), 1, 1, kItem, 1, 1, 0, st, args, nullptr));
+    return AQ_OK;
+  }
   dispatch(s, [&](auto tag_) {
     TAG_TYPES(tag_)
     if (*tiled)
@@ -33,7 +46,20 @@ int launch_g2p(aq_sim *s, void *state, int64_t n, const int32_t *ids, uint32_t s
   StateK sk = make_state(s, state, ids);
   CodecK cod = make_codec(s, step);
   const bool tiled = items_valid(s, state, n);
-  const uint32_t ctl = tiled ? take_list_ctl(s) : 0u;  // G2P only: carries a commit, appends nothing
+  uint32_t ctl = tiled ? take_list_ctl(s) : 0u;  // G2P only: carries a commit, appends nothing
+  if (tiled && s->jit_fn_g2p) {
+    ParamsT<float> P = s->pf;
+    const int4 *items = s->items;
+    const int32_t *nitems = s->item_off + s->nblocks;
+    const void *gv = s->grid_v;
+    void *tl = nullptr, *gf = nullptr;
+    uint32_t *ff = nullptr, *bm = nullptr;
+    int32_t *fl = nullptr;
+    uint32_t *gc = s->gcnt;
+    void *args[] = {&sk, &cod, &P, &n, &items, &nitems, &gv, &tl, &gf, &ff, &fl, &gc, &ctl, &bm};
+    HIP_TRY(hipModuleLaunchKernel(s->jit_fn_g2p, tblocks(s, n), 1, 1, kItem, 1, 1, 0, st, args, nullptr));
+    return AQ_OK;
+  }
   dispatch(s, [&](auto tag_) {
     TAG_TYPES(tag_)
     if (tiled)

@@ -116,7 +116,9 @@ struct aq_sim {
   bool stream_order;  // the encoder's field order has non-decreasing offsets
   int64_t ws_bytes;
   hipModule_t jit_mod = nullptr;      // fused kernel specialised to the layout (aq_sim_specialize)
-  hipFunction_t jit_fn = nullptr;
+  hipFunction_t jit_fn = nullptr;      // fused G2P -> P2G
+  hipFunction_t jit_fn_p2g = nullptr;  // P2G only (the first substep of a run)
+  hipFunction_t jit_fn_g2p = nullptr;  // G2P only (the last substep of a run)
   unsigned long long *wraps = nullptr;  // range-violation counters (aq_sim_count_wraps)
   bool has_collider = false;            // aq_sim_set_collider
   aq_collider col{};

@@ -5,6 +5,7 @@ elastic record and a record whose fields are out of component order; the
 code object holds the kernel, uses no scratch memory and no packed-FP32
 instructions; the cache returns the same object; plain storage is refused."""
 import os
+import re
 import shutil
 import subprocess
 
@@ -62,8 +63,10 @@ def test_code_object(built, kind):
         pytest.skip("llvm tools not installed")
     notes = subprocess.run([os.path.join(LLVM, "llvm-readelf"), "--notes", path], capture_output=True,
                            text=True).stdout
-    assert "aq_jit_tiled" in notes
-    assert ".private_segment_fixed_size: 0" in notes
+    for name in ("aq_jit_tiled", "aq_jit_p2g", "aq_jit_g2p"):  # fused, P2G only, G2P only
+        assert name in notes
+    sizes = re.findall(r"\.private_segment_fixed_size:\s*(\d+)", notes)
+    assert len(sizes) == 3 and all(v == "0" for v in sizes), sizes  # no register spills
     dis = subprocess.run([os.path.join(LLVM, "llvm-objdump"), "-d", path], capture_output=True, text=True).stdout
     assert "ds_add_u64" in dis
     for op in ("v_pk_add_f32", "v_pk_mul_f32", "v_pk_fma_f32"):
