@@ -130,7 +130,11 @@ template <class T> struct GatherArgs {
 // do); the margin nodes of the other items were not written and are not
 // read.  bm_clear (null: none): the flags the next tiled kernel sets,
 // cleared here for block b (every block with items is gathered).
-template <int DIM, class T, bool OP>
+// SPEC: the neighbours' item ranges are loaded with the block's flags
+// instead of after them (one round trip less for a block that has items
+// near it, wasted loads for one that has not: for gathers over lists of
+// blocks near particles).
+template <int DIM, class T, bool OP, bool SPEC = false>
 __device__ __forceinline__ void gather_block(const ParamsT<T> &P, const ColliderK &K, int32_t b,
                                              const int32_t *__restrict__ item_off,
                                              const uint32_t *__restrict__ near, const vec4_t<T> *__restrict__ tiles,
@@ -144,8 +148,9 @@ __device__ __forceinline__ void gather_block(const ParamsT<T> &P, const Collider
   if (bm_clear && lane == 0) bm_clear[b] = 0u;
   const bool fall = fflags[b] != 0u;
   // a block with no items in its neighbourhood and no untiled scatter has
-  // nothing to gather: skip the item ranges of its neighbours
-  if (item_off && !near[b] && !fall) {
+  // nothing to gather: skip the item ranges of its neighbours (with SPEC
+  // their empty ranges lead to the same exit below)
+  if (!SPEC && item_off && !near[b] && !fall) {
     if (act[b]) {
       vec4_t<T> z;
       z.x = z.y = z.z = z.w = (T)0;

@@ -401,11 +401,15 @@ __global__ void __launch_bounds__(256) k_gather_list(ParamsT<T> P, ColliderK K, 
   const uint32_t gc = gcnt[0], fc = gcnt[fslot];
   const uint32_t nw = gridDim.x * 4u;
   const int lane = threadIdx.x & 63;
-  for (uint32_t i = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * 4u + (threadIdx.x >> 6)));
-       i < gc + fc; i += nw) {
+  const uint32_t cap = (uint32_t)(DIM == 3 ? P.nb * P.nb * P.nb : P.nb * P.nb);  // glist's capacity
+  uint32_t i = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * 4u + (threadIdx.x >> 6)));
+  // the wave's first list entry, loaded with the counters (used only when
+  // it is below gcnt[0]: slots above may be taking appends meanwhile)
+  int32_t pre = i < cap ? glist[i] : 0;
+  for (; i < gc + fc; i += nw) {
     int32_t b;
     if (i < gc) {
-      b = glist[i];
+      b = pre;
     } else {
       b = flist[i - gc];
       if (listed[b]) continue;  // glist holds it
@@ -416,7 +420,8 @@ __global__ void __launch_bounds__(256) k_gather_list(ParamsT<T> P, ColliderK K, 
       }
     }
     b = __builtin_amdgcn_readfirstlane(b);
-    gather_block<DIM, T, OP>(P, K, b, item_off, near, tiles, grid_f, fflags, grid_v, act, bm_read, bm_clear);
+    pre = i + nw < cap ? glist[i + nw] : 0;
+    gather_block<DIM, T, OP, true>(P, K, b, item_off, near, tiles, grid_f, fflags, grid_v, act, bm_read, bm_clear);
   }
 }
 
