@@ -192,11 +192,12 @@ int aq_sim_unpack(aq_sim *sim, const uint64_t *words, int64_t n, void *vals, voi
 /* Run-time specialisation to the simulation's record layout (replaces the
    per-scheme kernel compilation of the reference's Taichi JIT,
    TS/taichi/codegen/codegen_llvm_quant.cpp: kernels are generated for each
-   quantization scheme).  The fused G2P/P2G kernel of aq_advance is compiled
-   with hipRTC from the kernel headers in src_dir (autoquantizer_amd/csrc)
-   and include_dir (include/) with the layout's descriptors as constants,
-   and used by every later tiled aq_advance / aq_advance_timed of `sim`
-   (results identical to the generic kernel).  Code objects are cached in
+   quantization scheme).  The fused G2P/P2G kernel of aq_advance and the
+   P2G-only and G2P-only kernels that open and close a run of substeps (also
+   those of aq_p2g / aq_g2p) are compiled with hipRTC from the kernel headers
+   in src_dir (autoquantizer_amd/csrc) and include_dir (include/) with the
+   layout's descriptors as constants, and used by every later tiled launch
+   of `sim` (results identical to the generic kernels).  Code objects are cached in
    cache_dir (NULL: compile, no cache).  src_dir NULL returns to the generic
    kernel.  Quantized storage only. */
 int aq_sim_specialize(aq_sim *sim, const char *src_dir, const char *include_dir, const char *cache_dir);
