@@ -404,8 +404,9 @@ __global__ void __launch_bounds__(256) k_gather_list(ParamsT<T> P, ColliderK K, 
   const uint32_t cap = (uint32_t)(DIM == 3 ? P.nb * P.nb * P.nb : P.nb * P.nb);  // glist's capacity
   uint32_t i = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(blockIdx.x * 4u + (threadIdx.x >> 6)));
   // the wave's first list entry, loaded with the counters (used only when
-  // it is below gcnt[0]: slots above may be taking appends meanwhile)
-  int32_t pre = i < cap ? glist[i] : 0;
+  // it is below gcnt[0]: slots above may be taking appends meanwhile, so
+  // these loads and the appends' stores are relaxed atomics)
+  int32_t pre = i < cap ? __hip_atomic_load(glist + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
   for (; i < gc + fc; i += nw) {
     int32_t b;
     if (i < gc) {
@@ -416,11 +417,11 @@ __global__ void __launch_bounds__(256) k_gather_list(ParamsT<T> P, ColliderK K, 
       __builtin_amdgcn_wave_barrier();
       if (lane == 0) {
         listed[b] = 1u;
-        glist[atomicAdd(&gcnt[3], 1u)] = b;
+        __hip_atomic_store(glist + atomicAdd(&gcnt[3], 1u), b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     b = __builtin_amdgcn_readfirstlane(b);
-    pre = i + nw < cap ? glist[i + nw] : 0;
+    pre = i + nw < cap ? __hip_atomic_load(glist + i + nw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
     gather_block<DIM, T, OP, true>(P, K, b, item_off, near, tiles, grid_f, fflags, grid_v, act, bm_read, bm_clear);
   }
 }
@@ -492,11 +493,17 @@ __global__ void __launch_bounds__(256) k_items_fill(const uint32_t *__restrict__
 
 // near[b]: some block within one block of b (per axis) holds particles, so
 // the gather of b may find tiles (the neighbours whose tiles can cover b).
+// Also clears, for the new order, b's drift-margin masks of both parities
+// (bm[b], bm[m + b]) and the gather list's 16 counters (gcnt).
 template <int DIM>
 __global__ void __launch_bounds__(256) k_near(const uint32_t *__restrict__ counts, int32_t nb,
-                                              uint32_t *__restrict__ near, int32_t m) {
+                                              uint32_t *__restrict__ near, uint32_t *__restrict__ bm,
+                                              uint32_t *__restrict__ gcnt, int32_t m) {
   const int32_t b = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (b < 16) gcnt[b] = 0u;
   if (b >= m) return;
+  bm[b] = 0u;
+  bm[m + b] = 0u;
   int bc[3];
   block_coords<DIM>(b, nb, bc);
   uint32_t any = 0u;
@@ -1493,15 +1500,14 @@ extern "C" int aq_sort(aq_sim *s, void *state, int64_t n, int32_t *ids, void *st
   // work items: up to kItem particles of one block each
   hipLaunchKernelGGL(k_items_fill, dim3((m + 255) / 256), dim3(256), 0, st, s->scount, s->soff, s->item_off,
                      s->items, m);
+  // near flags; the gather list's counters (glist, both flist counters,
+  // cursor) and the drift-margin masks of both parities cleared
   if (s->d.dim == 3)
-    hipLaunchKernelGGL(k_near<3>, dim3((m + 255) / 256), dim3(256), 0, st, s->scount, (int32_t)s->pf.nb, s->near, m);
+    hipLaunchKernelGGL(k_near<3>, dim3((m + 255) / 256), dim3(256), 0, st, s->scount, (int32_t)s->pf.nb, s->near,
+                       s->bmargin, s->gcnt, m);
   else
-    hipLaunchKernelGGL(k_near<2>, dim3((m + 255) / 256), dim3(256), 0, st, s->scount, (int32_t)s->pf.nb, s->near, m);
-  // the gather list of the new order (counters cleared: glist, both flist
-  // counters, cursor)
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, s->gcnt, 0u, (int64_t)16);
-  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((2 * (int64_t)m + 255) / 256)), dim3(256), 0, st, s->bmargin, 0u,
-                     2 * (int64_t)m);  // drift-margin flags of both parities
+    hipLaunchKernelGGL(k_near<2>, dim3((m + 255) / 256), dim3(256), 0, st, s->scount, (int32_t)s->pf.nb, s->near,
+                       s->bmargin, s->gcnt, m);
   s->commit_pending = 0;
   s->list_parity = 0;
   hipLaunchKernelGGL(k_glist_build, dim3((m + 255) / 256), dim3(256), 0, st, (const uint32_t *)s->near,

@@ -225,9 +225,11 @@ int aq_g2p(aq_sim *sim, void *state, int64_t n, const int32_t *ids, uint32_t ste
 /* nsteps substeps with G2P of substep t fused with P2G of substep t+1
    (one launch reads the state once, writes it once and scatters the next
    grid).  Same results as nsteps aq_substep calls up to float rounding
-   (accumulation order).  Particles are re-sorted by grid block every
-   `sort_every` substeps (0 = never: global-memory kernels); ids are
-   permuted along. */
+   (accumulation order).  Particles are re-sorted by grid block at the
+   call's start and then every `sort_every` substeps (0 = never:
+   global-memory kernels), so every call with sort_every > 0 costs at least
+   one sort: prefer few long calls to many short ones.  ids are permuted
+   along. */
 int aq_advance(aq_sim *sim, void *state, int64_t n, int nsteps, uint32_t step0,
                int sort_every, int32_t *ids, void *stream);
 
