@@ -13,7 +13,6 @@ import subprocess
 import pytest
 
 ORACLE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
-EXE = os.path.join(ORACLE, "_san", "sanitize_check")
 
 
 def _env():
@@ -25,14 +24,17 @@ def _env():
 
 
 @pytest.fixture(scope="module")
-def sanitized():
+def sanitized(tmp_path_factory):
     if shutil.which("gcc") is None or shutil.which("make") is None:
         pytest.skip("gcc or make not available")
-    r = subprocess.run(["make", "-s", "-C", ORACLE, "sanitize"], capture_output=True, text=True, timeout=600)
+    # built into this process's own directory: parallel test workers never share the file
+    exe = str(tmp_path_factory.mktemp("san") / "sanitize_check")
+    r = subprocess.run(["make", "-s", "-C", ORACLE, "sanitize", f"SAN_EXE={exe}"], capture_output=True, text=True,
+                       timeout=600)
     if r.returncode != 0 and "cannot find" in (r.stderr + r.stdout) and "asan" in (r.stderr + r.stdout):
         pytest.skip("the sanitizer runtimes are not installed")
     assert r.returncode == 0, r.stderr[-2000:]
-    return EXE
+    return exe
 
 
 def test_oracle_is_clean_under_sanitizers(sanitized):
